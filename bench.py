@@ -1,0 +1,169 @@
+"""Benchmark: trace-sampling propagation throughput on one MI355X.
+
+Workload (one "step" = one dt step of all M trace vectors, i.e. one Krylov
+rebuild at integrator_tolerance = 0, with the populations of the new state):
+  e1 = 2x14N + 6x1H, e2 = 4x1H  ->  N = 36 864;  full anisotropic hyperfine
+  tensors; B = 0.05 mT at 37 deg; J = 0.01 mT; kS = 2, kT = 0.5 us^-1;
+  M = 256 random-phase trace vectors; Krylov order m = 24; dt = 1 ns; complex128.
+
+Prints one JSON line.  value = N * M * steps / wall time of the timed steps
+(inputs resident on the device).  roofline: the kernel family with the most
+device time in the timed region, algorithmic bytes / HIP-event time.
+cpu_baseline: the CPU oracle (OpenMP over trace vectors, the reference's
+algorithm) on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("state·steps/s (dim×M×steps) at 1e-10 yield err, %HBM roofline; grad/fwd time ratio")
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, target_s: float):
+    """The oracle on n_threads trace vectors (one per thread, as the reference's
+    OpenMP loop), for as many steps as fit in about target_s seconds."""
+    from oracle import pyoracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_system
+    s = oracle_system(w.sys, w.B)
+    h = pyoracle.build_general_csr(s)
+    nt = _cpu_threads()
+    dt = w.sim.dt
+
+    def run(steps):
+        t_end = steps * dt
+        taken = pyoracle.steps_taken(t_end, dt)
+        t0 = time.perf_counter()
+        pyoracle.trace_sampling(h, s.Z, M=nt, m=w.sim.N_krylov, t_end=t_end, dt=dt, tol=0.0, vec_kind=1,
+                                seed=w.sim.seed, n_threads=nt)
+        return taken, time.perf_counter() - t0
+
+    k1, t1 = run(1)
+    steps = max(1, int(target_s / max(t1, 1e-3)))
+    k, t = run(steps) if steps > 1 else (k1, t1)
+    return {"value": w.N * nt * k / t, "unit": "state·steps/s", "cores": nt, "kind": "port",
+            "sample": f"CPU oracle (C, OpenMP, one trace vector per thread): N={w.N}, {nt} vectors x {k} steps, "
+                      f"m={w.sim.N_krylov}, MGS Arnoldi + Pade expm + convert_to_krylov, {t:.1f} s on {_cpu_model()}"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--M", type=int, default=256)
+    ap.add_argument("--krylov", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-profile", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    from spin_dynamics_amd import _lib, dynamics, workloads
+    ctx = _lib.Context(0)
+    w = workloads.config2(M=a.M, m=a.krylov)
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    lib = ctx.lib
+    N, M, m, dt = H.N, a.M, a.krylov, w.sim.dt
+
+    def run(steps):
+        sim = _lib.SimParams(M, m, steps * dt, dt, 0.0, 0, _lib.VEC_PHASE)
+        n_out = int(lib.spc_n_steps(sim.t_end, dt)) + 1
+        outs = [np.zeros(n_out) for _ in range(4)]
+        seed = np.array(w.sim.seed, dtype=np.int64)
+        ctx.check(lib.spc_trace_sample(ctx.h, sim, _lib.ptr(seed, _lib._i64p), None, *[_lib.ptr(o) for o in outs],
+                                       n_out, None, None))
+        _, rebuilds, taken = dynamics.last_run_stats(ctx)
+        return taken, outs
+
+    # warmup (allocations, code objects), then exactly K timed steps
+    run(max(a.warmup, 1))
+    t0 = time.perf_counter()
+    taken, outs = run(a.steps)
+    wall = time.perf_counter() - t0
+    dev_ms, _, _ = dynamics.last_run_stats(ctx)
+    value = N * M * taken / wall
+    ms_per_step = 1000.0 * wall / taken
+
+    roofline = None
+    families = {}
+    if not a.no_profile:
+        lib.spc_profile_enable(ctx.h, 1)
+        run(a.steps)
+        for fid, name in enumerate(_lib.PROF_FAMILIES):
+            ms = np.zeros(1)
+            n = np.zeros(1, np.int64)
+            by = np.zeros(1)
+            ctx.check(lib.spc_profile_read(ctx.h, fid, _lib.ptr(ms), _lib.ptr(n, _lib._i64p), _lib.ptr(by)))
+            if n[0]:
+                families[name] = {"ms": float(ms[0]), "launches": int(n[0]), "alg_bytes": float(by[0])}
+        lib.spc_profile_enable(ctx.h, 0)
+        streaming = {k: v for k, v in families.items() if v["alg_bytes"] > 0 and k != "reduce"}
+        dom = max(streaming, key=lambda k: streaming[k]["ms"])
+        f = streaming[dom]
+        achieved = f["alg_bytes"] / (f["ms"] * 1e-3) / 1e9
+        loop = [families[k] for k in ("pass1_spmm_dots", "pass2_update", "final_dots") if k in families]
+        loop_ms = sum(x["ms"] for x in loop)
+        loop_bytes = sum(x["alg_bytes"] for x in loop)
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                traffic = json.load(fh).get(dom)
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "bytes_per_launch": f["alg_bytes"] / f["launches"],
+                    "avg_launch_us": 1000.0 * f["ms"] / f["launches"],
+                    "arnoldi_loop": {"achieved": round(loop_bytes / (loop_ms * 1e-3) / 1e9, 1),
+                                     "frac": round(loop_bytes / (loop_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                     "ms_per_step": loop_ms / taken}}
+
+    cpu = None if a.no_cpu_baseline else cpu_baseline(w, a.cpu_seconds)
+    line = {"metric": METRIC, "value": value, "unit": "state·steps/s", "n_gpus": 1, "steps": int(taken),
+            "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+            "config": {"workload": f"config2 trace sampling: N={N} (2x14N+6x1H | 4x1H), anisotropic HFC, "
+                                   f"B=0.05 mT @37deg, J=0.01 mT, kS=2/kT=0.5 us^-1, M={M} random-phase vectors, "
+                                   f"m={m}, dt=1 ns, tol=0", "N": N, "M": M, "krylov": m, "nnz": H.nnz,
+                       "device_ms_per_step": dev_ms / max(taken, 1)},
+            "roofline": roofline, "cpu_baseline": cpu}
+    if families:
+        line["kernel_families"] = families
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,173 @@
+/*
+ * spinchem.h -- C-ABI of libspinchem_hip.so, the MI355X (gfx950) path for
+ * trace-sampling singlet-yield propagation.
+ *
+ * Conventions
+ *   - Complex numbers are interleaved doubles (re, im), the layout of C99
+ *     double _Complex and Fortran complex(c_double_complex).
+ *   - Host arrays are caller-allocated; device memory is owned by the context.
+ *   - Every call returns 0 (SPC_OK) or an SPC_ERR_* code; spc_last_error() gives
+ *     the message.  A context is used from one host thread at a time.
+ *   - Units are those of the reference after read_inp (spinsys.f90:60-135):
+ *     couplings and field in mT, rates divided by gamma_e, times multiplied by
+ *     gamma_e/1000 (ns -> rad/mT).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef SPINCHEM_H
+#define SPINCHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPC_OK 0
+#define SPC_ERR_INVALID 1     /* bad argument or input (reference: stop 'Error: ...') */
+#define SPC_ERR_UNSUPPORTED 2 /* multiplicity, Krylov order or size outside the device path */
+#define SPC_ERR_OOM 3         /* device allocation failed */
+#define SPC_ERR_HIP 4         /* HIP runtime error */
+#define SPC_ERR_BREAKDOWN 5   /* Krylov breakdown (h_{i+1,i} = 0) or non-finite result */
+#define SPC_ERR_NO_DEVICE 6   /* no HIP device */
+
+#define SPC_MAX_KRYLOV 32   /* largest N_krylov on the device path */
+#define SPC_MAX_NUCLEI 64
+
+/* Initial radical-pair state (qm_dynamics.f90:386-414 init_el_state). */
+#define SPC_STATE_SINGLET 0
+#define SPC_STATE_T0 1
+#define SPC_STATE_TP 2
+#define SPC_STATE_TM 3
+
+/* Trace-vector kinds. */
+#define SPC_VEC_SUZ 0      /* sample_SUZ: complex normals, normalised (qm_dynamics.f90:416-430) */
+#define SPC_VEC_PHASE 1    /* random phase: (x + i y)/sqrt(Z), (x, y) = rng%circle(1) */
+#define SPC_VEC_INJECTED 2 /* caller-provided nuclear vectors [M][Z] */
+
+typedef struct spc_ctx spc_ctx;
+
+/* Flattened sys_param (spinsys.f90:21-38) plus the field of main.f90:55-56.
+ * Tensors are column-major per nucleus as in the .ini (spinsys.f90:231):
+ * A(a,b) = hfc[9k + 3b + a].  With iso_rule = 1 the reference's isotropy test
+ * (A11 == A22 for every nucleus of both electrons, spinsys.f90:237-254) decides
+ * whether trace(A)/3 replaces each tensor, as getHiso_sp does. */
+typedef struct spc_spin_system {
+    int32_t n_nuc1, n_nuc2;    /* nuclei on electron 1 and 2                 */
+    const int32_t *mult;       /* n_nuc1 + n_nuc2 multiplicities (2 or 3)     */
+    const double *hfc;         /* 9 * (n_nuc1 + n_nuc2), mT                   */
+    double J;                  /* exchange, mT                                */
+    double D[9];               /* dipolar tensor, column-major, mT            */
+    double kS, kT;             /* recombination rates / gamma_e               */
+    double B;                  /* field magnitude, mT                         */
+    double field_dir[3];       /* unit direction; (0, 0, 1) is the reference  */
+    int32_t iso_rule;          /* 1: reference isotropy rule, 0: full tensors */
+} spc_spin_system;
+
+/* sim_param subset used by the propagation (spinsys.f90:40-56). */
+typedef struct spc_sim_params {
+    int32_t n_samples;   /* M trace vectors                                    */
+    int32_t n_krylov;    /* m (1..SPC_MAX_KRYLOV)                              */
+    double t_end;        /* simulation_time, converted                         */
+    double dt;           /* dt, converted                                      */
+    double tol;          /* integrator_tolerance (>= 0)                        */
+    int32_t init_state;  /* SPC_STATE_*                                        */
+    int32_t vec_kind;    /* SPC_VEC_*                                          */
+} spc_sim_params;
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart: owns the device, a stream and device buffers. */
+int spc_ctx_create(int device, spc_ctx **out);
+void spc_ctx_destroy(spc_ctx *ctx);
+const char *spc_last_error(const spc_ctx *ctx);
+/* Library version string. */
+const char *spc_version(void);
+
+/* ------------------------------------------------------------ Hamiltonian */
+/* Replaces getHiso_sp + coo_to_csr (hamiltonian.f90:17-205, utils.f90:52-89):
+ * assembles the CSR of H on the device from the spin-operator factors.  Also
+ * covers anisotropic tensors, tilted fields and full D, which the reference
+ * stops on (qm_dynamics.f90:300). */
+int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys);
+/* Dimensions of the current Hamiltonian. */
+int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz);
+/* Copies the device CSR (0-based rowptr[N+1], col[nnz], val[2*nnz]). */
+int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *val);
+
+/* ---------------------------------------------------------------- dynamics */
+/* ceiling((t_end+dt)/dt) (qm_dynamics.f90:305); outputs hold this + 1 values. */
+int64_t spc_n_steps(double t_end, double dt);
+
+/* Replaces trace_sampling_para (qm_dynamics.f90:257-384): per-sample seeds
+ * from seed (main.f90:50, qm_dynamics.f90:315-320), trace vectors, restarted
+ * Krylov propagation with the tol rule, ordered sum over samples and 1/M.
+ * trace_vectors: [M][Z] complex when vec_kind == SPC_VEC_INJECTED, else NULL.
+ * P_*: n_out = spc_n_steps()+1 values each.  Phi_S/Phi_T: NULL or n_out/2
+ * cumulative Simpson yields (observables.f90:74-101) of the averaged traces. */
+int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2],
+                     const double *trace_vectors, double *P_S, double *P_T0, double *P_Tp,
+                     double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T);
+
+/* Replaces exact_dynamics (qm_dynamics.f90:24-137): the Z nuclear basis
+ * states as trace vectors (sim->n_samples and vec_kind are ignored). */
+int spc_exact_trace(spc_ctx *ctx, const spc_sim_params *sim, double *P_S, double *P_T0,
+                    double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T);
+
+/* Replaces expm (matrix_exponential.f90:14-117) for one m x m matrix
+ * (column-major, m <= SPC_MAX_KRYLOV), on the device. */
+int spc_expm_small(spc_ctx *ctx, int32_t m, const double *A, double *E);
+
+/* Replaces get_krylov (qm_dynamics.f90:432-467) for M vectors at once: psi
+ * [M][N] in, Q [M][m][N] (vector-major), A [M][m][m] (column-major) and
+ * c0 = ||psi|| [M] out.  The product path keeps these on the device; this
+ * entry point exists to check the Arnoldi kernels. */
+int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, double *Q, double *A,
+                     double *beta);
+
+/* y = H x for M vectors ([M][N] complex each), the spmv of qm_dynamics.f90:452. */
+int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y);
+
+/* Last-run timing of the propagation loop: milliseconds of device time and
+ * number of Krylov rebuilds, for benchmarks. */
+int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps);
+
+/* Kernel-family event profiler (no reference counterpart; benchmarks only).
+ * When enabled, the next propagation records HIP events around every launch
+ * on the context stream; spc_profile_read sums one family's device time,
+ * launch count and algorithmic bytes (complex128 vector passes over the M
+ * trace vectors, plus the CSR for the sparse product). */
+#define SPC_PROF_PASS1 0  /* v = H u with Gram-Schmidt dots            */
+#define SPC_PROF_PASS2 1  /* basis update q_i, u_{i+1}                 */
+#define SPC_PROF_DOTS 2   /* last-column correction dots               */
+#define SPC_PROF_PSI 3    /* psi = Q c with fused populations          */
+#define SPC_PROF_REDUCE 4 /* workgroup-partial reductions              */
+#define SPC_PROF_COEFFS 5 /* per-vector Arnoldi coefficient algebra    */
+#define SPC_PROF_EXPM 6   /* m x m Pade exponential per vector         */
+#define SPC_PROF_STEP 7   /* c <- E c step loop                        */
+#define SPC_PROF_GRAM 8   /* projector Gram matrices (tol > 0)         */
+int spc_profile_enable(spc_ctx *ctx, int on);
+int spc_profile_read(spc_ctx *ctx, int family, double *total_ms, int64_t *launches, double *bytes);
+
+/* ----------------------------------------------------------- .ini reader */
+/* FiNeR file_ini subset used by read_inp (spinsys.f90:60-135): sections,
+ * key = value lines, space/comma separated lists, ';' '#' '!' comments. */
+typedef struct spc_ini spc_ini;
+int spc_ini_open(const char *path, spc_ini **out);
+void spc_ini_close(spc_ini *ini);
+int spc_ini_num_sections(const spc_ini *ini);
+int spc_ini_has_section(const spc_ini *ini, const char *section);
+/* Number of values of an option, or -1 if absent. */
+int spc_ini_count_values(const spc_ini *ini, const char *section, const char *option);
+/* Copies the raw value string (trimmed, quotes removed); 0 or SPC_ERR_INVALID if absent. */
+int spc_ini_get_string(const spc_ini *ini, const char *section, const char *option, char *buf,
+                       int32_t buflen);
+int spc_ini_get_doubles(const spc_ini *ini, const char *section, const char *option, double *out,
+                        int32_t n);
+int spc_ini_get_int64s(const spc_ini *ini, const char *section, const char *option, int64_t *out,
+                       int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,738 @@
+// capi.cpp -- the C-ABI of libspinchem_hip.so (declared in include/spinchem.h).
+//
+// Host side of the trace-sampling path: validates inputs the way read_inp and
+// getHiso_sp do, builds the Hamiltonian on the device, runs the lane-parallel
+// Krylov propagation (krylov.hip) chunk by chunk, and reduces the per-sample
+// populations in sample order (qm_dynamics.f90:373-378).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/spinchem.h"
+#include "internal.hpp"
+
+using namespace spc;
+
+namespace {
+
+constexpr double kGammaE = 176.08596;
+constexpr double kGamma1H = 0.26752218708;
+constexpr double kGamma14N = 0.0193297792;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+struct spc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t lds_limit = 65536;
+    // system
+    bool have_sys = false;
+    int64_t N = 0, Z = 0, nnz = 0;
+    DevSystem hsys{};
+    double kS = 0.0, kT = 0.0;
+    DevSystem *d_sys = nullptr;
+    int32_t *d_rowptr = nullptr, *d_col = nullptr;
+    double2 *d_val = nullptr;
+    // workspace
+    std::map<std::string, DevBuf> bufs;
+    // stats
+    Prof prof;
+    double last_ms = 0.0;
+    int64_t last_rebuilds = 0, last_steps = 0;
+};
+
+namespace {
+
+int fail(spc_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(spc_ctx *ctx, hipError_t e, const char *where)
+{
+    return fail(ctx, SPC_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                          \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+// Grow-only named device buffer.
+template <class T>
+int ensure(spc_ctx *ctx, const char *name, size_t count, T **out)
+{
+    DevBuf &b = ctx->bufs[name];
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.bytes < bytes) {
+        if (b.p) hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, SPC_ERR_OOM, std::string("device allocation of ") + std::to_string(bytes) +
+                                              " bytes for " + name + " failed");
+        }
+        b.bytes = bytes;
+    }
+    *out = (T *)b.p;
+    return SPC_OK;
+}
+
+// ------------------------------------------------------------ host RNG
+// Master stream for per-sample seeds (main.f90:50, qm_dynamics.f90:315-320).
+struct HostRng {
+    uint64_t s0, s1;
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        uint64_t a = s0, b = s1, r = a + b;
+        b ^= a;
+        s0 = rotl(a, 55) ^ b ^ (b << 14);
+        s1 = rotl(b, 36);
+        return r;
+    }
+    void jump()
+    {
+        const uint64_t j[2] = {0xbeac0467eba5facbull, 0xd86b048b86aa9922ull};
+        uint64_t t0 = 0, t1 = 0;
+        for (int i = 0; i < 2; ++i)
+            for (int b = 0; b < 64; ++b) {
+                if (j[i] & (1ull << b)) {
+                    t0 ^= s0;
+                    t1 ^= s1;
+                }
+                next();
+            }
+        s0 = t0;
+        s1 = t1;
+    }
+    void set_seed(const int64_t s[2])
+    {
+        s0 = (uint64_t)s[0];
+        s1 = (uint64_t)s[1];
+        jump();
+    }
+};
+
+int64_t steps_taken(double t_end, double dt)
+{
+    double t = 0.0;
+    int64_t k = 0;
+    while (t < t_end) {
+        t = t + dt;
+        ++k;
+    }
+    return k;
+}
+
+void theta_of(int st, double2 th[4], double pop[4])
+{
+    const double r = 1.0 / std::sqrt(2.0);
+    for (int i = 0; i < 4; ++i) {
+        th[i] = make_double2(0, 0);
+        pop[i] = 0;
+    }
+    switch (st) {
+    case SPC_STATE_SINGLET: th[1] = make_double2(r, 0); th[2] = make_double2(-r, 0); pop[0] = 1; break;
+    case SPC_STATE_T0: th[1] = make_double2(r, 0); th[2] = make_double2(r, 0); pop[1] = 1; break;
+    case SPC_STATE_TP: th[0] = make_double2(1, 0); pop[2] = 1; break;
+    default: th[3] = make_double2(1, 0); pop[3] = 1; break;
+    }
+}
+
+// observables.f90:74-101 on the averaged traces.
+void simpson(int64_t n, const double *PS, const double *PT0, const double *PTp, const double *PTm, double dt,
+             double kS, double kT, double *Phi_S, double *Phi_T)
+{
+    const int64_t h = n / 2;
+    if (h <= 0) return;
+    std::vector<double> PT(n);
+    for (int64_t i = 0; i < n; ++i) PT[i] = PTp[i] + PT0[i] + PTm[i];
+    double s = 0.0, t = 0.0;
+    if (Phi_S) Phi_S[0] = 0.0;
+    if (Phi_T) Phi_T[0] = 0.0;
+    for (int64_t i = 1; i <= h - 1; ++i) {
+        s = s + (1.0 / 3.0) * dt * kS * (PS[2 * i - 2] + 4 * PS[2 * i - 1] + PS[2 * i]);
+        t = t + (1.0 / 3.0) * dt * kT * (PT[2 * i - 2] + 4 * PT[2 * i - 1] + PT[2 * i]);
+        if (Phi_S) Phi_S[i] = s;
+        if (Phi_T) Phi_T[i] = t;
+    }
+}
+
+int check_sim(spc_ctx *ctx, const spc_sim_params *sim)
+{
+    if (!sim) return fail(ctx, SPC_ERR_INVALID, "sim is NULL");
+    if (sim->n_krylov < 1 || sim->n_krylov > SPC_MAX_KRYLOV)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "N_krylov must be in 1.." + std::to_string(SPC_MAX_KRYLOV));
+    if (!(sim->dt > 0.0) || !std::isfinite(sim->dt)) return fail(ctx, SPC_ERR_INVALID, "dt must be positive");
+    if (!std::isfinite(sim->t_end)) return fail(ctx, SPC_ERR_INVALID, "simulation_time must be finite");
+    if (!(sim->tol >= 0.0)) return fail(ctx, SPC_ERR_INVALID, "integrator_tolerance must be >= 0");
+    if (sim->init_state < 0 || sim->init_state > 3)
+        return fail(ctx, SPC_ERR_INVALID, "Error: Invalid initial radical pair state entered.");
+    if (4 * (size_t)sim->n_krylov * sim->n_krylov * sizeof(double2) > ctx->lds_limit)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "N_krylov too large for the device LDS");
+    return SPC_OK;
+}
+
+// Propagates M_total trace vectors (chunks of at most max_lanes) and writes
+// the sample-averaged populations; vi.kind selects how vectors are made.
+int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_total,
+              const std::vector<int64_t> &seeds, const double *injected, double *P_S, double *P_T0, double *P_Tp,
+              double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T)
+{
+    if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
+    int rc = check_sim(ctx, sim);
+    if (rc) return rc;
+    if (M_total < 1) return fail(ctx, SPC_ERR_INVALID, "N_samples must be >= 1");
+    const int64_t n_steps = spc_n_steps(sim->t_end, sim->dt);
+    if (n_out != n_steps + 1)
+        return fail(ctx, SPC_ERR_INVALID,
+                    "output length must be ceiling((t_end+dt)/dt)+1 = " + std::to_string(n_steps + 1));
+    if (!P_S || !P_T0 || !P_Tp || !P_Tm) return fail(ctx, SPC_ERR_INVALID, "output arrays must not be NULL");
+    const int64_t K = steps_taken(sim->t_end, sim->dt);
+    if (K + 1 > n_out) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
+    const int m = sim->n_krylov;
+    const int64_t N = ctx->N, Z = ctx->Z;
+
+    // lanes per chunk: basis + 3 vectors + populations, within half the free memory
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_lane = ((size_t)m + 3) * N * sizeof(double2) + (size_t)n_out * 4 * sizeof(double) +
+                            4 * (size_t)m * m * sizeof(double2) * 4;
+    int64_t max_lanes = (int64_t)std::min<size_t>(1024, (free_b / 2) / std::max<size_t>(per_lane, 1));
+    max_lanes = (max_lanes / 64) * 64;
+    if (max_lanes < 64) return fail(ctx, SPC_ERR_OOM, "not enough device memory for one 64-vector chunk");
+    const int64_t Mchunk = std::min<int64_t>(M_total, max_lanes);
+    const int Mp = (int)((Mchunk + 63) / 64 * 64);
+
+    KrylovWork w{};
+    w.N = N;
+    w.Z = Z;
+    w.Mp = Mp;
+    w.m = m;
+    w.n_out = n_out;
+    w.rowptr = ctx->d_rowptr;
+    w.col = ctx->d_col;
+    w.val = ctx->d_val;
+    w.nnz = ctx->nnz;
+    w.prof = &ctx->prof;
+    ctx->prof.clear();
+    // rows per wave: about 2048 workgroups of 4 waves per pass
+    const int nch = Mp / 64;
+    auto rpw_for = [&](int64_t rows, int64_t target) {
+        int64_t rows_per_wg = std::max<int64_t>(4, (rows * nch + target - 1) / target);
+        return (int)std::max<int64_t>(1, (rows_per_wg + 3) / 4);
+    };
+    w.rpw1 = rpw_for(N, 1024);
+    w.rpw2 = rpw_for(N, 2048);
+    w.rpwp = rpw_for(Z, 1024);
+    const bool need_gram = sim->tol > 0.0;
+
+    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
+    w.u = w.psi;
+    if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
+    size_t part_need = partials_needed(w);
+    if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
+    if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
+    w.part_capacity = part_need;
+    if ((rc = ensure(ctx, "dots", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, "E", (size_t)m * m * Mp, &w.E))) return rc;
+    if ((rc = ensure(ctx, "c", (size_t)m * Mp, &w.c))) return rc;
+    if (need_gram && (rc = ensure(ctx, "gram", (size_t)4 * m * m * Mp, &w.gram))) return rc;
+    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, "lane_t", (size_t)Mp, &w.lane_t))) return rc;
+    if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
+    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
+    if ((rc = ensure(ctx, "obs", (size_t)n_out * 4 * Mp, &w.obs))) return rc;
+    double *d_acc = nullptr;
+    if ((rc = ensure(ctx, "acc", (size_t)4 * n_out, &d_acc))) return rc;
+    int64_t *d_seeds = nullptr;
+    double2 *d_inj = nullptr;
+    if (vi.kind == 0 || vi.kind == 1)
+        if ((rc = ensure(ctx, "seeds", (size_t)2 * Mp, &d_seeds))) return rc;
+    if (vi.kind == 2)
+        if ((rc = ensure(ctx, "inj", (size_t)Mp * Z, &d_inj))) return rc;
+
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * n_out, s));
+    HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
+
+    double pop[4];
+    theta_of(sim->init_state, vi.theta, pop);
+    std::vector<double> obs0((size_t)4 * Mp, 0.0);
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, s));
+    int64_t rebuilds = 0;
+    bool breakdown = false;
+
+    for (int64_t start = 0; start < M_total; start += Mchunk) {
+        const int M = (int)std::min<int64_t>(Mchunk, M_total - start);
+        w.M = M;
+        VecInit v = vi;
+        if (vi.kind == 0 || vi.kind == 1) {
+            HIP_TRY(hipMemcpyAsync(d_seeds, seeds.data() + 2 * start, sizeof(int64_t) * 2 * M,
+                                   hipMemcpyHostToDevice, s));
+            v.seeds = d_seeds;
+        } else if (vi.kind == 2) {
+            HIP_TRY(hipMemcpyAsync(d_inj, injected + 2 * start * Z, sizeof(double2) * (size_t)M * Z,
+                                   hipMemcpyHostToDevice, s));
+            v.inj = d_inj;
+        } else {
+            v.basis_offset = start;
+        }
+        for (bool gram : {need_gram, true}) {
+            HIP_TRY(init_vectors(v, M, Mp, Z, w.psi, s));
+            HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, s));
+            HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
+            HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
+            HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * n_out * 4 * Mp, s));
+            for (int x = 0; x < 4; ++x)
+                for (int r = 0; r < M; ++r) obs0[(size_t)x * Mp + r] = pop[x];
+            HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
+            std::vector<double> lt(Mp);
+            for (int64_t round = 0;; ++round) {
+                if (!gram && round >= K) break;
+                HIP_TRY(arnoldi(w, s));
+                HIP_TRY(expm_lanes(w, sim->dt, s));
+                if (gram) HIP_TRY(gram_projectors(w, s));
+                HIP_TRY(step_lanes(w, sim->dt, sim->t_end, sim->tol, gram, s));
+                HIP_TRY(psi_update(w, !gram, s));
+                ++rebuilds;
+                if (gram) {
+                    HIP_TRY(hipMemcpyAsync(lt.data(), w.lane_t, sizeof(double) * Mp, hipMemcpyDeviceToHost, s));
+                    HIP_TRY(hipStreamSynchronize(s));
+                    bool done = true;
+                    for (int r = 0; r < M; ++r) done = done && !(lt[r] < sim->t_end);
+                    if (done) break;
+                    if (round > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
+                }
+            }
+            int32_t flags[4];
+            HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
+            breakdown = breakdown || flags[0];
+            if (!gram && flags[1]) continue; // a lane needed two steps per rebuild: redo with projector Grams
+            break;
+        }
+        HIP_TRY(obs_reduce_lanes(w.obs, n_out, M, Mp, d_acc, s));
+    }
+    HIP_TRY(hipEventRecord(ev1, s));
+    std::vector<double> acc((size_t)4 * n_out);
+    HIP_TRY(hipMemcpyAsync(acc.data(), d_acc, sizeof(double) * 4 * n_out, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    ctx->last_ms = ms;
+    ctx->last_rebuilds = rebuilds;
+    ctx->last_steps = K;
+    if (breakdown)
+        return fail(ctx, SPC_ERR_BREAKDOWN,
+                    "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+
+    const double scale = 1.0 / (double)M_total;
+    double *outs[4] = {P_S, P_T0, P_Tp, P_Tm};
+    for (int x = 0; x < 4; ++x)
+        for (int64_t k = 0; k < n_out; ++k) outs[x][k] = acc[(size_t)x * n_out + k] * scale;
+    for (int64_t k = 0; k < n_out; ++k)
+        for (int x = 0; x < 4; ++x)
+            if (!std::isfinite(outs[x][k])) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite population");
+    if (Phi_S || Phi_T)
+        simpson(n_out, P_S, P_T0, P_Tp, P_Tm, sim->dt, ctx->kS, ctx->kT, Phi_S, Phi_T);
+    return SPC_OK;
+}
+
+} // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char *spc_version(void) { return "spinchem-mi355x 0.1 (gfx950)"; }
+
+int64_t spc_n_steps(double t_end, double dt) { return (int64_t)std::ceil((t_end + dt) / dt); }
+
+int spc_ctx_create(int device, spc_ctx **out)
+{
+    if (!out) return SPC_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return SPC_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return SPC_ERR_INVALID;
+    spc_ctx *ctx = new spc_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return SPC_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->lds_limit = prop.sharedMemPerBlock;
+    *out = ctx;
+    return SPC_OK;
+}
+
+void spc_ctx_destroy(spc_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->bufs)
+        if (kv.second.p) hipFree(kv.second.p);
+    hipFree(ctx->d_sys);
+    hipFree(ctx->d_rowptr);
+    hipFree(ctx->d_col);
+    hipFree(ctx->d_val);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *spc_last_error(const spc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
+{
+    if (!ctx || !sys) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int nn = sys->n_nuc1 + sys->n_nuc2;
+    if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || nn > SPC_MAX_NUCLEI)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "number of nuclei outside 0.." + std::to_string(SPC_MAX_NUCLEI));
+    if (nn > 0 && (!sys->mult || !sys->hfc)) return fail(ctx, SPC_ERR_INVALID, "mult/hfc must not be NULL");
+    DevSystem S{};
+    S.nn = nn;
+    S.n1 = sys->n_nuc1;
+    int64_t Z = 1;
+    for (int k = 0; k < nn; ++k) {
+        const int g = sys->mult[k];
+        if (g != 2 && g != 3) // hamiltonian.f90:132-138
+            return fail(ctx, SPC_ERR_UNSUPPORTED, std::string("Error: sys%e") + (k < sys->n_nuc1 ? "1" : "2") +
+                                                      "%g_I(i) must be 3 or 2");
+        S.mult[k] = g;
+        Z *= g;
+        if (Z > (int64_t)1 << 28) return fail(ctx, SPC_ERR_UNSUPPORTED, "Hilbert space too large");
+    }
+    S.Z = Z;
+    {
+        int64_t p = 1;
+        for (int k = nn - 1; k >= 0; --k) {
+            S.stride[k] = p;
+            p *= S.mult[k];
+        }
+    }
+    // isotropy rule of the reference (spinsys.f90:237-254): if every tensor of
+    // both electrons has A11 == A22, trace(A)/3 replaces it (getHiso_sp).
+    bool iso = sys->iso_rule != 0;
+    if (iso)
+        for (int k = 0; k < nn; ++k)
+            if (sys->hfc[9 * k + 0] != sys->hfc[9 * k + 4]) iso = false;
+    for (int k = 0; k < nn; ++k) {
+        const double *A = sys->hfc + 9 * k;
+        if (iso) {
+            const double a = (A[0] + A[4] + A[8]) / 3.0;
+            for (int q = 0; q < 9; ++q) S.A[k][q] = (q % 4 == 0) ? a : 0.0;
+        } else {
+            for (int q = 0; q < 9; ++q) S.A[k][q] = A[q];
+        }
+        const double gam = S.mult[k] == 3 ? kGamma14N : kGamma1H;
+        S.zeta[k] = sys->B * gam / kGammaE; // -w gamma_N/gamma_e with w = -B
+    }
+    double nrm = std::sqrt(sys->field_dir[0] * sys->field_dir[0] + sys->field_dir[1] * sys->field_dir[1] +
+                           sys->field_dir[2] * sys->field_dir[2]);
+    if (!(nrm > 0.0)) return fail(ctx, SPC_ERR_INVALID, "field direction must be nonzero");
+    for (int a = 0; a < 3; ++a) {
+        S.dir[a] = sys->field_dir[a] / nrm;
+        S.w[a] = -sys->B * S.dir[a];
+    }
+    if (nrm == 1.0)
+        for (int a = 0; a < 3; ++a) S.dir[a] = sys->field_dir[a], S.w[a] = -sys->B * sys->field_dir[a];
+    const double dk = (sys->kS - sys->kT) / 4.0;
+    for (int q = 0; q < 9; ++q) {
+        S.Tre[q] = sys->D[q] + ((q % 4 == 0) ? 2.0 * sys->J : 0.0);
+        S.Tim[q] = (q % 4 == 0) ? 2.0 * dk : 0.0;
+    }
+    S.kbar = (sys->kS + 3.0 * sys->kT) / 4.0;
+
+    const int64_t N = 4 * Z;
+    hipStream_t s = ctx->stream;
+    if (!ctx->d_sys) HIP_TRY(hipMalloc(&ctx->d_sys, sizeof(DevSystem)));
+    HIP_TRY(hipMemcpyAsync(ctx->d_sys, &S, sizeof(DevSystem), hipMemcpyHostToDevice, s));
+    hipFree(ctx->d_rowptr);
+    ctx->d_rowptr = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_rowptr, sizeof(int32_t) * (N + 1)));
+    int32_t *counts = nullptr;
+    int rc = ensure(ctx, "hcounts", (size_t)N + 1, &counts);
+    if (rc) return rc;
+    HIP_TRY(h_count(ctx->d_sys, N, counts, s));
+    size_t tmp_bytes = 0;
+    HIP_TRY(h_scan(counts, ctx->d_rowptr, N, nullptr, tmp_bytes, s));
+    char *tmp = nullptr;
+    if ((rc = ensure(ctx, "scan_tmp", tmp_bytes, &tmp))) return rc;
+    HIP_TRY(h_scan(counts, ctx->d_rowptr, N, tmp, tmp_bytes, s));
+    int32_t nnz = 0;
+    HIP_TRY(hipMemcpyAsync(&nnz, ctx->d_rowptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    hipFree(ctx->d_col);
+    hipFree(ctx->d_val);
+    ctx->d_col = nullptr;
+    ctx->d_val = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_col, sizeof(int32_t) * std::max(nnz, 1)));
+    HIP_TRY(hipMalloc(&ctx->d_val, sizeof(double2) * std::max(nnz, 1)));
+    HIP_TRY(h_fill(ctx->d_sys, N, ctx->d_rowptr, ctx->d_col, ctx->d_val, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx->hsys = S;
+    ctx->kS = sys->kS;
+    ctx->kT = sys->kT;
+    ctx->N = N;
+    ctx->Z = Z;
+    ctx->nnz = nnz;
+    ctx->have_sys = true;
+    return SPC_OK;
+}
+
+int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz)
+{
+    if (!ctx || !ctx->have_sys) return SPC_ERR_INVALID;
+    if (N) *N = ctx->N;
+    if (Z) *Z = ctx->Z;
+    if (nnz) *nnz = ctx->nnz;
+    return SPC_OK;
+}
+
+int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *val)
+{
+    if (!ctx || !ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (rowptr) HIP_TRY(hipMemcpy(rowptr, ctx->d_rowptr, sizeof(int32_t) * (ctx->N + 1), hipMemcpyDeviceToHost));
+    if (col) HIP_TRY(hipMemcpy(col, ctx->d_col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
+    if (val) HIP_TRY(hipMemcpy(val, ctx->d_val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
+    return SPC_OK;
+}
+
+int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2], const double *trace_vectors,
+                     double *P_S, double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S,
+                     double *Phi_T)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (!sim) return fail(ctx, SPC_ERR_INVALID, "sim is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t M = sim->n_samples;
+    VecInit vi{};
+    std::vector<int64_t> seeds;
+    if (sim->vec_kind == SPC_VEC_SUZ || sim->vec_kind == SPC_VEC_PHASE) {
+        if (!seed) return fail(ctx, SPC_ERR_INVALID, "seed is NULL");
+        HostRng g;
+        g.set_seed(seed);
+        seeds.resize(2 * std::max<int64_t>(M, 0));
+        for (int64_t i = 0; i < M; ++i) {
+            seeds[2 * i] = (int64_t)g.next();
+            seeds[2 * i + 1] = (int64_t)g.next();
+        }
+        vi.kind = sim->vec_kind;
+    } else if (sim->vec_kind == SPC_VEC_INJECTED) {
+        if (!trace_vectors) return fail(ctx, SPC_ERR_INVALID, "trace_vectors is NULL");
+        vi.kind = 2;
+    } else {
+        return fail(ctx, SPC_ERR_INVALID, "unknown vec_kind");
+    }
+    return propagate(ctx, sim, vi, M, seeds, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
+int spc_exact_trace(spc_ctx *ctx, const spc_sim_params *sim, double *P_S, double *P_T0, double *P_Tp, double *P_Tm,
+                    int64_t n_out, double *Phi_S, double *Phi_T)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    VecInit vi{};
+    vi.kind = 3;
+    return propagate(ctx, sim, vi, ctx->Z, {}, nullptr, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
+int spc_expm_small(spc_ctx *ctx, int32_t m, const double *A, double *E)
+{
+    if (!ctx || !A || !E) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    if (m < 1 || m > SPC_MAX_KRYLOV) return fail(ctx, SPC_ERR_UNSUPPORTED, "m outside 1..32");
+    if (4 * (size_t)m * m * sizeof(double2) > ctx->lds_limit)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "m too large for the device LDS");
+    HIP_TRY(hipSetDevice(ctx->device));
+    double2 *dA = nullptr, *dE = nullptr;
+    int rc;
+    if ((rc = ensure(ctx, "expmA", (size_t)m * m, &dA))) return rc;
+    if ((rc = ensure(ctx, "expmE", (size_t)m * m, &dE))) return rc;
+    HIP_TRY(hipMemcpyAsync(dA, A, sizeof(double2) * m * m, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(expm_single(m, dA, dE, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(E, dE, sizeof(double2) * m * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPC_OK;
+}
+
+// Shared set-up for the test hooks: a work struct for M lanes.
+static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
+{
+    if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
+    if (M < 1) return fail(ctx, SPC_ERR_INVALID, "M must be >= 1");
+    const int Mp = (M + 63) / 64 * 64;
+    w = KrylovWork{};
+    w.N = ctx->N;
+    w.Z = ctx->Z;
+    w.M = M;
+    w.Mp = Mp;
+    w.m = std::max(m, 1);
+    w.rowptr = ctx->d_rowptr;
+    w.col = ctx->d_col;
+    w.val = ctx->d_val;
+    w.rpw1 = w.rpw2 = w.rpwp = 4;
+    int rc;
+    if ((rc = ensure(ctx, "Q", (size_t)w.m * w.N * Mp, &w.Q))) return rc;
+    if ((rc = ensure(ctx, "psi", (size_t)w.N * Mp, &w.psi))) return rc;
+    w.u = w.psi;
+    if ((rc = ensure(ctx, "v", (size_t)w.N * Mp, &w.v))) return rc;
+    size_t pn = partials_needed(w);
+    if ((rc = ensure(ctx, "part", pn, &w.part))) return rc;
+    w.part_capacity = pn;
+    if ((rc = ensure(ctx, "dots", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, "hess", (size_t)w.m * w.m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
+    return SPC_OK;
+}
+
+// host [M][N] <-> device lane-fastest [N][Mp]
+static void to_lanes(const double *x, int M, int Mp, int64_t N, std::vector<double2> &out)
+{
+    out.assign((size_t)N * Mp, make_double2(0, 0));
+    for (int r = 0; r < M; ++r)
+        for (int64_t n = 0; n < N; ++n) out[n * Mp + r] = make_double2(x[2 * (r * N + n)], x[2 * (r * N + n) + 1]);
+}
+
+int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y)
+{
+    if (!ctx || !x || !y) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    KrylovWork w;
+    int rc = hook_work(ctx, M, 1, w);
+    if (rc) return rc;
+    std::vector<double2> h;
+    to_lanes(x, M, w.Mp, w.N, h);
+    HIP_TRY(hipMemcpy(w.u, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+    HIP_TRY(spmm_only(w, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(h.data(), w.v, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < M; ++r)
+        for (int64_t n = 0; n < w.N; ++n) {
+            y[2 * (r * w.N + n)] = h[n * w.Mp + r].x;
+            y[2 * (r * w.N + n) + 1] = h[n * w.Mp + r].y;
+        }
+    return SPC_OK;
+}
+
+int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, double *Q, double *A, double *beta)
+{
+    if (!ctx || !psi) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    if (m < 1 || m > SPC_MAX_KRYLOV) return fail(ctx, SPC_ERR_UNSUPPORTED, "m outside 1..32");
+    HIP_TRY(hipSetDevice(ctx->device));
+    KrylovWork w;
+    int rc = hook_work(ctx, M, m, w);
+    if (rc) return rc;
+    std::vector<double2> h;
+    to_lanes(psi, M, w.Mp, w.N, h);
+    HIP_TRY(hipMemcpy(w.psi, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(w.hess, 0, sizeof(double2) * m * m * w.Mp));
+    HIP_TRY(hipMemset(w.flags, 0, sizeof(int32_t) * 4));
+    HIP_TRY(arnoldi(w, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int32_t flags[4];
+    HIP_TRY(hipMemcpy(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost));
+    if (Q) {
+        std::vector<double2> q((size_t)m * w.N * w.Mp);
+        HIP_TRY(hipMemcpy(q.data(), w.Q, sizeof(double2) * q.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < M; ++r)
+            for (int j = 0; j < m; ++j)
+                for (int64_t n = 0; n < w.N; ++n) {
+                    const double2 z = q[((size_t)j * w.N + n) * w.Mp + r];
+                    const size_t o = ((size_t)r * m + j) * w.N + n;
+                    Q[2 * o] = z.x;
+                    Q[2 * o + 1] = z.y;
+                }
+    }
+    if (A) {
+        std::vector<double2> hs((size_t)m * m * w.Mp);
+        HIP_TRY(hipMemcpy(hs.data(), w.hess, sizeof(double2) * hs.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < M; ++r)
+            for (int idx = 0; idx < m * m; ++idx) {
+                A[2 * ((size_t)r * m * m + idx)] = hs[(size_t)idx * w.Mp + r].x;
+                A[2 * ((size_t)r * m * m + idx) + 1] = hs[(size_t)idx * w.Mp + r].y;
+            }
+    }
+    if (beta) HIP_TRY(hipMemcpy(beta, w.beta0, sizeof(double) * M, hipMemcpyDeviceToHost));
+    if (flags[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown");
+    return SPC_OK;
+}
+
+int spc_profile_enable(spc_ctx *ctx, int on)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    ctx->prof.on = on != 0;
+    ctx->prof.clear();
+    return SPC_OK;
+}
+
+int spc_profile_read(spc_ctx *ctx, int family, double *total_ms, int64_t *launches, double *bytes)
+{
+    if (!ctx || family < 0 || family >= PF_COUNT) return fail(ctx, SPC_ERR_INVALID, "bad profile family");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double ms = 0.0, by = 0.0;
+    int64_t n = 0;
+    for (const auto &r : ctx->prof.recs) {
+        if (r.fam != family || !r.a || !r.b) continue;
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+        ms += t;
+        by += r.bytes;
+        ++n;
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = n;
+    if (bytes) *bytes = by;
+    return SPC_OK;
+}
+
+int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (device_ms) *device_ms = ctx->last_ms;
+    if (rebuilds) *rebuilds = ctx->last_rebuilds;
+    if (steps) *steps = ctx->last_steps;
+    return SPC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,131 @@
+// internal.hpp -- host-side declarations shared by the HIP translation units
+// and the C-ABI layer.  Not installed; the public interface is spinchem.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace spc {
+
+// Spin system as the device sees it (built once per field value).
+struct DevSystem {
+    int nn;                  // number of nuclei
+    int n1;                  // nuclei on electron 1 (the first n1 entries)
+    int64_t Z;               // nuclear dimension
+    int mult[64];
+    int64_t stride[64];      // digit stride in the nuclear index
+    double A[64][9];         // A(a,b) at A[k][3*b + a]
+    double zeta[64];         // nuclear Zeeman B*gamma_k/gamma_e
+    double w[3];             // electron Zeeman vector, -B n
+    double dir[3];           // field direction n
+    double Tre[9], Tim[9];   // S1.T.S2 coupling T(a,b) at [3*b + a], T = (2J + 2i dk) 1 + D
+    double kbar;             // (kS + 3 kT)/4
+};
+
+// ---------------------------------------------------------- hamiltonian.hip
+// counts[N]: entries per row; then rowptr by exclusive scan; then fill.
+hipError_t h_count(const DevSystem *d_sys, int64_t N, int32_t *counts, hipStream_t s);
+hipError_t h_scan(const int32_t *counts, int32_t *rowptr, int64_t N, void *tmp, size_t &tmp_bytes,
+                  hipStream_t s);
+hipError_t h_fill(const DevSystem *d_sys, int64_t N, const int32_t *rowptr, int32_t *col, double2 *val,
+                  hipStream_t s);
+
+// ------------------------------------------------------------- sampling.hip
+struct VecInit {
+    int kind;            // 0 SU(Z), 1 phase, 2 injected, 3 basis
+    const int64_t *seeds; // [M][2] per-lane seeds (kinds 0, 1)
+    const double2 *inj;   // [M][Z] injected nuclear vectors (kind 2)
+    int64_t basis_offset; // first basis index (kind 3)
+    double2 theta[4];     // electron state
+};
+hipError_t init_vectors(const VecInit &vi, int M, int Mp, int64_t Z, double2 *psi, hipStream_t s);
+
+// --------------------------------------------------------------- krylov.hip
+// Kernel families timed by the optional event profiler (spc_profile_*).
+enum ProfFamily { PF_PASS1 = 0, PF_PASS2, PF_DOTS, PF_PSI, PF_REDUCE, PF_COEFFS, PF_EXPM, PF_STEP, PF_GRAM, PF_COUNT };
+
+// Event pairs around launches on the work stream; read after a synchronize.
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int fam;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t get()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int fam, double bytes, hipStream_t s)
+    {
+        Rec r{fam, bytes, get(), get()};
+        if (r.a) hipEventRecord(r.a, s);
+        recs.push_back(r);
+    }
+    void end(hipStream_t s)
+    {
+        if (!recs.empty() && recs.back().b) hipEventRecord(recs.back().b, s);
+    }
+    void clear()
+    {
+        recs.clear();
+        used = 0;
+    }
+    ~Prof()
+    {
+        for (auto e : pool) hipEventDestroy(e);
+    }
+};
+
+// Device workspace of one propagation chunk; all arrays lane-fastest.
+struct KrylovWork {
+    int64_t N, Z;
+    int M, Mp, m;
+    const int32_t *rowptr;
+    const int32_t *col;
+    const double2 *val;
+    double2 *Q;      // [m][N][Mp]
+    double2 *u;      // [N][Mp]
+    double2 *v;      // [N][Mp]
+    double2 *psi;    // [N][Mp]
+    double2 *part;   // partial sums [nrb][nacc][Mp]
+    double2 *dots;   // reduced sums [nacc][Mp]
+    double2 *coef;   // pass-2 coefficients [2*kMaxKrylov + 4][Mp]
+    double2 *tprev;  // tentative column [kMaxKrylov + 1][Mp]
+    double2 *hess;   // Hessenberg [m][m][Mp] (col-major per lane)
+    double2 *E;      // expm [m][m][Mp]
+    double2 *c;      // Krylov coefficients [m][Mp]
+    double2 *gram;   // projector Gram matrices [4][m][m][Mp] (tol > 0)
+    double *beta0;   // [Mp]
+    double *lane_t;  // [Mp]
+    int32_t *lane_k; // [Mp] next output index
+    int32_t *flags;  // [4]: 0 breakdown, 1 direct-mode violation, 2 non-finite
+    double *obs;     // [n_out][4][Mp]
+    int64_t n_out;
+    size_t part_capacity; // elements of part
+    int rpw1, rpw2, rpwp;  // rows per wave for pass 1, pass 2 and psi update
+    int64_t nnz;           // Hamiltonian entries (algorithmic bytes of the product)
+    Prof *prof;            // optional kernel timing
+};
+
+size_t partials_needed(const KrylovWork &w);
+hipError_t arnoldi(KrylovWork &w, hipStream_t s);               // psi -> Q, hess, beta0
+hipError_t expm_lanes(KrylovWork &w, double dt, hipStream_t s);   // hess -> E
+hipError_t gram_projectors(KrylovWork &w, hipStream_t s);         // Q -> gram
+hipError_t step_lanes(KrylovWork &w, double dt, double t_end, double tol, bool gram, hipStream_t s);
+hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s); // psi = Q c (+ observables)
+hipError_t obs_reduce_lanes(const double *obs, int64_t n_out, int M, int Mp, double *acc, hipStream_t s);
+hipError_t spmm_only(KrylovWork &w, hipStream_t s);              // v = H u
+hipError_t expm_single(int m, const double2 *A, double2 *E, hipStream_t s);
+
+} // namespace spc

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP path against the CPU oracle and the dense model.
+
+Tolerances: Hamiltonian entries and single products to 1e-14 relative;
+populations and yields to 1e-10 relative (the bar BASELINE sets for yields)."""
+import numpy as np
+import pytest
+
+import dense_model as dm
+from helpers import max_rel, oracle_system
+from oracle import pyoracle
+from spin_dynamics_amd import _lib, dynamics, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense(h):
+    return h.scipy().toarray()
+
+
+def test_hamiltonian_isotropic_matches_reference_builder(ctx):
+    w = workloads.config1()
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    ref = pyoracle.build_iso_csr(oracle_system(w.sys, w.B))
+    assert H.N == 192 and H.nnz <= ref.nnz
+    assert max_rel(H.scipy().toarray(), ref.scipy().toarray()) < 1e-14
+
+
+def test_hamiltonian_anisotropic_tilted_matches_oracle_and_dense(ctx):
+    w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3))
+    s = oracle_system(w.sys, w.B)
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    rowptr, col, val = H.csr()
+    g = pyoracle.build_general_csr(s).scipy()
+    assert np.array_equal(rowptr, g.indptr) and np.array_equal(col, g.indices)
+    assert max_rel(val, g.data) < 1e-14
+    Hd = dm.hamiltonian(s.mult1, s.mult2, s.tensors(), B=s.B, direction=s.direction, J=s.J, D=s.D, kS=s.kS, kT=s.kT)
+    assert max_rel(H.scipy().toarray(), Hd) < 1e-14
+
+
+@pytest.mark.parametrize("M", [1, 64, 70])
+def test_spmm(ctx, M):
+    w = workloads.small_anisotropic()
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    rng = np.random.default_rng(M)
+    X = rng.normal(size=(M, H.N)) + 1j * rng.normal(size=(M, H.N))
+    Y = H.spmm(X)
+    ref = (H.scipy() @ X.T).T
+    assert max_rel(Y, ref) < 1e-14
+
+
+@pytest.mark.parametrize("m,scale", [(1, 0.3), (5, 0.01), (12, 1.0), (20, 4.0), (24, 40.0), (32, 2.0)])
+def test_expm_small(ctx, m, scale):
+    from scipy.linalg import expm
+    rng = np.random.default_rng(m)
+    A = (rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))) * scale / m
+    E = dynamics.expm(A, ctx)
+    assert max_rel(E, pyoracle.expm(A)) < 1e-13
+    assert max_rel(E, expm(A)) < 1e-12
+
+
+def test_krylov_basis_matches_mgs(ctx):
+    w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 2))
+    s = oracle_system(w.sys, w.B)
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    ref_h = pyoracle.build_general_csr(s)
+    rng = np.random.default_rng(3)
+    M, m = 5, 16
+    psi = rng.normal(size=(M, H.N)) + 1j * rng.normal(size=(M, H.N))
+    Q, A, beta = H.krylov(psi, m)
+    Hs = H.scipy()
+    for r in range(M):
+        Qr, cr, Ar = pyoracle.get_krylov(ref_h, psi[r], m)
+        assert abs(beta[r] - cr[0].real) < 1e-13 * abs(cr[0])
+        assert max_rel(A[r], Ar) < 1e-11
+        Qm = Q[r].T  # N x m
+        assert np.max(np.abs(Qm.conj().T @ Qm - np.eye(m))) < 1e-13
+        # Arnoldi relation H Q_{m-1} = Q_m A[:, :m-1]
+        assert max_rel(Hs @ Qm[:, :m - 1], Qm @ A[r][:, :m - 1]) < 1e-12
+
+
+@pytest.mark.parametrize("M,tol", [(8, 0.0), (70, 0.0)])
+def test_trace_sampling_injected_vs_oracle(ctx, M, tol):
+    w = workloads.small_anisotropic(M=M, tol=tol)
+    s = oracle_system(w.sys, w.B)
+    Z = s.Z
+    vecs = np.array([pyoracle.sample_vector(sd, Z, 1) for sd in pyoracle.sample_seeds((7, 11), M)])
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), Z, M=M, m=w.sim.N_krylov, t_end=w.sim.t_end,
+                                  dt=w.sim.dt, tol=tol, vec_kind=2, injected=vecs)
+    for k in ("P_S", "P_T0", "P_Tp", "P_Tm"):
+        assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
+    _, phs, _ = pyoracle.get_kinetics(ref["P_S"], ref["P_T0"], ref["P_Tp"], ref["P_Tm"], w.sim.dt, s.kS, s.kT)
+    assert abs(res.Phi_S[-1] - phs[-1]) < 1e-10 * abs(phs[-1])
+
+
+def test_trace_sampling_tol_gram_mode(ctx):
+    # tol > 0: several steps per Krylov basis, populations from projector Grams.
+    # Rebuild decisions compare |c_m|^2 with tol |c|^2; a rounding-level flip
+    # would move results by ~tol, so the bound is loose where a flip is possible.
+    w = workloads.small_anisotropic(M=6, tol=1e-8)
+    s = oracle_system(w.sys, w.B)
+    vecs = np.array([pyoracle.sample_vector(sd, s.Z, 1) for sd in pyoracle.sample_seeds((5, 6), 6)])
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=6, m=w.sim.N_krylov, t_end=w.sim.t_end,
+                                  dt=w.sim.dt, tol=1e-8, vec_kind=2, injected=vecs)
+    assert max_rel(res.P_S, ref["P_S"]) < 1e-9
+    _, rebuilds, steps = dynamics.last_run_stats(ctx)
+    assert rebuilds < steps
+
+
+@pytest.mark.parametrize("kind", ["phase", "SUZ"])
+def test_config1_vs_oracle_fixture(ctx, kind):
+    """Config 1 (N=192, M=32, m=20, T=5 us) with on-device trace vectors."""
+    import os
+    w = workloads.config1(vec_kind=kind)
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", f"oracle_config1_{kind}.npz"))
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    for k in ("P_S", "P_T0", "P_Tp", "P_Tm"):
+        assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
+    assert abs(res.Phi_S[-1] - ref["Phi_S"][-1]) < 1e-10 * abs(ref["Phi_S"][-1])
+
+
+def test_exact_dynamics_vs_oracle(ctx):
+    w = workloads.small_anisotropic(m1=(2,), m2=(3,), T_ns=150.0)
+    s = oracle_system(w.sys, w.B)
+    res = dynamics.exact_dynamics(w.sys, w.sim, B=w.B, ctx=ctx)
+    ref = pyoracle.exact_dynamics(pyoracle.build_general_csr(s), s.Z, m=w.sim.N_krylov, t_end=w.sim.t_end,
+                                  dt=w.sim.dt)
+    assert max_rel(res.P_S, ref["P_S"]) < 1e-10
+    # exact trace against dense expm propagation, averaged over basis states
+    Hd = dm.hamiltonian(s.mult1, s.mult2, s.tensors(), B=s.B, direction=s.direction, J=s.J, D=s.D, kS=s.kS, kT=s.kT)
+    K = pyoracle.steps_taken(w.sim.t_end, w.sim.dt)
+    acc = 0
+    for i in range(s.Z):
+        e = np.zeros(s.Z, complex)
+        e[i] = 1
+        acc = acc + dm.propagate_dense(Hd, np.kron(dm.theta(), e), w.sim.dt, K)
+    assert max_rel(res.P_S[:K + 1], acc[0] / s.Z) < 1e-10
+
+
+def test_errors_are_reported(ctx):
+    w = workloads.small_anisotropic()
+    w.sys.e1.g_I = [4, 2]
+    with pytest.raises(_lib.SpinchemError) as ei:
+        dynamics.Hamiltonian(w.sys, w.B, ctx)
+    assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED and "must be 3 or 2" in str(ei.value)
+    w = workloads.small_anisotropic()
+    w.sim.N_krylov = 33
+    with pytest.raises(_lib.SpinchemError) as ei:
+        dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED
