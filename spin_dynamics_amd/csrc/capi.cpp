@@ -47,6 +47,9 @@ struct spc_ctx {
     DevSystem *d_sys = nullptr;
     int32_t *d_rowptr = nullptr, *d_col = nullptr;
     double2 *d_val = nullptr;
+    int32_t *d_browptr = nullptr, *d_bcol = nullptr; // 4x4 blocks over nuclear indices
+    double2 *d_bval = nullptr;
+    int64_t nblocks = 0;
     // workspace
     std::map<std::string, DevBuf> bufs;
     // stats
@@ -131,6 +134,9 @@ struct HostRng {
         jump();
     }
 };
+
+// Reference basis index e*Z + nuc -> device row 4*nuc + e.
+inline int64_t dev_row(int64_t n, int64_t Z) { return 4 * (n % Z) + n / Z; }
 
 int64_t steps_taken(double t_end, double dt)
 {
@@ -232,18 +238,20 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     w.rowptr = ctx->d_rowptr;
     w.col = ctx->d_col;
     w.val = ctx->d_val;
+    w.browptr = ctx->d_browptr;
+    w.bcol = ctx->d_bcol;
+    w.bval = ctx->d_bval;
     w.nnz = ctx->nnz;
     w.prof = &ctx->prof;
     ctx->prof.clear();
-    // rows per wave: about 2048 workgroups of 4 waves per pass
+    w.kS = ctx->kS;
+    w.kT = ctx->kT;
+    // launch geometry: the sparse product in small row blocks (gathers stay in
+    // L2), the streaming passes in about 512 blocks per pass
     const int nch = Mp / 64;
-    auto rpw_for = [&](int64_t rows, int64_t target) {
-        int64_t rows_per_wg = std::max<int64_t>(4, (rows * nch + target - 1) / target);
-        return (int)std::max<int64_t>(1, (rows_per_wg + 3) / 4);
-    };
-    w.rpw1 = rpw_for(N, 1024);
-    w.rpw2 = rpw_for(N, 2048);
-    w.rpwp = rpw_for(Z, 1024);
+    w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
+    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     const bool need_gram = sim->tol > 0.0;
 
     if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
@@ -254,7 +262,8 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
     if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
     w.part_capacity = part_need;
-    if ((rc = ensure(ctx, "dots", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
     if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
     if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
     if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
@@ -410,6 +419,9 @@ void spc_ctx_destroy(spc_ctx *ctx)
     hipFree(ctx->d_rowptr);
     hipFree(ctx->d_col);
     hipFree(ctx->d_val);
+    hipFree(ctx->d_browptr);
+    hipFree(ctx->d_bcol);
+    hipFree(ctx->d_bval);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -504,7 +516,27 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     HIP_TRY(hipMalloc(&ctx->d_col, sizeof(int32_t) * std::max(nnz, 1)));
     HIP_TRY(hipMalloc(&ctx->d_val, sizeof(double2) * std::max(nnz, 1)));
     HIP_TRY(h_fill(ctx->d_sys, N, ctx->d_rowptr, ctx->d_col, ctx->d_val, s));
+    // the same matrix in 4x4 blocks over nuclear indices (the product's format)
+    hipFree(ctx->d_browptr);
+    ctx->d_browptr = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_browptr, sizeof(int32_t) * (Z + 1)));
+    HIP_TRY(h_block_count(ctx->d_rowptr, ctx->d_col, Z, counts, s));
+    size_t tmp2 = 0;
+    HIP_TRY(h_scan(counts, ctx->d_browptr, Z, nullptr, tmp2, s));
+    if ((rc = ensure(ctx, "scan_tmp", std::max(tmp2, tmp_bytes), &tmp))) return rc;
+    HIP_TRY(h_scan(counts, ctx->d_browptr, Z, tmp, tmp2, s));
+    int32_t nb = 0;
+    HIP_TRY(hipMemcpyAsync(&nb, ctx->d_browptr + Z, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    hipFree(ctx->d_bcol);
+    hipFree(ctx->d_bval);
+    ctx->d_bcol = nullptr;
+    ctx->d_bval = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_bcol, sizeof(int32_t) * std::max(nb, 1)));
+    HIP_TRY(hipMalloc(&ctx->d_bval, sizeof(double2) * 16 * std::max(nb, 1)));
+    HIP_TRY(h_block_fill(ctx->d_rowptr, ctx->d_col, ctx->d_val, Z, ctx->d_browptr, ctx->d_bcol, ctx->d_bval, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx->nblocks = nb;
     ctx->hsys = S;
     ctx->kS = sys->kS;
     ctx->kT = sys->kT;
@@ -528,9 +560,34 @@ int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *va
 {
     if (!ctx || !ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set");
     HIP_TRY(hipSetDevice(ctx->device));
-    if (rowptr) HIP_TRY(hipMemcpy(rowptr, ctx->d_rowptr, sizeof(int32_t) * (ctx->N + 1), hipMemcpyDeviceToHost));
-    if (col) HIP_TRY(hipMemcpy(col, ctx->d_col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
-    if (val) HIP_TRY(hipMemcpy(val, ctx->d_val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
+    // device CSR is in the electron-fastest order; return the reference order
+    const int64_t N = ctx->N, Z = ctx->Z;
+    std::vector<int32_t> rp(N + 1), cl(std::max<int64_t>(ctx->nnz, 1));
+    std::vector<double2> vl(std::max<int64_t>(ctx->nnz, 1));
+    HIP_TRY(hipMemcpy(rp.data(), ctx->d_rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cl.data(), ctx->d_col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vl.data(), ctx->d_val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
+    std::vector<std::pair<int32_t, double2>> row;
+    int64_t pos = 0;
+    for (int64_t n = 0; n < N; ++n) {
+        const int64_t d = dev_row(n, Z);
+        row.clear();
+        for (int32_t e = rp[d]; e < rp[d + 1]; ++e) {
+            const int32_t c = cl[e];
+            row.push_back({(int32_t)((c & 3) * Z + (c >> 2)), vl[e]});
+        }
+        std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        if (rowptr) rowptr[n] = (int32_t)pos;
+        for (const auto &x : row) {
+            if (col) col[pos] = x.first;
+            if (val) {
+                val[2 * pos] = x.second.x;
+                val[2 * pos + 1] = x.second.y;
+            }
+            ++pos;
+        }
+    }
+    if (rowptr) rowptr[N] = (int32_t)pos;
     return SPC_OK;
 }
 
@@ -607,6 +664,12 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     w.rowptr = ctx->d_rowptr;
     w.col = ctx->d_col;
     w.val = ctx->d_val;
+    w.browptr = ctx->d_browptr;
+    w.bcol = ctx->d_bcol;
+    w.bval = ctx->d_bval;
+    w.nnz = ctx->nnz;
+    w.kS = ctx->kS;
+    w.kT = ctx->kT;
     w.rpw1 = w.rpw2 = w.rpwp = 4;
     int rc;
     if ((rc = ensure(ctx, "Q", (size_t)w.m * w.N * Mp, &w.Q))) return rc;
@@ -617,6 +680,7 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     if ((rc = ensure(ctx, "part", pn, &w.part))) return rc;
     w.part_capacity = pn;
     if ((rc = ensure(ctx, "dots", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
     if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
     if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
     if ((rc = ensure(ctx, "hess", (size_t)w.m * w.m * Mp, &w.hess))) return rc;
@@ -625,12 +689,14 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     return SPC_OK;
 }
 
-// host [M][N] <-> device lane-fastest [N][Mp]
+// host [M][N] (reference order) <-> device lane-fastest [N][Mp] (device order)
 static void to_lanes(const double *x, int M, int Mp, int64_t N, std::vector<double2> &out)
 {
+    const int64_t Z = N / 4;
     out.assign((size_t)N * Mp, make_double2(0, 0));
     for (int r = 0; r < M; ++r)
-        for (int64_t n = 0; n < N; ++n) out[n * Mp + r] = make_double2(x[2 * (r * N + n)], x[2 * (r * N + n) + 1]);
+        for (int64_t n = 0; n < N; ++n)
+            out[dev_row(n, Z) * Mp + r] = make_double2(x[2 * (r * N + n)], x[2 * (r * N + n) + 1]);
 }
 
 int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y)
@@ -648,8 +714,8 @@ int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y)
     HIP_TRY(hipMemcpy(h.data(), w.v, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
     for (int r = 0; r < M; ++r)
         for (int64_t n = 0; n < w.N; ++n) {
-            y[2 * (r * w.N + n)] = h[n * w.Mp + r].x;
-            y[2 * (r * w.N + n) + 1] = h[n * w.Mp + r].y;
+            y[2 * (r * w.N + n)] = h[dev_row(n, w.Z) * w.Mp + r].x;
+            y[2 * (r * w.N + n) + 1] = h[dev_row(n, w.Z) * w.Mp + r].y;
         }
     return SPC_OK;
 }
@@ -677,7 +743,7 @@ int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, doub
         for (int r = 0; r < M; ++r)
             for (int j = 0; j < m; ++j)
                 for (int64_t n = 0; n < w.N; ++n) {
-                    const double2 z = q[((size_t)j * w.N + n) * w.Mp + r];
+                    const double2 z = q[((size_t)j * w.N + dev_row(n, w.Z)) * w.Mp + r];
                     const size_t o = ((size_t)r * m + j) * w.N + n;
                     Q[2 * o] = z.x;
                     Q[2 * o + 1] = z.y;

@@ -4,9 +4,10 @@
 // The reference builds H from Kronecker products of spin-operator factors
 // (spinop.f90:58-95, kroenecker.f90:56-200); here each row is generated from
 // the digits of its basis index: a term acting on a set of spins connects the
-// row to the column whose digits differ on exactly that set, so the entries of
-// a row come out grouped by electron block and, inside a block, by nuclear
-// offset -- already in ascending column order.
+// row to the column whose digits differ on exactly that set.  The device basis
+// keeps the electron pair as the fastest index (row = 4*nuc + e, the
+// reference's e*Z + nuc permuted), so electron flips stay within one 4-row
+// group and the gathers of the sparse product stay local.
 //
 // H = -B n.(S1+S2) + sum_k [S_e(k).A_k.I_k + zeta_k n.I_k] + S1.T.S2 - i kbar/2
 // with T = (2J + 2i dk) 1 + D, dk = (kS - kT)/4, kbar = (kS + 3kT)/4; this is
@@ -90,13 +91,15 @@ __device__ double2 enuc_value(const DevSystem &S, int k, int se, int d, int dc)
     return cscale(nuc_ladder(S.mult[k], d, dc), c);
 }
 
-// Emits the row's entries in ascending column order.
+// Emits the row's entries in ascending column order.  Device basis order is
+// electron-fastest: row = 4*nuc + e with e = 2*s1 + s2 (0 = alpha), so the
+// entries of a row come as nuclear offsets nuc - s_k (k ascending), nuc,
+// nuc + s_k (k descending), and within one nuclear column by electron state.
 template <class Emit>
 __device__ void gen_row(const DevSystem &S, int64_t row, Emit &&emit)
 {
-    const int64_t Z = S.Z;
-    const int e = (int)(row / Z);
-    const int64_t nuc = row - (int64_t)e * Z;
+    const int e = (int)(row & 3);
+    const int64_t nuc = row >> 2;
     const int s1 = e >> 1, s2 = e & 1;
     const double m1 = 0.5 - s1, m2 = 0.5 - s2;
     int d[kMaxNuclei];
@@ -105,29 +108,32 @@ __device__ void gen_row(const DevSystem &S, int64_t row, Emit &&emit)
         d[k] = (int)((nuc / S.stride[k]) % S.mult[k]);
         mk[k] = 0.5 * (S.mult[k] - 1) - d[k];
     }
-    const int e1f = 2 * (1 - s1) + s2, e2f = 2 * s1 + (1 - s2), ebf = 2 * (1 - s1) + (1 - s2);
+    const int e1f = 2 * (1 - s1) + s2, e2f = 2 * s1 + (1 - s2);
 
-    for (int ec = 0; ec < 4; ++ec) {
-        int kind; // 0 same, 1 e1 flip, 2 e2 flip, 3 both
-        if (ec == e) kind = 0;
-        else if (ec == e1f) kind = 1;
-        else if (ec == e2f) kind = 2;
-        else kind = 3;
-        const int64_t base = (int64_t)ec * Z;
-        int k0 = 0, k1 = 0; // nuclei that flip together with this electron block change
-        if (kind == 0) { k0 = 0; k1 = S.nn; }
-        else if (kind == 1) { k0 = 0; k1 = S.n1; }
-        else if (kind == 2) { k0 = S.n1; k1 = S.nn; }
-        // column nuc - stride_k (dc = d - 1), k ascending = descending stride
-        for (int k = k0; k < k1; ++k) {
-            if (d[k] < 1) continue;
-            double me = k < S.n1 ? m1 : m2;
-            int se = k < S.n1 ? s1 : s2;
-            double2 v = kind == 0 ? nuc_value(S, k, d[k], d[k] - 1, me) : enuc_value(S, k, se, d[k], d[k] - 1);
-            if (nonzero(v)) emit(base + nuc - S.stride[k], v);
+    // nuclear flip of nucleus k to digit dc: the nucleus alone (electron e) and
+    // together with its electron (state ef), emitted in electron order
+    auto flip_pair = [&](int k, int dc, int64_t nuc_c) {
+        const int el = k < S.n1 ? 0 : 1;
+        const int se = el == 0 ? s1 : s2;
+        const int ef = el == 0 ? e1f : e2f;
+        const double2 vn = nuc_value(S, k, d[k], dc, el == 0 ? m1 : m2);
+        const double2 ve = enuc_value(S, k, se, d[k], dc);
+        const int64_t b = 4 * nuc_c;
+        if (e < ef) {
+            if (nonzero(vn)) emit(b + e, vn);
+            if (nonzero(ve)) emit(b + ef, ve);
+        } else {
+            if (nonzero(ve)) emit(b + ef, ve);
+            if (nonzero(vn)) emit(b + e, vn);
         }
-        // centre column (same nuclear index)
-        if (kind == 0) {
+    };
+
+    for (int k = 0; k < S.nn; ++k) // nuc - stride_k, descending stride
+        if (d[k] >= 1) flip_pair(k, d[k] - 1, nuc - S.stride[k]);
+
+    for (int ec = 0; ec < 4; ++ec) { // same nuclear index
+        const int64_t col = 4 * nuc + ec;
+        if (ec == e) {
             double2 v = cz(0.0, -0.5 * S.kbar);
             v = cadd(v, cz(S.w[2] * (m1 + m2), 0.0));
             v = cadd(v, cscale(m1 * m2, Tab(S, 2, 2)));
@@ -135,31 +141,28 @@ __device__ void gen_row(const DevSystem &S, int64_t row, Emit &&emit)
                 double me = k < S.n1 ? m1 : m2;
                 v.x = fma(S.A[k][8] * me + S.zeta[k] * S.dir[2], mk[k], v.x);
             }
-            emit(row, v); // the diagonal is always stored
-        } else if (kind == 1 || kind == 2) {
-            int sr = kind == 1 ? s1 : s2;
-            double mo = kind == 1 ? m2 : m1;
+            emit(col, v); // the diagonal is always stored
+        } else if (ec == e1f || ec == e2f) {
+            const bool first = ec == e1f;
+            const int sr = first ? s1 : s2;
+            const double mo = first ? m2 : m1;
+            const int k0 = first ? 0 : S.n1, k1 = first ? S.n1 : S.nn;
             double2 f[2];
             for (int a = 0; a < 2; ++a) {
                 f[a] = cz(S.w[a], 0.0);
-                f[a] = cadd(f[a], cscale(mo, kind == 1 ? Tab(S, a, 2) : Tab(S, 2, a)));
+                f[a] = cadd(f[a], cscale(mo, first ? Tab(S, a, 2) : Tab(S, 2, a)));
                 for (int k = k0; k < k1; ++k) f[a].x = fma(S.A[k][3 * 2 + a], mk[k], f[a].x);
             }
             const double2 v = ladder1(f[0], f[1], sr == 0 ? 1 : -1);
-            if (nonzero(v)) emit(base + nuc, v);
+            if (nonzero(v)) emit(col, v);
         } else {
             const double2 v = ladder2(S.Tre, S.Tim, s1 == 0 ? 1 : -1, s2 == 0 ? 1 : -1);
-            if (nonzero(v)) emit(base + nuc, v);
-        }
-        // column nuc + stride_k (dc = d + 1), k descending = ascending stride
-        for (int k = k1 - 1; k >= k0; --k) {
-            if (d[k] + 1 >= S.mult[k]) continue;
-            double me = k < S.n1 ? m1 : m2;
-            int se = k < S.n1 ? s1 : s2;
-            double2 v = kind == 0 ? nuc_value(S, k, d[k], d[k] + 1, me) : enuc_value(S, k, se, d[k], d[k] + 1);
-            if (nonzero(v)) emit(base + nuc + S.stride[k], v);
+            if (nonzero(v)) emit(col, v);
         }
     }
+
+    for (int k = S.nn - 1; k >= 0; --k) // nuc + stride_k, ascending stride
+        if (d[k] + 1 < S.mult[k]) flip_pair(k, d[k] + 1, nuc + S.stride[k]);
 }
 
 __global__ void k_h_count(const DevSystem *__restrict__ sys, int64_t N, int32_t *__restrict__ counts)
@@ -187,7 +190,79 @@ __global__ void k_h_fill(const DevSystem *__restrict__ sys, int64_t N, const int
     });
 }
 
+// Block row n merges CSR rows 4n..4n+3 (each sorted by column, so by nuclear
+// column nc = col >> 2): the union of their nc is the block columns.
+__global__ void k_block_count(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int64_t Z,
+                              int32_t *__restrict__ bcounts)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n > Z) return;
+    if (n == Z) {
+        bcounts[Z] = 0;
+        return;
+    }
+    int p[4], pe[4];
+    for (int e = 0; e < 4; ++e) {
+        p[e] = rowptr[4 * n + e];
+        pe[e] = rowptr[4 * n + e + 1];
+    }
+    int cnt = 0;
+    for (;;) {
+        int best = INT32_MAX;
+        for (int e = 0; e < 4; ++e)
+            if (p[e] < pe[e]) best = min(best, col[p[e]] >> 2);
+        if (best == INT32_MAX) break;
+        for (int e = 0; e < 4; ++e)
+            while (p[e] < pe[e] && (col[p[e]] >> 2) == best) ++p[e];
+        ++cnt;
+    }
+    bcounts[n] = cnt;
+}
+
+__global__ void k_block_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                             const double2 *__restrict__ val, int64_t Z, const int32_t *__restrict__ browptr,
+                             int32_t *__restrict__ bcol, double2 *__restrict__ bval)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Z) return;
+    int p[4], pe[4];
+    for (int e = 0; e < 4; ++e) {
+        p[e] = rowptr[4 * n + e];
+        pe[e] = rowptr[4 * n + e + 1];
+    }
+    int b = browptr[n];
+    for (;;) {
+        int best = INT32_MAX;
+        for (int e = 0; e < 4; ++e)
+            if (p[e] < pe[e]) best = min(best, col[p[e]] >> 2);
+        if (best == INT32_MAX) break;
+        bcol[b] = best;
+        double2 *B = bval + (int64_t)b * 16;
+        for (int q = 0; q < 16; ++q) B[q] = cz(0, 0);
+        for (int e = 0; e < 4; ++e)
+            while (p[e] < pe[e] && (col[p[e]] >> 2) == best) {
+                B[4 * e + (col[p[e]] & 3)] = val[p[e]];
+                ++p[e];
+            }
+        ++b;
+    }
+}
+
 } // namespace
+
+hipError_t h_block_count(const int32_t *rowptr, const int32_t *col, int64_t Z, int32_t *bcounts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_block_count, dim3((unsigned)((Z + 1 + 255) / 256)), dim3(256), 0, s, rowptr, col, Z, bcounts);
+    return hipGetLastError();
+}
+
+hipError_t h_block_fill(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
+                        const int32_t *browptr, int32_t *bcol, double2 *bval, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_block_fill, dim3((unsigned)((Z + 255) / 256)), dim3(256), 0, s, rowptr, col, val, Z, browptr,
+                       bcol, bval);
+    return hipGetLastError();
+}
 
 hipError_t h_count(const DevSystem *d_sys, int64_t N, int32_t *counts, hipStream_t s)
 {

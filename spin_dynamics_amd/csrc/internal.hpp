@@ -31,6 +31,11 @@ hipError_t h_scan(const int32_t *counts, int32_t *rowptr, int64_t N, void *tmp, 
                   hipStream_t s);
 hipError_t h_fill(const DevSystem *d_sys, int64_t N, const int32_t *rowptr, int32_t *col, double2 *val,
                   hipStream_t s);
+// 4x4 blocks over nuclear indices from the CSR: bcounts[Z+1], then browptr by
+// the same scan, then bcol[nb] and bval[nb][16].
+hipError_t h_block_count(const int32_t *rowptr, const int32_t *col, int64_t Z, int32_t *bcounts, hipStream_t s);
+hipError_t h_block_fill(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
+                        const int32_t *browptr, int32_t *bcol, double2 *bval, hipStream_t s);
 
 // ------------------------------------------------------------- sampling.hip
 struct VecInit {
@@ -91,15 +96,19 @@ struct Prof {
 struct KrylovWork {
     int64_t N, Z;
     int M, Mp, m;
-    const int32_t *rowptr;
+    const int32_t *rowptr; // CSR of H (device basis order)
     const int32_t *col;
     const double2 *val;
+    const int32_t *browptr; // the same H in 4x4 blocks over nuclear indices
+    const int32_t *bcol;
+    const double2 *bval;     // [nblocks][4][4]
     double2 *Q;      // [m][N][Mp]
     double2 *u;      // [N][Mp]
     double2 *v;      // [N][Mp]
     double2 *psi;    // [N][Mp]
     double2 *part;   // partial sums [nrb][nacc][Mp]
-    double2 *dots;   // reduced sums [nacc][Mp]
+    double2 *dots;   // reduced update-pass sums [2*kMaxKrylov + 2][Mp]
+    double2 *dots1;  // reduced product-pass sums [2][Mp] (also the populations)
     double2 *coef;   // pass-2 coefficients [2*kMaxKrylov + 4][Mp]
     double2 *tprev;  // tentative column [kMaxKrylov + 1][Mp]
     double2 *hess;   // Hessenberg [m][m][Mp] (col-major per lane)
@@ -113,8 +122,11 @@ struct KrylovWork {
     double *obs;     // [n_out][4][Mp]
     int64_t n_out;
     size_t part_capacity; // elements of part
-    int rpw1, rpw2, rpwp;  // rows per wave for pass 1, pass 2 and psi update
+    int rpw1;              // nuclear indices per wave of the sparse product
+    int rpw2;              // nuclear indices per block of the update pass
+    int rpwp;              // nuclear indices per wave of the psi pass
     int64_t nnz;           // Hamiltonian entries (algorithmic bytes of the product)
+    double kS, kT;         // recombination rates (mT units), for Q^H H u via H^H = H + iK
     Prof *prof;            // optional kernel timing
 };
 

@@ -2,26 +2,31 @@
 //
 // Replaces get_krylov / expm / convert_to_krylov and the inner step loop of
 // qm_dynamics.f90:340-366.  All vectors share H, so the M trace vectors are the
-// lane dimension: every basis vector is an [N][Mp] block, lane r = trace vector
-// r, one 1 KiB row segment per wavefront.  The sparse product gathers whole
-// rows (one scalar-loaded H entry, 64 lanes of x), and the Gram-Schmidt dot
-// products need no cross-lane reduction: each lane sums its own vector over the
-// rows it visits, waves of a workgroup combine through LDS, workgroups through
-// a deterministic second pass.
+// lane dimension: every basis vector is an [N][Mp] block in the device basis
+// order (row = 4*nuc + e), lane r = trace vector r, one 1 KiB row segment per
+// wavefront.  The sparse product gathers whole rows (one scalar-loaded H entry,
+// 64 lanes of x), and the Gram-Schmidt dot products need no cross-lane
+// reduction: each lane sums its own vector over the rows it visits, waves of a
+// workgroup combine through LDS, workgroups through a deterministic tree pass.
 //
-// Orthogonalisation: the reference uses modified Gram-Schmidt (i dependent
-// reductions per column).  Here each column costs two streaming passes and one
-// reduction, with classical Gram-Schmidt plus a delayed re-orthogonalisation
-// (the "DCGS2" form of Swirydowicz et al., 2020):
-//   pass 1 (column i): v = H u_i  and  a = Q^H u_i, b = Q^H v, c = u_i^H v,
-//                      d = u_i^H u_i
-//   coefficients:      beta_i^2 = d - |a|^2, column i-1 corrected by a,
-//                      w_i = H q_i = (v - Q g - g_i q_i)/beta_i via the
-//                      Arnoldi relation (g = Hbar a), t_i = Q^H w_i from a,b,c
-//   pass 2:            q_i = (u_i - Q a)/beta_i,  u_{i+1} = w_i - Q t_i
-// A last dots pass corrects column m-1.  The result is the reference's
+// Orthogonalisation.  The reference uses modified Gram-Schmidt (i dependent
+// reductions per column).  Here each column costs one sparse product and one
+// streaming pass over the basis, with classical Gram-Schmidt plus a delayed
+// re-orthogonalisation (the "DCGS2" form of Swirydowicz et al., 2020):
+//   product  (column i): v = H u_i,  c = u_i^H v,  d = u_i^H u_i
+//   coefficients:        a = Q^H u_i and s = <P_S Q, P_S u_i> come from the
+//                        previous update pass; beta_i^2 = d - |a|^2; column i-1
+//                        corrected by a; b = Q^H v from the Arnoldi relation
+//                        H^H = H + iK, K = kT + (kS - kT) P_S (Haberkorn):
+//                        b_k = sum_l conj(h_lk) a_l - i(kT a_k + (kS-kT) s_k);
+//                        w_i = H q_i = (v - Q g - g_i q_i)/beta_i (g = Hbar a),
+//                        t_i = Q^H w_i from a, b, c
+//   update:              q_i = (u_i - Q a)/beta_i,  u_{i+1} = w_i - Q t_i,
+//                        and the dots a, s of u_{i+1} against q_0..q_i
+// The last update's dots correct column m-1.  The result is the reference's
 // Hessenberg matrix A (qm_dynamics.f90:432-467) to rounding.
 #include <algorithm>
+#include <array>
 #include <utility>
 
 #include "common.hpp"
@@ -34,156 +39,260 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 
-// --------------------------------------------------------------- pass 1
-// NQ = i previous basis vectors.  SPMM: v = H u plus dots (a, b, c, d);
-// otherwise dots a = Q^H u only (final correction of the last column).
-template <int NQ, bool SPMM>
-__global__ __launch_bounds__(kBlock) void k_pass1(const int32_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ colidx,
-                                                   const double2 *__restrict__ hval,
-                                                   const double2 *__restrict__ Q,
-                                                   const double2 *__restrict__ u, double2 *__restrict__ v,
-                                                   double2 *__restrict__ part, int64_t N, int Mp, int rpw)
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Algorithmic bytes: one complex128 pass over the M useful lanes, and the CSR.
+inline double vec_bytes(const KrylovWork &w) { return 16.0 * (double)w.N * (double)w.M; }
+inline double csr_bytes(const KrylovWork &w) { return 20.0 * (double)w.nnz + 4.0 * (double)(w.N + 1); }
+
+struct ProfScope {
+    const KrylovWork &w;
+    hipStream_t s;
+    bool on;
+    ProfScope(const KrylovWork &w_, int fam, double bytes, hipStream_t s_) : w(w_), s(s_), on(w_.prof && w_.prof->on)
+    {
+        if (on) w.prof->begin(fam, bytes, s);
+    }
+    ~ProfScope()
+    {
+        if (on) w.prof->end(s);
+    }
+};
+
+// Workgroup -> (row block, lane chunk).  Workgroups are dealt round-robin to
+// the 8 XCDs; each XCD takes one contiguous eighth of the row blocks and walks
+// it chunk after chunk, so the rows an XCD gathers from stay in its L2.
+__device__ __forceinline__ void block_map(int bid, int nrb, int nch, int &rb, int &ch)
 {
-    constexpr int NACC = SPMM ? 2 * NQ + 2 : NQ;
-    constexpr int NACC1 = NACC > 0 ? NACC : 1;
-    const int nch = Mp / kWave;
-    const int lid = xcd_remap(blockIdx.x, gridDim.x);
-    const int ch = lid % nch, rb = lid / nch;
+    if ((nrb & 7) == 0) {
+        const int x = bid & 7, j = bid >> 3, per = nrb >> 3;
+        ch = j / per;
+        rb = x * per + j % per;
+    } else {
+        ch = bid / nrb;
+        rb = bid % nrb;
+    }
+}
+
+// Sums NACC per-lane accumulators over the 4 waves of the block into
+// part[rb][j][r]; red must hold (kWaves-1)*8*kWave double2.
+template <int NACC>
+__device__ __forceinline__ void block_reduce(double2 (&acc)[NACC > 0 ? NACC : 1], double2 *red, double2 *part,
+                                             int rb, int r, int lane, int wave, int Mp)
+{
+#pragma unroll
+    for (int base = 0; base < NACC; base += 8) {
+        if (wave > 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (base + j < NACC) red[((wave - 1) * 8 + j) * kWave + lane] = acc[base + j];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (base + j < NACC) {
+                    double2 s = acc[base + j];
+#pragma unroll
+                    for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[(w * 8 + j) * kWave + lane]);
+                    part[((int64_t)rb * NACC + base + j) * Mp + r] = s;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------- sparse product
+// v = H u with H in 4x4 blocks over nuclear indices (block row n = rows
+// 4n..4n+3, block column nc = rows 4nc..4nc+3): a thread computes the 4 rows of
+// its nuclear index, loading each neighbouring nuclear index's 4 rows once.
+// Block values are wave-uniform (scalar loads).  Also c = u^H v, d = |u|^2.
+__global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ browptr, const int32_t *__restrict__ bcol,
+                                                  const double2 *__restrict__ bval, const double2 *__restrict__ u,
+                                                  double2 *__restrict__ v, double2 *__restrict__ part, int64_t Z,
+                                                  int Mp, int rpw, int nrb)
+{
+    __shared__ double2 red[(kWaves - 1) * 8 * kWave];
+    int rb, ch;
+    block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
-    const int64_t NMp = N * (int64_t)Mp;
-
-    double2 acc[NACC1];
+    double2 acc[2] = {cz(0, 0), cz(0, 0)};
+    const int64_t n0 = ((int64_t)rb * kWaves + wave) * rpw;
+    const int64_t n1 = n0 + rpw < Z ? n0 + rpw : Z;
+    for (int64_t n = n0; n < n1; ++n) {
+        const int b0 = browptr[n], b1 = browptr[n + 1];
+        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+        int b = b0;
+        for (; b + 2 <= b1; b += 2) {
+            double2 x[2][4];
 #pragma unroll
-    for (int j = 0; j < NACC1; ++j) acc[j] = cz(0, 0);
-
-    const int64_t row0 = ((int64_t)rb * kWaves + wave) * rpw;
-    const int64_t row1 = row0 + rpw < N ? row0 + rpw : N;
-    for (int64_t row = row0; row < row1; ++row) {
-        const int64_t off = row * Mp + r;
-        const double2 uu = u[off];
-        double2 vv = cz(0, 0);
-        if constexpr (SPMM) {
-            const int e0 = rowptr[row], e1 = rowptr[row + 1];
-            int e = e0;
-            for (; e + 4 <= e1; e += 4) {
-                const int c0 = colidx[e], c1 = colidx[e + 1], c2 = colidx[e + 2], c3 = colidx[e + 3];
-                const double2 x0 = u[(int64_t)c0 * Mp + r], x1 = u[(int64_t)c1 * Mp + r];
-                const double2 x2 = u[(int64_t)c2 * Mp + r], x3 = u[(int64_t)c3 * Mp + r];
-                vv = cfma(hval[e], x0, vv);
-                vv = cfma(hval[e + 1], x1, vv);
-                vv = cfma(hval[e + 2], x2, vv);
-                vv = cfma(hval[e + 3], x3, vv);
+            for (int h = 0; h < 2; ++h) {
+                const int64_t cb = (int64_t)bcol[b + h] * 4 * Mp + r;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[h][e] = u[cb + e * Mp];
             }
-            for (; e < e1; ++e) vv = cfma(hval[e], u[(int64_t)colidx[e] * Mp + r], vv);
-            v[off] = vv;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double2 *B = bval + (int64_t)(b + h) * 16;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) y[e] = cfma(B[4 * e + f], x[h][f], y[e]);
+            }
+        }
+        if (b < b1) {
+            const int64_t cb = (int64_t)bcol[b] * 4 * Mp + r;
+            double2 x[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[e] = u[cb + e * Mp];
+            const double2 *B = bval + (int64_t)b * 16;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) y[e] = cfma(B[4 * e + f], x[f], y[e]);
         }
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            const double2 q = Q[k * NMp + off];
-            acc[k] = cfma_conj(q, uu, acc[k]);
-            if constexpr (SPMM) acc[NQ + k] = cfma_conj(q, vv, acc[NQ + k]);
-        }
-        if constexpr (SPMM) {
-            acc[2 * NQ] = cfma_conj(uu, vv, acc[2 * NQ]);
-            acc[2 * NQ + 1].x += cabs2(uu);
+        for (int e = 0; e < 4; ++e) {
+            const int64_t off = (4 * n + e) * Mp + r;
+            const double2 uu = u[off];
+            v[off] = y[e];
+            acc[0] = cfma_conj(uu, y[e], acc[0]);
+            acc[1].x += cabs2(uu);
         }
     }
-
-    if constexpr (NACC > 0) {
-        __shared__ double2 red[kWaves - 1][8][kWave];
-#pragma unroll
-        for (int base = 0; base < NACC; base += 8) {
-            if (wave > 0) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (base + j < NACC) red[wave - 1][j][lane] = acc[base + j];
-            }
-            __syncthreads();
-            if (wave == 0) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (base + j < NACC) {
-                        double2 s = acc[base + j];
-#pragma unroll
-                        for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][j][lane]);
-                        part[((int64_t)rb * NACC + base + j) * Mp + r] = s;
-                    }
-            }
-            __syncthreads();
-        }
-    }
+    block_reduce<2>(acc, red, part, rb, r, lane, wave, Mp);
 }
 
-// Deterministic reduction of the workgroup partials: dots[j][r] = sum_rb part[rb][j][r].
-__global__ void k_reduce(const double2 *__restrict__ part, double2 *__restrict__ dots, int nacc, int nrb, int Mp)
+// Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].
+__global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ part, double2 *__restrict__ out,
+                                                    int nacc, int nrb, int Mp)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nacc * Mp) return;
-    const int j = (int)(idx / Mp), r = (int)(idx % Mp);
-    double2 s = cz(0, 0);
-    for (int rb = 0; rb < nrb; ++rb) s = cadd(s, part[((int64_t)rb * nacc + j) * Mp + r]);
-    dots[idx] = s;
+    __shared__ double2 red[kWaves][kWave];
+    const int j = blockIdx.x, ch = blockIdx.y;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int r = ch * kWave + lane;
+    double2 s0 = cz(0, 0), s1 = cz(0, 0);
+    int rb = w;
+    for (; rb + kWaves < nrb; rb += 2 * kWaves) {
+        s0 = cadd(s0, part[((int64_t)rb * nacc + j) * Mp + r]);
+        s1 = cadd(s1, part[((int64_t)(rb + kWaves) * nacc + j) * Mp + r]);
+    }
+    for (; rb < nrb; rb += kWaves) s0 = cadd(s0, part[((int64_t)rb * nacc + j) * Mp + r]);
+    red[w][lane] = cadd(s0, s1);
+    __syncthreads();
+    if (w == 0) {
+        double2 s = red[0][lane];
+        for (int q = 1; q < kWaves; ++q) s = cadd(s, red[q][lane]);
+        out[(int64_t)j * Mp + r] = s;
+    }
 }
 
 // ------------------------------------------------------------ coefficients
-// One thread per lane; i = index of the basis vector being finalised.
-__global__ void k_coeffs(int i, int m, int M, int Mp, const double2 *__restrict__ dots, double2 *__restrict__ coef,
-                         double2 *__restrict__ tprev, double2 *__restrict__ hess, double *__restrict__ beta0,
-                         int32_t *__restrict__ flags)
+// Column i of the Arnoldi process for 64 lanes per block, 4 threads per lane
+// (q = thread / 64 takes the indices k = q mod 4).
+//   d1: [c, d] of u_i from the product;  d2: [a_0..a_{i-1}, s_0..s_{i-1}]
+//   from the previous update.  Writes coef = [cq_0..cq_{i-1}, cu_0..cu_{i-1},
+//   cqu, cuu, cuv], tprev = t_i, Hessenberg column i-1, beta0 (i = 0).
+__global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, double kS, double kT,
+                                                    const double2 *__restrict__ d1, const double2 *__restrict__ d2,
+                                                    double2 *__restrict__ coef, double2 *__restrict__ tprev,
+                                                    double2 *__restrict__ hess, double *__restrict__ beta0,
+                                                    int32_t *__restrict__ flags)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= Mp) return;
-    const int ncoef = 2 * i + 3;
-    if (r >= M) {
-        for (int j = 0; j < ncoef; ++j) coef[(int64_t)j * Mp + r] = cz(0, 0);
-        return;
+    __shared__ double2 sb[kMaxKrylov][kWave];
+    __shared__ double red[kWaves][kWave];
+    __shared__ double2 redc[kWaves][kWave];
+    __shared__ double sbeta[kWave];
+    __shared__ double2 sgi[kWave], ss[kWave];
+    const int lane = threadIdx.x & (kWave - 1), q = threadIdx.x / kWave;
+    const int r = blockIdx.x * kWave + lane;
+    const bool active = r < M;
+    const int64_t ld = Mp;
+    auto A = [&](int k) { return d2[(int64_t)k * ld + r]; };
+    auto S = [&](int k) { return d2[(int64_t)(i + k) * ld + r]; };
+    auto H = [&](int row, int col) -> double2 & { return hess[((int64_t)col * m + row) * ld + r]; };
+
+    // beta_i^2 = d - |a|^2
+    double p = 0.0;
+    if (active)
+        for (int k = q; k < i; k += kWaves) p += cabs2(A(k));
+    red[q][lane] = p;
+    __syncthreads();
+    if (q == 0) {
+        double beta2 = active ? d1[ld + r].x : 1.0;
+        for (int w = 0; w < kWaves; ++w) beta2 -= red[w][lane];
+        double beta = 0.0;
+        if (active) {
+            if (!(beta2 > 0.0) || !isfinite(beta2))
+                flags[0] = 1; // Krylov breakdown (the reference divides by zero here)
+            else
+                beta = sqrt(beta2);
+            if (i == 0) beta0[r] = beta;
+        }
+        sbeta[lane] = beta;
     }
-    double2 a[kMaxKrylov + 1], b[kMaxKrylov + 1], g[kMaxKrylov + 1], t[kMaxKrylov + 1];
-    for (int k = 0; k < i; ++k) {
-        a[k] = dots[(int64_t)k * Mp + r];
-        b[k] = dots[(int64_t)(i + k) * Mp + r];
+    __syncthreads();
+    const double beta = sbeta[lane];
+    const bool ok = active && beta > 0.0;
+    const double ib = ok ? 1.0 / beta : 0.0;
+    // finalise column i-1: h(k,i-1) = t_{i-1}[k] + a_k, h(i,i-1) = beta
+    if (ok && i > 0) {
+        for (int k = q; k < i; k += kWaves) H(k, i - 1) = cadd(tprev[(int64_t)k * ld + r], A(k));
+        if (q == 0) H(i, i - 1) = cz(beta, 0.0);
     }
-    const double2 c = dots[(int64_t)(2 * i) * Mp + r];
-    double beta2 = dots[(int64_t)(2 * i + 1) * Mp + r].x;
-    for (int k = 0; k < i; ++k) beta2 -= cabs2(a[k]);
-    if (!(beta2 > 0.0) || !isfinite(beta2)) { // Krylov breakdown (reference: division by 0)
-        flags[0] = 1;
-        for (int j = 0; j < ncoef; ++j) coef[(int64_t)j * Mp + r] = cz(0, 0);
-        return;
+    __syncthreads();
+    // b_k and g_k for k = q mod 4; g_i kept for the last coefficient
+    double2 abp = cz(0, 0);
+    if (ok) {
+        for (int k = q; k <= i; k += kWaves) {
+            double2 g = cz(0, 0); // g_k = sum_{j >= k-1} h(k,j) a_j
+            for (int j = k > 0 ? k - 1 : 0; j < i; ++j) g = cfma(H(k, j), A(j), g);
+            if (k == i) {
+                sgi[lane] = g;
+                continue;
+            }
+            double2 b = cz(0, 0); // (H q_k)^H u_i
+            const int lmax = k + 1 < i ? k + 1 : i - 1;
+            for (int l = 0; l <= lmax; ++l) b = cfma_conj(H(l, k), A(l), b);
+            if (k == i - 1) b.x += beta * beta; // h(i,i-1) q_i^H u_i = beta^2
+            const double2 a = A(k), s = S(k);
+            // - i (kT a_k + (kS - kT) s_k)
+            const double2 ka = cz(kT * a.x + (kS - kT) * s.x, kT * a.y + (kS - kT) * s.y);
+            b = cadd(b, cz(ka.y, -ka.x));
+            sb[k][lane] = b;
+            tprev[(int64_t)k * ld + r] = cscale(ib, csub(b, g));
+            coef[(int64_t)k * ld + r] = cscale(-ib, a);
+            abp = cfma_conj(a, b, abp);
+        }
     }
-    const double beta = sqrt(beta2), ib = 1.0 / beta;
-    auto H = [&](int row, int col) -> double2 & { return hess[((int64_t)col * m + row) * Mp + r]; };
-    if (i == 0) {
-        beta0[r] = beta;
-    } else {
-        for (int k = 0; k < i; ++k) H(k, i - 1) = cadd(tprev[(int64_t)k * Mp + r], a[k]);
-        H(i, i - 1) = cz(beta, 0.0);
-    }
-    for (int l = 0; l <= i; ++l) { // g = Hbar a
+    redc[q][lane] = abp;
+    __syncthreads();
+    if (q == 0) {
         double2 s = cz(0, 0);
-        for (int k = l > 0 ? l - 1 : 0; k < i; ++k) s = cfma(H(l, k), a[k], s);
-        g[l] = s;
+        if (ok) {
+            double2 ab = cz(0, 0);
+            for (int w = 0; w < kWaves; ++w) ab = cadd(ab, redc[w][lane]);
+            const double2 c = d1[r];
+            const double2 gi = sgi[lane];
+            const double2 ti = cscale(ib, csub(cscale(ib, csub(c, ab)), gi));
+            tprev[(int64_t)i * ld + r] = ti;
+            s = cadd(cscale(ib, gi), ti);
+        }
+        ss[lane] = s;
+        coef[(int64_t)(2 * i) * ld + r] = cz(ib, 0.0);
+        coef[(int64_t)(2 * i + 1) * ld + r] = cscale(-ib, s);
+        coef[(int64_t)(2 * i + 2) * ld + r] = cz(ib, 0.0);
     }
-    double2 ab = cz(0, 0);
-    for (int k = 0; k < i; ++k) {
-        t[k] = cscale(ib, csub(b[k], g[k]));
-        ab = cfma_conj(a[k], b[k], ab);
+    __syncthreads();
+    const double2 s = ss[lane];
+    for (int k = q; k < i; k += kWaves) { // cu_k = -(b_k - s a_k)/beta
+        double2 cu = cz(0, 0);
+        if (ok) cu = cscale(-ib, csub(sb[k][lane], cmul(s, A(k))));
+        coef[(int64_t)(i + k) * ld + r] = cu;
+        if (!ok) coef[(int64_t)k * ld + r] = cz(0, 0);
     }
-    t[i] = cscale(ib, csub(cscale(ib, csub(c, ab)), g[i]));
-    const double2 s = cadd(cscale(ib, g[i]), t[i]);
-    for (int k = 0; k < i; ++k) {
-        coef[(int64_t)k * Mp + r] = cscale(-ib, a[k]);
-        double2 x = cadd(cscale(ib, g[k]), t[k]);
-        x = csub(x, cscale(ib, cmul(s, a[k])));
-        coef[(int64_t)(i + k) * Mp + r] = cscale(-1.0, x);
-    }
-    coef[(int64_t)(2 * i) * Mp + r] = cz(ib, 0.0);
-    coef[(int64_t)(2 * i + 1) * Mp + r] = cscale(-ib, s);
-    coef[(int64_t)(2 * i + 2) * Mp + r] = cz(ib, 0.0);
-    for (int k = 0; k <= i; ++k) tprev[(int64_t)k * Mp + r] = t[k];
 }
 
 // Last column: h(k, m-1) = t_{m-1}[k] + a_k.
@@ -198,76 +307,123 @@ __global__ void k_finalize(int m, int M, int Mp, const double2 *__restrict__ dot
     }
 }
 
-// ------------------------------------------------------------------ pass 2
-template <int NQ>
-__global__ __launch_bounds__(kBlock) void k_pass2(const double2 *__restrict__ Q, double2 *__restrict__ qout,
-                                                   double2 *__restrict__ u, const double2 *__restrict__ v,
-                                                   const double2 *__restrict__ coef, int64_t N, int Mp, int rpw)
+// ---------------------------------------------------------------- update
+// NQ = i.  The 4 waves of a block take the 4 electron rows e = wave of one
+// nuclear index n at a time (row 4n + e): q_i = cqu u + sum cq_k q_k and
+// u_{i+1} = cuv v + cuu u + sum cu_k q_k.  The new rows go through LDS, and
+// wave w then forms, for k = w mod 4, the dots of u_{i+1} with q_k over the 4
+// rows (a) and of the singlet parts, rows ab - ba, (s).  LAST: column m-1 only
+// needs a, and u_m is not stored.
+template <int NQ, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q, const double2 *__restrict__ Qre,
+                                                    double2 *__restrict__ qout, double2 *__restrict__ u,
+                                                    const double2 *__restrict__ v, const double2 *__restrict__ coef,
+                                                    double2 *__restrict__ part, int64_t Z, int Mp, int npb, int nrb)
 {
-    const int nch = Mp / kWave;
-    const int lid = xcd_remap(blockIdx.x, gridDim.x);
-    const int ch = lid % nch, rb = lid / nch;
+    constexpr int NC = 2 * NQ + 3;
+    constexpr int NA = NQ + 1;
+    constexpr int NACC = LAST ? NA : 2 * NA;
+    constexpr int NK = (NA + kWaves - 1) / kWaves; // dot indices per wave
+    __shared__ double2 sc[NC * kWave];
+    __shared__ double2 sun[kWaves * kWave], sqi[kWaves * kWave];
+    int rb, ch;
+    block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
-    const int64_t NMp = N * (int64_t)Mp;
-    double2 cq[NQ > 0 ? NQ : 1], cu[NQ > 0 ? NQ : 1];
+    const int64_t NMp = 4 * Z * (int64_t)Mp;
+    for (int idx = threadIdx.x; idx < NC * kWave; idx += kBlock)
+        sc[idx] = coef[(int64_t)(idx / kWave) * Mp + ch * kWave + idx % kWave];
+    __syncthreads();
+    const double2 cqu = sc[(2 * NQ) * kWave + lane], cuu = sc[(2 * NQ + 1) * kWave + lane];
+    const double2 cuv = sc[(2 * NQ + 2) * kWave + lane];
+    double2 aa[NK], as[NK];
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-        cq[k] = coef[(int64_t)k * Mp + r];
-        cu[k] = coef[(int64_t)(NQ + k) * Mp + r];
-    }
-    const double2 cqu = coef[(int64_t)(2 * NQ) * Mp + r];
-    const double2 cuu = coef[(int64_t)(2 * NQ + 1) * Mp + r];
-    const double2 cuv = coef[(int64_t)(2 * NQ + 2) * Mp + r];
-    const int64_t row0 = ((int64_t)rb * kWaves + wave) * rpw;
-    const int64_t row1 = row0 + rpw < N ? row0 + rpw : N;
-    for (int64_t row = row0; row < row1; ++row) {
-        const int64_t off = row * Mp + r;
+    for (int j = 0; j < NK; ++j) aa[j] = as[j] = cz(0, 0);
+
+    const int64_t n0 = (int64_t)rb * npb;
+    const int64_t n1 = n0 + npb < Z ? n0 + npb : Z;
+    for (int64_t n = n0; n < n1; ++n) {
+        // compiler barrier: coefficients are re-read from LDS per row instead
+        // of being held in registers across the loop
+        __asm__ volatile("" ::: "memory");
+        const int64_t off = (4 * n + wave) * Mp + r;
         const double2 uu = u[off], vv = v[off];
         double2 qi = cmul(cqu, uu);
         double2 un = cfma(cuv, vv, cmul(cuu, uu));
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            const double2 q = Q[k * NMp + off];
-            qi = cfma(cq[k], q, qi);
-            un = cfma(cu[k], q, un);
+            const double2 x = Q[k * NMp + off];
+            qi = cfma(sc[k * kWave + lane], x, qi);
+            un = cfma(sc[(NQ + k) * kWave + lane], x, un);
         }
         qout[off] = qi;
-        u[off] = un;
+        if constexpr (!LAST) u[off] = un;
+        sun[wave * kWave + lane] = un;
+        sqi[wave * kWave + lane] = qi;
+        __syncthreads();
+        double2 w4[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w4[e] = sun[e * kWave + lane];
+        const double2 dun = csub(w4[1], w4[2]);
+        const int64_t base = (4 * n) * Mp + r;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int k = wave + kWaves * j;
+            if (k > NQ) break;
+            double2 x[4];
+            if (k < NQ) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[e] = Qre[k * NMp + base + e * Mp];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[e] = sqi[e * kWave + lane];
+            }
+            double2 t = aa[j];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) t = cfma_conj(x[e], w4[e], t);
+            aa[j] = t;
+            if constexpr (!LAST) as[j] = cfma_conj(cscale(0.5, csub(x[1], x[2])), dun, as[j]);
+        }
+        __syncthreads(); // sun/sqi are rewritten for the next nuclear index
+    }
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = wave + kWaves * j;
+        if (k > NQ) break;
+        part[((int64_t)rb * NACC + k) * Mp + r] = aa[j];
+        if constexpr (!LAST) part[((int64_t)rb * NACC + NA + k) * Mp + r] = as[j];
     }
 }
 
 // ----------------------------------------------------------------- psi = Q c
-// Visits nuclear rows n' and the four electron rows e*Z + n'; with OBS the
-// singlet/triplet populations of the new state are summed on the fly:
-// P_Tp = |psi_aa|^2, P_Tm = |psi_bb|^2, P_S = |psi_ab - psi_ba|^2/2,
-// P_T0 = |psi_ab + psi_ba|^2/2 (equal to Re(c^H P_X c) of
-// convert_to_krylov, qm_dynamics.f90:483-496 and 356-359).
+// Per nuclear index: psi rows 4n + e = sum_k c_k q_k, and with OBS the
+// populations of the new state: P_S = |psi_ab - psi_ba|^2/2, P_T0 =
+// |psi_ab + psi_ba|^2/2, P_Tp = |psi_aa|^2, P_Tm = |psi_bb|^2 (equal to
+// Re(c^H P_X c) of convert_to_krylov, qm_dynamics.f90:483-496 and 356-359).
 template <int MK, bool OBS>
 __global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, const double2 *__restrict__ cvec,
                                                  double2 *__restrict__ psi, double2 *__restrict__ part, int64_t Z,
-                                                 int Mp, int rpw)
+                                                 int Mp, int rpw, int nrb)
 {
-    const int nch = Mp / kWave;
-    const int lid = xcd_remap(blockIdx.x, gridDim.x);
-    const int ch = lid % nch, rb = lid / nch;
+    __shared__ double2 red[(kWaves - 1) * 8 * kWave];
+    int rb, ch;
+    block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
-    const int64_t N = 4 * Z;
-    const int64_t NMp = N * (int64_t)Mp;
+    const int64_t NMp = 4 * Z * (int64_t)Mp;
     double2 cc[MK];
 #pragma unroll
     for (int k = 0; k < MK; ++k) cc[k] = cvec[(int64_t)k * Mp + r];
-    double pS = 0, pT0 = 0, pTp = 0, pTm = 0;
+    double2 acc[2] = {cz(0, 0), cz(0, 0)};
     const int64_t n0 = ((int64_t)rb * kWaves + wave) * rpw;
     const int64_t n1 = n0 + rpw < Z ? n0 + rpw : Z;
     for (int64_t n = n0; n < n1; ++n) {
         double2 p[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int64_t off = (e * Z + n) * Mp + r;
+            const int64_t off = (4 * n + e) * Mp + r;
             double2 s = cz(0, 0);
 #pragma unroll
             for (int k = 0; k < MK; ++k) s = cfma(Q[k * NMp + off], cc[k], s);
@@ -275,29 +431,13 @@ __global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, c
             psi[off] = s;
         }
         if constexpr (OBS) {
-            pTp += cabs2(p[0]);
-            pTm += cabs2(p[3]);
-            pS += 0.5 * cabs2(csub(p[1], p[2]));
-            pT0 += 0.5 * cabs2(cadd(p[1], p[2]));
+            acc[0].x += 0.5 * cabs2(csub(p[1], p[2])); // P_S
+            acc[0].y += 0.5 * cabs2(cadd(p[1], p[2])); // P_T0
+            acc[1].x += cabs2(p[0]);                   // P_Tp
+            acc[1].y += cabs2(p[3]);                   // P_Tm
         }
     }
-    if constexpr (OBS) {
-        __shared__ double2 red[kWaves - 1][2][kWave];
-        double2 a0 = cz(pS, pT0), a1 = cz(pTp, pTm);
-        if (wave > 0) {
-            red[wave - 1][0][lane] = a0;
-            red[wave - 1][1][lane] = a1;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            for (int w = 0; w < kWaves - 1; ++w) {
-                a0 = cadd(a0, red[w][0][lane]);
-                a1 = cadd(a1, red[w][1][lane]);
-            }
-            part[((int64_t)rb * 2 + 0) * Mp + r] = a0;
-            part[((int64_t)rb * 2 + 1) * Mp + r] = a1;
-        }
-    }
+    if constexpr (OBS) block_reduce<2>(acc, red, part, rb, r, lane, wave, Mp);
 }
 
 // Writes the reduced populations of each lane at its current output index.
@@ -562,9 +702,9 @@ __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, d
     auto load = [&](int j, int64_t n) -> double2 {
         // x follows the output order: 0 P_S, 1 P_T0, 2 P_Tp, 3 P_Tm
         const double2 *q = Q + (int64_t)j * NMp;
-        if (x == 2) return q[n * Mp + r];
-        if (x == 3) return q[(3 * Z + n) * Mp + r];
-        const double2 ab = q[(Z + n) * Mp + r], ba = q[(2 * Z + n) * Mp + r];
+        if (x == 2) return q[(4 * n) * Mp + r];
+        if (x == 3) return q[(4 * n + 3) * Mp + r];
+        const double2 ab = q[(4 * n + 1) * Mp + r], ba = q[(4 * n + 2) * Mp + r];
         return x == 0 ? cscale(h, csub(ab, ba)) : cscale(h, cadd(ab, ba));
     };
     const int64_t n0 = rb * rows_per_block;
@@ -577,25 +717,19 @@ __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, d
 }
 
 // ---------------------------------------------------------- dispatch tables
-using Pass1Fn = void (*)(const int32_t *, const int32_t *, const double2 *, const double2 *, const double2 *,
-                         double2 *, double2 *, int64_t, int, int);
-using Pass2Fn = void (*)(const double2 *, double2 *, double2 *, const double2 *, const double2 *, int64_t, int, int);
-using PsiFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int);
+using UpdateFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, const double2 *, const double2 *,
+                          double2 *, int64_t, int, int, int);
+using PsiFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int, int);
 
 template <size_t... I>
-constexpr auto make_pass1(std::index_sequence<I...>, std::true_type)
+constexpr auto make_update(std::index_sequence<I...>, std::false_type)
 {
-    return std::array<Pass1Fn, sizeof...(I)>{&k_pass1<(int)I, true>...};
+    return std::array<UpdateFn, sizeof...(I)>{&k_update<(int)I, false>...};
 }
 template <size_t... I>
-constexpr auto make_pass1(std::index_sequence<I...>, std::false_type)
+constexpr auto make_update(std::index_sequence<I...>, std::true_type)
 {
-    return std::array<Pass1Fn, sizeof...(I)>{&k_pass1<(int)I, false>...};
-}
-template <size_t... I>
-constexpr auto make_pass2(std::index_sequence<I...>)
-{
-    return std::array<Pass2Fn, sizeof...(I)>{&k_pass2<(int)I>...};
+    return std::array<UpdateFn, sizeof...(I)>{&k_update<(int)I, true>...};
 }
 template <size_t... I>
 constexpr auto make_psi(std::index_sequence<I...>, std::true_type)
@@ -608,40 +742,27 @@ constexpr auto make_psi(std::index_sequence<I...>, std::false_type)
     return std::array<PsiFn, sizeof...(I)>{&k_psi<(int)I + 1, false>...};
 }
 
-const auto kPass1Spmm = make_pass1(std::make_index_sequence<kMaxKrylov + 1>{}, std::true_type{});
-const auto kPass1Dots = make_pass1(std::make_index_sequence<kMaxKrylov + 1>{}, std::false_type{});
-const auto kPass2 = make_pass2(std::make_index_sequence<kMaxKrylov + 1>{});
+const auto kUpdate = make_update(std::make_index_sequence<kMaxKrylov>{}, std::false_type{});
+const auto kUpdateLast = make_update(std::make_index_sequence<kMaxKrylov>{}, std::true_type{});
 const auto kPsiObs = make_psi(std::make_index_sequence<kMaxKrylov>{}, std::true_type{});
 const auto kPsiPlain = make_psi(std::make_index_sequence<kMaxKrylov>{}, std::false_type{});
 
-inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
-
-// Algorithmic bytes: one complex128 pass over the M useful lanes, and the CSR.
-inline double vec_bytes(const KrylovWork &w) { return 16.0 * (double)w.N * (double)w.M; }
-inline double csr_bytes(const KrylovWork &w) { return 20.0 * (double)w.nnz + 4.0 * (double)(w.N + 1); }
-
-struct ProfScope {
-    const KrylovWork &w;
-    hipStream_t s;
-    bool on;
-    ProfScope(const KrylovWork &w_, int fam, double bytes, hipStream_t s_) : w(w_), s(s_), on(w_.prof && w_.prof->on)
-    {
-        if (on) w.prof->begin(fam, bytes, s);
-    }
-    ~ProfScope()
-    {
-        if (on) w.prof->end(s);
-    }
-};
-
 inline int nrb_for(int64_t rows, int rpw) { return (int)ceil_div(rows, (int64_t)kWaves * rpw); }
 
-hipError_t reduce(const KrylovWork &w, int nacc, int nrb, hipStream_t s)
+hipError_t reduce(const KrylovWork &w, int nacc, int nrb, double2 *out, hipStream_t s)
 {
-    const int64_t n = (int64_t)nacc * w.Mp;
-    if (n == 0) return hipSuccess;
-    ProfScope ps(w, PF_REDUCE, 16.0 * (double)n * (nrb + 1), s);
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, w.part, w.dots, nacc, nrb, w.Mp);
+    if (nacc == 0) return hipSuccess;
+    ProfScope ps(w, PF_REDUCE, 16.0 * (double)nacc * w.Mp * (nrb + 1), s);
+    hipLaunchKernelGGL(k_reduce, dim3(nacc, w.Mp / kWave), dim3(kBlock), 0, s, w.part, out, nacc, nrb, w.Mp);
+    return hipGetLastError();
+}
+
+hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
+{
+    const int nrb1 = nrb_for(w.Z, w.rpw1);
+    ProfScope ps(w, PF_PASS1, 2 * vec_bytes(w) + csr_bytes(w), s);
+    hipLaunchKernelGGL(k_spmm, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.browptr, w.bcol, w.bval, w.u, w.v,
+                       w.part, w.Z, w.Mp, w.rpw1, nrb1);
     return hipGetLastError();
 }
 
@@ -649,60 +770,44 @@ hipError_t reduce(const KrylovWork &w, int nacc, int nrb, hipStream_t s)
 
 size_t partials_needed(const KrylovWork &w)
 {
-    const size_t nrb1 = nrb_for(w.N, w.rpw1), nrbp = nrb_for(w.Z, w.rpwp);
-    size_t need = nrb1 * (size_t)(2 * w.m + 2) * w.Mp;
+    const size_t nrb1 = nrb_for(w.Z, w.rpw1), nrb2 = ceil_div(w.Z, w.rpw2), nrbp = nrb_for(w.Z, w.rpwp);
+    size_t need = nrb1 * 2 * (size_t)w.Mp;
+    need = std::max(need, nrb2 * (size_t)(2 * w.m) * w.Mp);
     need = std::max(need, nrbp * (size_t)2 * w.Mp);
     return need;
 }
 
 hipError_t arnoldi(KrylovWork &w, hipStream_t s)
 {
-    const int nch = w.Mp / kWave;
-    const int nrb1 = nrb_for(w.N, w.rpw1), nrb2 = nrb_for(w.N, w.rpw2);
-    const int cblocks = (int)ceil_div(w.Mp, 64);
-    hipError_t e = hipSuccess;
-    // u_0 = psi: w.u aliases w.psi (pass 2 overwrites it in place; psi is
+    // u_0 = psi: w.u aliases w.psi (the update overwrites it in place; psi is
     // rebuilt from Q c after the step)
+    const int nch = w.Mp / kWave;
+    const int nrb1 = nrb_for(w.Z, w.rpw1), nrb2 = (int)ceil_div(w.Z, w.rpw2);
+    hipError_t e = hipSuccess;
     for (int i = 0; i < w.m; ++i) {
+        if ((e = spmm_launch(w, s)) != hipSuccess) return e;
+        if ((e = reduce(w, 2, nrb1, w.dots1, s)) != hipSuccess) return e;
         {
-        ProfScope ps(w, PF_PASS1, (i + 2) * vec_bytes(w) + csr_bytes(w), s);
-        hipLaunchKernelGGL(kPass1Spmm[i], dim3(nrb1 * nch), dim3(kBlock), 0, s, w.rowptr, w.col, w.val, w.Q, w.u,
-                           w.v, w.part, w.N, w.Mp, w.rpw1);
+            ProfScope ps(w, PF_COEFFS, 0.0, s);
+            hipLaunchKernelGGL(k_coeffs, dim3(nch), dim3(kBlock), 0, s, i, w.m, w.M, w.Mp, w.kS, w.kT, w.dots1,
+                               w.dots, w.coef, w.tprev, w.hess, w.beta0, w.flags);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = reduce(w, 2 * i + 2, nrb1, s)) != hipSuccess) return e;
+        const bool last = i == w.m - 1;
         {
-        ProfScope ps(w, PF_COEFFS, 0.0, s);
-        hipLaunchKernelGGL(k_coeffs, dim3(cblocks), dim3(64), 0, s, i, w.m, w.M, w.Mp, w.dots, w.coef, w.tprev,
-                           w.hess, w.beta0, w.flags);
+            ProfScope ps(w, PF_PASS2, (i + (last ? 3 : 4)) * vec_bytes(w), s);
+            hipLaunchKernelGGL(last ? kUpdateLast[i] : kUpdate[i], dim3(nrb2 * nch), dim3(kBlock), 0, s, w.Q, w.Q,
+                               w.Q + (int64_t)i * w.N * w.Mp, w.u, w.v, w.coef, w.part, w.Z, w.Mp, w.rpw2, nrb2);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        {
-        ProfScope ps(w, PF_PASS2, (i + 4) * vec_bytes(w), s);
-        hipLaunchKernelGGL(kPass2[i], dim3(nrb2 * nch), dim3(kBlock), 0, s, w.Q, w.Q + (int64_t)i * w.N * w.Mp, w.u,
-                           w.v, w.coef, w.N, w.Mp, w.rpw2);
-        }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = reduce(w, last ? i + 1 : 2 * (i + 1), nrb2, w.dots, s)) != hipSuccess) return e;
     }
-    {
-    ProfScope ps(w, PF_DOTS, (w.m + 1) * vec_bytes(w), s);
-    hipLaunchKernelGGL(kPass1Dots[w.m], dim3(nrb1 * nch), dim3(kBlock), 0, s, w.rowptr, w.col, w.val, w.Q, w.u, w.v,
-                       w.part, w.N, w.Mp, w.rpw1);
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = reduce(w, w.m, nrb1, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finalize, dim3(cblocks), dim3(64), 0, s, w.m, w.M, w.Mp, w.dots, w.tprev, w.hess);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.m, w.M, w.Mp, w.dots,
+                       w.tprev, w.hess);
     return hipGetLastError();
 }
 
-hipError_t spmm_only(KrylovWork &w, hipStream_t s)
-{
-    const int nch = w.Mp / kWave;
-    const int nrb1 = nrb_for(w.N, w.rpw1);
-    hipLaunchKernelGGL(kPass1Spmm[0], dim3(nrb1 * nch), dim3(kBlock), 0, s, w.rowptr, w.col, w.val, w.Q, w.u, w.v,
-                       w.part, w.N, w.Mp, w.rpw1);
-    return hipGetLastError();
-}
+hipError_t spmm_only(KrylovWork &w, hipStream_t s) { return spmm_launch(w, s); }
 
 hipError_t expm_lanes(KrylovWork &w, double dt, hipStream_t s)
 {
@@ -727,14 +832,13 @@ hipError_t gram_projectors(KrylovWork &w, hipStream_t s)
     int nrb = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)(w.part_capacity / per)));
     int64_t rows = ceil_div(w.Z, nrb);
     nrb = (int)ceil_div(w.Z, rows);
-    ProfScope ps(w, PF_GRAM, 0.0, s);
-    hipLaunchKernelGGL(k_gram, dim3(nrb * nch, 4 * w.m), dim3(kWave), 0, s, w.Q, w.part, w.m, w.Z, w.Mp, rows);
+    {
+        ProfScope ps(w, PF_GRAM, 0.0, s);
+        hipLaunchKernelGGL(k_gram, dim3(nrb * nch, 4 * w.m), dim3(kWave), 0, s, w.Q, w.part, w.m, w.Z, w.Mp, rows);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int nacc = 4 * w.m * w.m;
-    const int64_t n = (int64_t)nacc * w.Mp;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, w.part, w.gram, nacc, nrb, w.Mp);
-    return hipGetLastError();
+    return reduce(w, 4 * w.m * w.m, nrb, w.gram, s);
 }
 
 hipError_t step_lanes(KrylovWork &w, double dt, double t_end, double tol, bool gram, hipStream_t s)
@@ -751,13 +855,14 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s)
     const int nrbp = nrb_for(w.Z, w.rpwp);
     const PsiFn fn = observe ? kPsiObs[w.m - 1] : kPsiPlain[w.m - 1];
     {
-    ProfScope ps(w, PF_PSI, (w.m + 1) * vec_bytes(w), s);
-    hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, w.c, w.psi, w.part, w.Z, w.Mp, w.rpwp);
+        ProfScope ps(w, PF_PSI, (w.m + 1) * vec_bytes(w), s);
+        hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, w.c, w.psi, w.part, w.Z, w.Mp, w.rpwp,
+                           nrbp);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !observe) return e;
-    if ((e = reduce(w, 2, nrbp, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_obs_store, dim3((unsigned)ceil_div(w.M, 64)), dim3(64), 0, s, w.M, w.Mp, w.dots, w.lane_k,
+    if ((e = reduce(w, 2, nrbp, w.dots1, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_obs_store, dim3((unsigned)ceil_div(w.M, 64)), dim3(64), 0, s, w.M, w.Mp, w.dots1, w.lane_k,
                        w.obs, w.n_out);
     return hipGetLastError();
 }

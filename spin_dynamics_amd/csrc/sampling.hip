@@ -1,5 +1,5 @@
-// sampling.hip -- trace vectors psi_r = theta (x) z_r, written lane-fastest
-// (psi[(e*Z + n)*Mp + r]).
+// sampling.hip -- trace vectors psi_r = theta (x) z_r, written lane-fastest in
+// the device basis order (psi[(4*n + e)*Mp + r]).
 //
 // Kinds: SU(Z) coherent states (sample_SUZ, qm_dynamics.f90:416-430: complex
 // normals re then im from the Marsaglia polar method, normalised), random
@@ -40,13 +40,13 @@ __global__ void k_init_vectors(VecInit vi, int M, int Mp, int64_t Z, double2 *__
                 rng_circle(g, x, y);
                 z = cz(x * f, y * f);
             }
-            psi[n * ld + r] = z; // staged in electron block 0
+            psi[(4 * n) * ld + r] = z; // staged in the alpha-alpha row
         }
         const double nrm = vi.kind == 0 ? sqrt(nrm2) : 1.0;
         for (int64_t n = 0; n < Z; ++n) {
-            double2 z = psi[n * ld + r];
+            double2 z = psi[(4 * n) * ld + r];
             if (vi.kind == 0) z = cz(z.x / nrm, z.y / nrm);
-            for (int e = 0; e < 4; ++e) psi[(e * Z + n) * ld + r] = cmul(vi.theta[e], z);
+            for (int e = 0; e < 4; ++e) psi[(4 * n + e) * ld + r] = cmul(vi.theta[e], z);
         }
         return;
     }
@@ -56,7 +56,7 @@ __global__ void k_init_vectors(VecInit vi, int M, int Mp, int64_t Z, double2 *__
             z = vi.inj[(int64_t)r * Z + n];
         else
             z = (n == vi.basis_offset + r) ? cz(1.0, 0.0) : cz(0.0, 0.0);
-        for (int e = 0; e < 4; ++e) psi[(e * Z + n) * ld + r] = cmul(vi.theta[e], z);
+        for (int e = 0; e < 4; ++e) psi[(4 * n + e) * ld + r] = cmul(vi.theta[e], z);
     }
 }
 
