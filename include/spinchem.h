@@ -108,6 +108,13 @@ int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed
                      const double *trace_vectors, double *P_S, double *P_T0, double *P_Tp,
                      double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T);
 
+/* As spc_trace_sample, with the per-sample seeds given ([M][2]): the pairs
+ * trace_sampling_para draws from the caller's generator with two next() calls
+ * per sample (qm_dynamics.f90:315-320) and passes to set_seed (:326). */
+int spc_trace_sample_seeds(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
+                           double *P_S, double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out,
+                           double *Phi_S, double *Phi_T);
+
 /* Replaces exact_dynamics (qm_dynamics.f90:24-137): the Z nuclear basis
  * states as trace vectors (sim->n_samples and vec_kind are ignored). */
 int spc_exact_trace(spc_ctx *ctx, const spc_sim_params *sim, double *P_S, double *P_T0,

@@ -620,6 +620,21 @@ int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed
     return propagate(ctx, sim, vi, M, seeds, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
 }
 
+int spc_trace_sample_seeds(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, double *P_S,
+                           double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (!sim || !sample_seeds) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    if (sim->vec_kind != SPC_VEC_SUZ && sim->vec_kind != SPC_VEC_PHASE)
+        return fail(ctx, SPC_ERR_INVALID, "vec_kind must be SPC_VEC_SUZ or SPC_VEC_PHASE");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t M = sim->n_samples;
+    std::vector<int64_t> seeds(sample_seeds, sample_seeds + 2 * std::max<int64_t>(M, 0));
+    VecInit vi{};
+    vi.kind = sim->vec_kind;
+    return propagate(ctx, sim, vi, M, seeds, nullptr, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
 int spc_exact_trace(spc_ctx *ctx, const spc_sim_params *sim, double *P_S, double *P_T0, double *P_Tp, double *P_Tm,
                     int64_t n_out, double *Phi_S, double *Phi_T)
 {

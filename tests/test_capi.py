@@ -112,3 +112,25 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(_lib.SpinchemError) as ei:
         _lib.Context(0)
     assert ei.value.code == _lib.SPC_ERR_NO_DEVICE
+
+
+def test_driver_reads_input_and_reports_missing_device(tmp_path):
+    drv = os.path.join(ROOT, "spin_dynamics_amd", "bin", "spinchem")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    import subprocess
+    w = workloads.config1(M=4, T_ns=10.0)
+    out = tmp_path / "out"
+    out.mkdir()
+    ini = tmp_path / "in.ini"
+    workloads.write_ini(w, str(ini), str(out))
+    r = subprocess.run([drv, str(ini)], capture_output=True, text=True, timeout=60)
+    assert "Program initialised" in r.stdout and "no HIP device" in r.stdout
+    r = subprocess.run([drv, str(tmp_path / "missing.ini")], capture_output=True, text=True, timeout=60)
+    assert "Input file does not exist" in r.stdout + r.stderr
