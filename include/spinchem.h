@@ -134,6 +134,28 @@ int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, doub
 /* y = H x for M vectors ([M][N] complex each), the spmv of qm_dynamics.f90:452. */
 int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y);
 
+/* ------------------------------------------------------- semiclassical */
+/* Semiclassical spin system (isotropic couplings, as src/sc uses a_iso). */
+typedef struct spc_sc_system {
+    int32_t n_nuc1, n_nuc2;  /* nuclei on electron 1 and 2 (n_nuc1 + n_nuc2 <= 64) */
+    const int32_t *mult;     /* multiplicities (spin I = (mult-1)/2)               */
+    const double *a_iso;     /* isotropic hyperfine couplings, mT                   */
+    double B;                /* field magnitude, mT                                 */
+    double field_dir[3];     /* unit direction                                      */
+    double J;                /* exchange, mT (2J S1.S2)                              */
+    double kS, kT;           /* rates / gamma_e                                     */
+} spc_sc_system;
+
+/* Replaces semi_classical (sc_dynamics.f90:17-100, class_radical_pair.f90:
+ * 12-251; unreachable and non-compiling in the reference): classical electron
+ * and nuclear spin vectors, one sample per lane pair.  Samples are seeded with
+ * two next() draws each from set_seed(seed).  P_S, P_T: n_steps + 1 values of
+ * 4 <P_S(0) P_S(t)> and 4 <P_S(0) P_T(t)> with P_T = x_16 - P_S
+ * (class_radical_pair.f90:174-175); P_T, Phi_S, Phi_T may be NULL; Phi_S and
+ * Phi_T hold (n_steps + 1)/2 Simpson yields (observables.f90:74-101). */
+int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2],
+                          double dt, int64_t n_steps, double *P_S, double *P_T, double *Phi_S, double *Phi_T);
+
 /* Last-run timing of the propagation loop: milliseconds of device time and
  * number of Krylov rebuilds, for benchmarks. */
 int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps);

@@ -283,7 +283,8 @@ def get_kinetics(P_S, P_T0, P_Tp, P_Tm, dt: float, kS: float, kT: float):
 
 def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1), J: float = 0.0,
                   kS: float = 0.0, kT: float = 0.0, n_samples: int, dt: float, n_steps_: int,
-                  seed=(42, 99), n_threads: int = 0) -> np.ndarray:
+                  seed=(42, 99), n_threads: int = 0, with_pt: bool = False):
+    """P_S(t) (and P_T(t) with with_pt) of the semiclassical model, n_steps_ + 1 values."""
     a = np.ascontiguousarray(list(a_iso1) + list(a_iso2), dtype=float)
     mult = np.ascontiguousarray(list(mult1) + list(mult2), dtype=np.intc)
     s = OrcScSystem()
@@ -294,8 +295,10 @@ def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1)
     for i in range(3):
         s.dir[i] = direction[i]
     out = np.zeros(n_steps_ + 1)
+    out_t = np.zeros(n_steps_ + 1)
     rc = lib().orc_semiclassical(ctypes.byref(s), ctypes.c_int64(n_samples), (ctypes.c_int64 * 2)(*seed),
-                                 ctypes.c_double(dt), ctypes.c_int64(n_steps_), _dp(out), ctypes.c_int(n_threads))
+                                 ctypes.c_double(dt), ctypes.c_int64(n_steps_), _dp(out), _dp(out_t),
+                                 ctypes.c_int(n_threads))
     if rc:
         raise RuntimeError(f"orc_semiclassical failed with code {rc}")
-    return out
+    return (out, out_t) if with_pt else out

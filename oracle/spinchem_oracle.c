@@ -1108,7 +1108,7 @@ static void sc_electron_half(double *x, const double *KJ, const double *w1, cons
 }
 
 int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t master_seed[2], double dt,
-                      int64_t n_steps, double *P_S, int n_threads)
+                      int64_t n_steps, double *P_S, double *P_T, int n_threads)
 {
     const int nn = s->n1 + s->n2;
     if (n_samples < 1 || n_steps < 0) return 1;
@@ -1126,7 +1126,10 @@ int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t m
         for (int i = 0; i < 16; ++i)
             for (int j = 0; j < 16; ++j) KJ[16 * i + j] = Ez[2 * (i + 16 * j)];
     }
-    const double wz[3] = {-s->B * s->dir[0], -s->B * s->dir[1], -s->B * s->dir[2]};
+    /* the field is B along the unit vector of dir */
+    const double dn = sqrt(s->dir[0] * s->dir[0] + s->dir[1] * s->dir[1] + s->dir[2] * s->dir[2]);
+    if (!(dn > 0.0)) return 1;
+    const double wz[3] = {-s->B * s->dir[0] / dn, -s->B * s->dir[1] / dn, -s->B * s->dir[2] / dn};
     const double shalf = sqrt(0.75);
     int64_t *seeds = (int64_t *)malloc(sizeof(int64_t) * 2 * n_samples);
     {
@@ -1139,6 +1142,7 @@ int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t m
         }
     }
     double *contrib = (double *)malloc(sizeof(double) * (n_steps + 1) * (size_t)n_samples);
+    double *contrib_t = (double *)malloc(sizeof(double) * (n_steps + 1) * (size_t)n_samples);
 #ifdef _OPENMP
     int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
@@ -1160,7 +1164,9 @@ int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t m
         x[15] = 1.0;
         double p0 = sc_ps(x);
         double *out = contrib + (n_steps + 1) * (size_t)smp;
+        double *out_t = contrib_t + (n_steps + 1) * (size_t)smp;
         out[0] = p0 * p0;
+        out_t[0] = p0 * x[15];
         for (int64_t st = 1; st <= n_steps; ++st) {
             for (int half = 0; half < 2; ++half) {
                 double w1[3] = {wz[0], wz[1], wz[2]}, w2[3] = {wz[0], wz[1], wz[2]};
@@ -1182,15 +1188,22 @@ int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t m
                 }
             }
             out[st] = p0 * sc_ps(x);
+            out_t[st] = p0 * x[15];
         }
         free(Iv);
     }
+    /* P_T = x_16 - P_S per sample (class_radical_pair.f90:174-175) */
     for (int64_t st = 0; st <= n_steps; ++st) {
-        double acc = 0.0;
-        for (int64_t smp = 0; smp < n_samples; ++smp) acc += contrib[(n_steps + 1) * (size_t)smp + st];
+        double acc = 0.0, acc_t = 0.0;
+        for (int64_t smp = 0; smp < n_samples; ++smp) {
+            acc += contrib[(n_steps + 1) * (size_t)smp + st];
+            acc_t += contrib_t[(n_steps + 1) * (size_t)smp + st];
+        }
         P_S[st] = 4.0 * acc / (double)n_samples;
+        if (P_T) P_T[st] = 4.0 * acc_t / (double)n_samples - P_S[st];
     }
     free(contrib);
+    free(contrib_t);
     free(seeds);
     return 0;
 }

@@ -171,7 +171,7 @@ typedef struct {
     const double *a_iso;   /* n1+n2 isotropic couplings, mT                   */
     const int *mult;       /* n1+n2 multiplicities                            */
     double B;              /* field magnitude, mT (electron Zeeman -B n.S)     */
-    double dir[3];
+    double dir[3];         /* field direction (normalised here)                */
     double J;              /* exchange, mT (2J S1.S2)                          */
     double kS, kT;         /* mT units                                         */
 } orc_sc_system;
@@ -179,10 +179,12 @@ typedef struct {
 /* Semiclassical (Schulten-Wolynes / Manolopoulos-Hore) singlet correlation
  * function.  Semantics of src/sc (sc_dynamics.f90:17-100,
  * class_radical_pair.f90:147-251) with the defects of that code removed.
- * P_S[n_out], n_out = n_steps + 1.  Samples seeded by orc_sample_seeds. */
+ * P_S[n_out], n_out = n_steps + 1, and P_T[n_out] (NULL allowed) with
+ * P_T = x_16 - P_S per sample (class_radical_pair.f90:174-175).  Samples
+ * seeded by orc_sample_seeds. */
 int orc_semiclassical(const orc_sc_system *s, int64_t n_samples,
                       const int64_t master_seed[2], double dt, int64_t n_steps,
-                      double *P_S, int n_threads);
+                      double *P_S, double *P_T, int n_threads);
 
 #ifdef __cplusplus
 }

@@ -807,6 +807,139 @@ int spc_profile_read(spc_ctx *ctx, int family, double *total_ms, int64_t *launch
     return SPC_OK;
 }
 
+// 16x16 generator K + J of the semiclassical electron vector (row-major):
+// Haberkorn recombination (sc_dynamics.f90:102-142, with K(13,9) set) and the
+// exchange coupling (sc_dynamics.f90:144-185).
+static void sc_generator(double kS, double kT, double J, double *G)
+{
+    const double kb = (kS + 3.0 * kT) / 4.0, dk = (kS - kT) / 4.0;
+    for (int i = 0; i < 256; ++i) G[i] = 0.0;
+    auto S = [&](int i, int j, double v) { G[16 * (i - 1) + (j - 1)] += v; };
+    for (int i = 1; i <= 16; ++i) S(i, i, -kb);
+    for (int i = 1; i <= 3; ++i) S(i, i + 3, dk), S(i + 3, i, dk);
+    S(8, 10, dk), S(10, 8, dk), S(9, 13, dk), S(13, 9, dk), S(12, 14, dk), S(14, 12, dk);
+    S(7, 11, -dk), S(11, 7, -dk), S(7, 15, -dk), S(15, 7, -dk), S(11, 15, -dk), S(15, 11, -dk);
+    for (int i = 7; i <= 15; i += 4) S(i, 16, 0.25 * dk), S(16, i, 4.0 * dk);
+    const double t = 2.0 * J, h = 0.5 * J;
+    S(1, 12, t), S(1, 14, -t), S(2, 13, t), S(2, 9, -t), S(3, 8, t), S(3, 10, -t);
+    S(4, 12, -t), S(4, 14, t), S(5, 13, -t), S(5, 9, t), S(6, 8, -t), S(6, 10, t);
+    S(8, 3, -h), S(8, 6, h), S(9, 2, h), S(9, 5, -h), S(10, 3, h), S(10, 6, -h);
+    S(12, 1, -h), S(12, 4, h), S(13, 2, -h), S(13, 5, h), S(14, 1, h), S(14, 4, -h);
+}
+
+int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2], double dt,
+                          int64_t n_steps, double *P_S, double *P_T, double *Phi_S, double *Phi_T)
+{
+    if (!ctx || !sys || !seed || !P_S) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    const int nn = sys->n_nuc1 + sys->n_nuc2;
+    if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || nn > 64) return fail(ctx, SPC_ERR_UNSUPPORTED, "at most 64 nuclei");
+    if (nn > 0 && (!sys->mult || !sys->a_iso)) return fail(ctx, SPC_ERR_INVALID, "mult/a_iso must not be NULL");
+    if (n_samples < 1 || n_steps < 0 || !(dt > 0.0)) return fail(ctx, SPC_ERR_INVALID, "bad sample/step counts");
+    const int slots = sc_slots(sys->n_nuc1, sys->n_nuc2);
+    if (slots < 0) return fail(ctx, SPC_ERR_UNSUPPORTED, "too many nuclei on one electron for the lane-pair split");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ScParams p{};
+    p.nn = nn;
+    p.n1 = sys->n_nuc1;
+    p.nl1 = (sys->n_nuc1 + 1) / 2;
+    p.slots = slots;
+    p.dt = dt;
+    const double shalf = std::sqrt(0.75);
+    std::vector<double> nuc(128, 0.0); // [a | sqrt(I(I+1))] x 64, draw order
+    std::vector<double> cs(4 * slots); // [slot][lane][cos, sin]
+    for (int i = 0; i < 2 * slots; ++i) cs[2 * i] = 1.0, cs[2 * i + 1] = 0.0;
+    for (int k = 0; k < nn; ++k) {
+        const int g = sys->mult[k];
+        if (g < 2) return fail(ctx, SPC_ERR_INVALID, "multiplicity must be >= 2");
+        const double I = 0.5 * (g - 1);
+        const double a = sys->a_iso[k];
+        nuc[k] = a;
+        nuc[64 + k] = std::sqrt(I * (I + 1.0));
+        // lane-pair slot of nucleus k (the kernel's mapping)
+        const bool e2 = k >= sys->n_nuc1;
+        const int j = e2 ? k - sys->n_nuc1 : k;
+        const int slot = (j >> 1) + (e2 ? p.nl1 : 0), lane = j & 1;
+        cs[2 * (2 * slot + lane)] = std::cos(a * shalf * dt);
+        cs[2 * (2 * slot + lane) + 1] = std::sin(a * shalf * dt);
+    }
+    double nrm = std::sqrt(sys->field_dir[0] * sys->field_dir[0] + sys->field_dir[1] * sys->field_dir[1] +
+                           sys->field_dir[2] * sys->field_dir[2]);
+    if (!(nrm > 0.0)) return fail(ctx, SPC_ERR_INVALID, "field direction must be nonzero");
+    for (int a = 0; a < 3; ++a) p.w[a] = -sys->B * sys->field_dir[a] / nrm;
+    // exp(dt/2 (K + J)) with the device Pade kernel (as a complex matrix)
+    double G[256];
+    sc_generator(sys->kS, sys->kT, sys->J, G);
+    std::vector<double2> Gz(256), Ez(256);
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) Gz[i + 16 * j] = make_double2(0.5 * dt * G[16 * i + j], 0.0);
+    double2 *dA = nullptr, *dE = nullptr;
+    int rc;
+    if ((rc = ensure(ctx, "expmA", 256, &dA))) return rc;
+    if ((rc = ensure(ctx, "expmE", 256, &dE))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dA, Gz.data(), sizeof(double2) * 256, hipMemcpyHostToDevice, s));
+    HIP_TRY(expm_single(16, dA, dE, s));
+    HIP_TRY(hipMemcpyAsync(Ez.data(), dE, sizeof(double2) * 256, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // G is block diagonal over (S1x,S2x,T32,T23), (S1y,S2y,T13,T31),
+    // (S1z,S2z,T21,T12) and (T11,T22,T33,1), and the three spin blocks are the
+    // same 4x4 matrix, so exp(dt/2 G) is given by its first and last blocks
+    static const int blk[2][4] = {{0, 3, 11, 13}, {6, 10, 14, 15}};
+    std::vector<double> kjb(64, 0.0);
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) kjb[16 * b + 4 * i + j] = Ez[blk[b][i] + 16 * blk[b][j]].x;
+    // per-sample seeds: two next() of the master stream each
+    HostRng g;
+    g.set_seed(seed);
+    std::vector<int64_t> seeds(2 * n_samples);
+    for (int64_t i = 0; i < 2 * n_samples; ++i) seeds[i] = (int64_t)g.next();
+    // persistent grid: a fixed block count keeps the summation order independent of the device
+    const int64_t groups = (n_samples + kScThreads / 2 - 1) / (kScThreads / 2);
+    const int nblocks = (int)std::min<int64_t>(groups, 1024);
+    const int64_t nwaves = (int64_t)nblocks * (kScThreads / 64);
+    int64_t *d_seeds = nullptr;
+    double *d_part = nullptr, *d_out = nullptr, *d_tab = nullptr;
+    if ((rc = ensure(ctx, "sc_seeds", (size_t)2 * n_samples, &d_seeds))) return rc;
+    if ((rc = ensure(ctx, "sc_tab", (size_t)(128 + 64 + 4 * slots), &d_tab))) return rc;
+    if ((rc = ensure(ctx, "sc_part", (size_t)2 * (n_steps + 1) * nwaves, &d_part))) return rc;
+    if ((rc = ensure(ctx, "sc_out", (size_t)2 * (n_steps + 1), &d_out))) return rc;
+    std::vector<double> tab(nuc);
+    tab.insert(tab.end(), kjb.begin(), kjb.end());
+    tab.insert(tab.end(), cs.begin(), cs.end());
+    HIP_TRY(hipMemcpyAsync(d_seeds, seeds.data(), sizeof(int64_t) * seeds.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s));
+    p.nuc = d_tab;
+    p.kjb = d_tab + 128;
+    p.cs = d_tab + 192;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    HIP_TRY(sc_run(p, d_seeds, d_part, nblocks, n_samples, n_steps, d_out, d_out + n_steps + 1, s));
+    HIP_TRY(hipEventRecord(e1, s));
+    std::vector<double> pt(n_steps + 1);
+    HIP_TRY(hipMemcpyAsync(P_S, d_out, sizeof(double) * (n_steps + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pt.data(), d_out + n_steps + 1, sizeof(double) * (n_steps + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->last_ms = ms;
+    ctx->last_rebuilds = 0;
+    ctx->last_steps = n_steps;
+    if (P_T) std::copy(pt.begin(), pt.end(), P_T);
+    if (Phi_S || Phi_T) {
+        // P_T enters the triplet yield as the single triplet population
+        std::vector<double> zero(n_steps + 1, 0.0), phis((n_steps + 1) / 2 + 1), phit((n_steps + 1) / 2 + 1);
+        simpson(n_steps + 1, P_S, pt.data(), zero.data(), zero.data(), dt, sys->kS, sys->kT, phis.data(), phit.data());
+        if (Phi_S) std::copy(phis.begin(), phis.begin() + (n_steps + 1) / 2, Phi_S);
+        if (Phi_T) std::copy(phit.begin(), phit.begin() + (n_steps + 1) / 2, Phi_T);
+    }
+    return SPC_OK;
+}
+
 int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps)
 {
     if (!ctx) return SPC_ERR_INVALID;

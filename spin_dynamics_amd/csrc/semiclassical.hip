@@ -1,0 +1,322 @@
+// semiclassical.hip -- semiclassical (Schulten-Wolynes / Manolopoulos-Hore)
+// singlet correlation function, one sample per lane pair.
+//
+// Semantics of the reference's src/sc (sc_dynamics.f90:17-100,
+// class_radical_pair.f90:147-251), with its defects removed: electron state
+// x = [S1, S2, T12 (column-major), 1] (16 values); per step: electron half step
+// (x <- exp(dt/2 (K+J)) x, then S1 and the T12 columns precess about
+// w1 = -B n + sum_{k in e1} a_k I_k, S2 and the T12 rows about w2), nuclear full
+// step (I_k precesses about a_k S_e normalised to sqrt(3)/2), electron half
+// step.  P_S = x_16/4 - tr T12 and P_S(t) = 4 <P_S(0) P_S(t)> over samples.
+//
+// Mapping to CDNA4 (FP64 VALU-bound, all state in VGPRs):
+// - A sample's nuclei are split over a lane pair; lane h of the pair holds the
+//   e1 nuclei j = 2q + h in slots q < nl1 and the e2 nuclei j = 2(q - nl1) + h
+//   in slots q >= nl1 (nl1 = ceil(n1/2) is wave-uniform, so the electron a
+//   slot couples to is a uniform branch).  The pair exchanges its partial
+//   hyperfine fields with one shuffle per component, combined in a fixed order
+//   so both lanes hold bit-identical electron states.
+// - Nuclear spins are stored scaled, I'_k = a_k I_k: the field is the plain
+//   sum of the I'_k, and a rotation commutes with the scaling.  The nuclear
+//   precession angle a_k sqrt(3)/2 dt is constant, so its cos/sin are loaded
+//   once per lane.
+// - exp(dt/2 (K+J)) is block diagonal with four invariant 4x4 blocks, and K+J
+//   restricted to (S1x,S2x,T32,T23), (S1y,S2y,T13,T31) and (S1z,S2z,T21,T12)
+//   is one and the same 4x4 matrix: 64 FMAs per half step instead of 256, with
+//   32 wave-uniform coefficients held in scalar registers.
+// - The fields after the nuclear step serve both the second half of this
+//   step and the first half of the next, so one field sum and one pair of
+//   sincos per step.
+// - P_0 P_S(t) is summed over each wave's samples with shuffles and added to
+//   part[st][wave] by lane 0: no barriers in the step loop; the grid is
+//   persistent and fixed, so the sums are deterministic.
+#include <algorithm>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace spc {
+
+namespace {
+
+// Positions in x of the three spin blocks (one shared 4x4 generator) and of
+// the block (T11, T22, T33, 1).
+__device__ constexpr int kSpinBlk[3][4] = {{0, 3, 11, 13}, {1, 4, 12, 8}, {2, 5, 7, 9}};
+__device__ constexpr int kDiagBlk[4] = {6, 10, 14, 15};
+
+__device__ __forceinline__ void rotate_cs(double *v, const double *h, double c, double s)
+{
+    // v_par + c v_perp + s (h x v), h a unit vector
+    const double dp = v[0] * h[0] + v[1] * h[1] + v[2] * h[2];
+    const double cx = h[1] * v[2] - h[2] * v[1];
+    const double cy = h[2] * v[0] - h[0] * v[2];
+    const double cz_ = h[0] * v[1] - h[1] * v[0];
+    const double r0 = dp * h[0] + c * (v[0] - dp * h[0]) + s * cx;
+    const double r1 = dp * h[1] + c * (v[1] - dp * h[1]) + s * cy;
+    const double r2 = dp * h[2] + c * (v[2] - dp * h[2]) + s * cz_;
+    v[0] = r0;
+    v[1] = r1;
+    v[2] = r2;
+}
+
+// axis and sincos of a precession about field f for time t (identity for f = 0)
+struct Rot {
+    double h[3], c, s;
+};
+__device__ __forceinline__ Rot make_rot(const double *f, double t)
+{
+    Rot r;
+    const double w = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    const bool z = !(w > 0.0);
+    const double ws = z ? 1.0 : w;
+    r.h[0] = z ? 0.0 : f[0] / ws;
+    r.h[1] = z ? 0.0 : f[1] / ws;
+    r.h[2] = z ? 1.0 : f[2] / ws;
+    double sn, cs;
+    sincos(z ? 0.0 : w * t, &sn, &cs);
+    r.c = cs;
+    r.s = sn;
+    return r;
+}
+
+template <int NL>
+__device__ __forceinline__ void fields(const double *wz, int nl1, const double (&I)[NL][3], double *w1, double *w2)
+{
+    double f1[3] = {0, 0, 0}, f2[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        if (q < nl1) {
+            f1[0] += I[q][0];
+            f1[1] += I[q][1];
+            f1[2] += I[q][2];
+        } else {
+            f2[0] += I[q][0];
+            f2[1] += I[q][1];
+            f2[2] += I[q][2];
+        }
+    }
+    // lane-pair combine, low lane's partial first in both lanes
+    const bool low = (threadIdx.x & 1) == 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double o1 = __shfl_xor(f1[d], 1), o2 = __shfl_xor(f2[d], 1);
+        w1[d] = wz[d] + (low ? f1[d] + o1 : o1 + f1[d]);
+        w2[d] = wz[d] + (low ? f2[d] + o2 : o2 + f2[d]);
+    }
+}
+
+// x <- blockdiag(KJ) x, then the precessions of S1, S2 and T12.  kA: the
+// spin-block coefficients, kB: the (T11, T22, T33, 1) block, both row-major.
+__device__ __forceinline__ void electron_half(double *x, const double *kA, const double *kB, const Rot &r1,
+                                              const Rot &r2)
+{
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        double y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s = fma(kA[4 * i + j], x[kSpinBlk[b][j]], s);
+            y[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[kSpinBlk[b][i]] = y[i];
+    }
+    {
+        double y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s = fma(kB[4 * i + j], x[kDiagBlk[j]], s);
+            y[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[kDiagBlk[i]] = y[i];
+    }
+    rotate_cs(x + 0, r1.h, r1.c, r1.s);
+    rotate_cs(x + 3, r2.h, r2.c, r2.s);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) rotate_cs(x + 6 + 3 * b, r1.h, r1.c, r1.s); // T(:,b) rotates with S1
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {                                             // T(a,:) rotates with S2
+        double v[3] = {x[6 + a], x[9 + a], x[12 + a]};
+        rotate_cs(v, r2.h, r2.c, r2.s);
+        x[6 + a] = v[0];
+        x[9 + a] = v[1];
+        x[12 + a] = v[2];
+    }
+}
+
+__device__ __forceinline__ double ps_of(const double *x) { return 0.25 * x[15] - (x[6] + x[10] + x[14]); }
+
+// part[st][wave] += sums over the wave's samples of (P_S(0) P_S(t_st),
+// P_S(0) x_16(t_st)), for every sample group the wave processes (kScThreads/2
+// samples per group).
+template <int NL>
+__global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__restrict__ seeds,
+                                                   double *__restrict__ part, int64_t n_samples, int64_t n_steps,
+                                                   int64_t n_groups)
+{
+    const int h = threadIdx.x & 1;
+    double kA[16], kB[16]; // wave-uniform: scalar registers
+#pragma unroll
+    for (int i = 0; i < 16; ++i) kA[i] = p.kjb[i], kB[i] = p.kjb[16 + i];
+    double cq[NL], sq[NL]; // this lane's nuclear precession cos/sin per slot
+#pragma unroll
+    for (int q = 0; q < NL; ++q) cq[q] = p.cs[4 * q + 2 * h], sq[q] = p.cs[4 * q + 2 * h + 1];
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * (kScThreads / kWave);
+    const int64_t gwave = (int64_t)blockIdx.x * (kScThreads / kWave) + (threadIdx.x >> 6);
+    const double shalf = 0.86602540378443864676; // sqrt(3/4)
+    const double wz[3] = {p.w[0], p.w[1], p.w[2]};
+    const double th = 0.5 * p.dt;
+    const int nl1 = p.nl1;
+
+    for (int64_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+        const int64_t sample = grp * (kScThreads / 2) + (threadIdx.x >> 1);
+        const bool active = sample < n_samples;
+        double x[16];
+        double I[NL][3];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) I[q][0] = I[q][1] = I[q][2] = 0.0;
+        {
+            Rng g;
+            const int64_t sidx = active ? sample : 0;
+            rng_set_seed(g, seeds[2 * sidx], seeds[2 * sidx + 1]);
+            rng_sphere(g, shalf, x + 0);
+            rng_sphere(g, shalf, x + 3);
+            double tmp[3];
+            for (int k = 0; k < p.nn; ++k) { // the same stream in both lanes; each keeps its own nuclei
+                rng_sphere(g, p.nuc[64 + k], tmp);
+                const double a = p.nuc[k];
+                const bool e2 = k >= p.n1;
+                const int j = e2 ? k - p.n1 : k;
+                const int slot = (j >> 1) + (e2 ? nl1 : 0);
+                const bool mine = (j & 1) == h;
+#pragma unroll
+                for (int q = 0; q < NL; ++q)
+                    if (mine && q == slot) {
+                        I[q][0] = a * tmp[0];
+                        I[q][1] = a * tmp[1];
+                        I[q][2] = a * tmp[2];
+                    }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) x[6 + a + 3 * b] = x[a] * x[3 + b];
+        x[15] = 1.0;
+        const double p0 = ps_of(x);
+        const bool counts = active && h == 0;
+        double w1[3], w2[3];
+        fields<NL>(wz, nl1, I, w1, w2);
+        Rot r1 = make_rot(w1, th), r2 = make_rot(w2, th);
+
+        for (int64_t st = 0; st <= n_steps; ++st) {
+            if (st > 0) {
+                electron_half(x, kA, kB, r1, r2);
+                // nuclear full step about S_e / |S_e| (angle a_k sqrt(3)/2 dt)
+                const double n1 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                const double n2 = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+                const bool z1 = !(n1 > 0.0), z2 = !(n2 > 0.0);
+                const double d1 = z1 ? 1.0 : n1, d2 = z2 ? 1.0 : n2;
+                const double h1[3] = {x[0] / d1, x[1] / d1, x[2] / d1};
+                const double h2[3] = {x[3] / d2, x[4] / d2, x[5] / d2};
+#pragma unroll
+                for (int q = 0; q < NL; ++q) {
+                    if (q < nl1)
+                        rotate_cs(I[q], h1, z1 ? 1.0 : cq[q], z1 ? 0.0 : sq[q]);
+                    else
+                        rotate_cs(I[q], h2, z2 ? 1.0 : cq[q], z2 ? 0.0 : sq[q]);
+                }
+                fields<NL>(wz, nl1, I, w1, w2);
+                r1 = make_rot(w1, th);
+                r2 = make_rot(w2, th);
+                electron_half(x, kA, kB, r1, r2);
+            }
+            double v = counts ? p0 * ps_of(x) : 0.0, u = counts ? p0 * x[15] : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                v += __shfl_xor(v, o);
+                u += __shfl_xor(u, o);
+            }
+            if (lane == 0) {
+                double2 *pp = reinterpret_cast<double2 *>(part) + st * nwaves + gwave;
+                const double2 old = *pp;
+                *pp = make_double2(old.x + v, old.y + u);
+            }
+        }
+    }
+}
+
+// ps[st] = 4/n_samples * sum_w part[st][w].x, pt[st] = 4/n_samples * sum_w part[st][w].y - ps[st]
+__global__ void k_sc_reduce(const double2 *__restrict__ part, int64_t nwaves, int64_t n_steps, int64_t n_samples,
+                            double *__restrict__ ps, double *__restrict__ pt)
+{
+    const int64_t st = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (st > n_steps) return;
+    double a = 0.0, b = 0.0;
+    for (int64_t w = 0; w < nwaves; ++w) {
+        const double2 v = part[st * nwaves + w];
+        a += v.x;
+        b += v.y;
+    }
+    const double f = 4.0 / (double)n_samples;
+    ps[st] = f * a;
+    pt[st] = f * b - f * a;
+}
+
+__global__ void k_fill_zero(double *__restrict__ a, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0.0;
+}
+
+template <int NL>
+hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
+                  int64_t n_steps, int64_t n_groups, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sc<NL>, dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, n_samples, n_steps, n_groups);
+    return hipGetLastError();
+}
+
+} // namespace
+
+int sc_slots(int n1, int n2)
+{
+    const int need = (n1 + 1) / 2 + (n2 + 1) / 2;
+    for (int nl : {2, 4, 8, 12, 16, 24, 32})
+        if (need <= nl) return nl;
+    return -1;
+}
+
+hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
+                  int64_t n_steps, double *ps, double *pt, hipStream_t s)
+{
+    const int64_t nwaves = (int64_t)nblocks * (kScThreads / kWave);
+    const int64_t n_groups = (n_samples + kScThreads / 2 - 1) / (kScThreads / 2);
+    const int64_t npart = 2 * (n_steps + 1) * nwaves;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((npart + 255) / 256, 4096)), dim3(256), 0, s,
+                       part, npart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (p.slots) {
+    case 2: e = launch<2>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 4: e = launch<4>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 8: e = launch<8>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 12: e = launch<12>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 16: e = launch<16>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 24: e = launch<24>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    case 32: e = launch<32>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sc_reduce, dim3((unsigned)((n_steps + 1 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(part), nwaves, n_steps, n_samples, ps, pt);
+    return hipGetLastError();
+}
+
+} // namespace spc

@@ -18,6 +18,7 @@ class Observables:
     iden: Optional[np.ndarray] = None
     Phi_S: Optional[np.ndarray] = None
     Phi_T: Optional[np.ndarray] = None
+    P_T: Optional[np.ndarray] = None  # total triplet, when sublevels are not resolved (semiclassical)
 
     @classmethod
     def zeros(cls, n: int) -> "Observables":
@@ -34,7 +35,7 @@ class Observables:
     def get_kinetics(self, dt: float, kS: float, kT: float):
         """observables.f90:74-101: cumulative Simpson yields over size/2 pairs."""
         n = len(self.P_S)
-        P_T = self.P_Tp + self.P_T0 + self.P_Tm
+        P_T = self.P_Tp + self.P_T0 + self.P_Tm if self.P_T is None else self.P_T
         self.iden = self.P_S + P_T
         h = n // 2
         self.Phi_S = np.zeros(h)
@@ -48,9 +49,12 @@ class Observables:
 
     def output(self, path: str):
         """observables.f90:103-148: one value per line."""
-        files = [("S_prob.data", self.P_S), ("Tp_prob.data", self.P_Tp), ("T0_prob.data", self.P_T0),
-                 ("Tm_prob.data", self.P_Tm), ("total_prob.data", self.iden), ("S_yield.data", self.Phi_S),
-                 ("T_yield.data", self.Phi_T)]
+        if self.P_T is None:
+            pops = [("S_prob.data", self.P_S), ("Tp_prob.data", self.P_Tp), ("T0_prob.data", self.P_T0),
+                    ("Tm_prob.data", self.P_Tm)]
+        else:
+            pops = [("S_prob.data", self.P_S), ("T_prob.data", self.P_T)]
+        files = pops + [("total_prob.data", self.iden), ("S_yield.data", self.Phi_S), ("T_yield.data", self.Phi_T)]
         for name, data in files:
             if data is None:
                 continue

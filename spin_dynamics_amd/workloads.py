@@ -152,6 +152,56 @@ def small_anisotropic(m1=(3, 2), m2=(2,), M=8, m=12, T_ns=200.0, tol=0.0, vec_ki
                     1.0)
 
 
+@dataclass
+class ScWorkload:
+    """A semiclassical system (isotropic couplings) and its run parameters."""
+    name: str
+    a1: List[float]
+    a2: List[float]
+    m1: List[int]
+    m2: List[int]
+    B: float
+    direction: tuple
+    J: float
+    kS: float
+    kT: float
+    n_samples: int
+    dt: float
+    n_steps: int
+
+    def kwargs(self) -> dict:
+        return dict(B=self.B, direction=self.direction, J=self.J, kS=self.kS, kT=self.kT,
+                    n_samples=self.n_samples, dt=self.dt, n_steps_=self.n_steps)
+
+
+def gamma_like_couplings(rng: Xoroshiro, n: int, lo: float = 0.05, hi: float = 1.2) -> List[float]:
+    """n couplings from a gamma(2) spectrum, g = -(ln u1 + ln u2), mapped
+    linearly so the smallest is lo and the largest hi (mT); drawn in order."""
+    g = [-(math.log(1.0 - rng.unif01()) + math.log(1.0 - rng.unif01())) for _ in range(n)]
+    gmin, gmax = min(g), max(g)
+    return [lo + (hi - lo) * (x - gmin) / (gmax - gmin) for x in g]
+
+
+def config5(n_samples: int = 1 << 20, T_ns: float = 2000.0, dt_ns: float = 0.5, n_1h: int = 20,
+            n_14n: int = 4) -> ScWorkload:
+    """Semiclassical Monte Carlo at large nuclear counts: per radical 20 1H
+    then 4 14N, couplings from gamma_like_couplings (radical 1 then radical 2);
+    1H couplings negative where a further unif_01 < 1/2.  B = 0.05 mT along z,
+    J = 0, kS = 2, kT = 0.5 us^-1 (the plain-method rates), dt = 0.5 ns, T = 2 us,
+    N_steps = ceiling((T + dt)/dt) as the reference's time loop."""
+    rng = Xoroshiro()
+    a, m = [], []
+    for _ in range(2):
+        c = gamma_like_couplings(rng, n_1h + n_14n)
+        sign = [(-1.0 if rng.unif01() < 0.5 else 1.0) for _ in range(n_1h)] + [1.0] * n_14n
+        a.append([s * x for s, x in zip(sign, c)])
+        m.append([2] * n_1h + [3] * n_14n)
+    dt = dt_ns * GAMMA_E / 1000.0
+    t_end = T_ns * GAMMA_E / 1000.0
+    return ScWorkload("config5", a[0], a[1], m[0], m[1], 0.05, (0.0, 0.0, 1.0), 0.0, 2.0 / GAMMA_E, 0.5 / GAMMA_E,
+                      n_samples, dt, math.ceil((t_end + dt) / dt))
+
+
 def write_ini(w: Workload, path: str, output_folder: str, *, simulation_type: str = "trace_sampling",
               fields: Sequence[float] = None):
     """Writes the workload with the reference's keys (spinsys.f90:60-235);
