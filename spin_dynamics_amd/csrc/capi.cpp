@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -835,19 +836,33 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
     if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || nn > 64) return fail(ctx, SPC_ERR_UNSUPPORTED, "at most 64 nuclei");
     if (nn > 0 && (!sys->mult || !sys->a_iso)) return fail(ctx, SPC_ERR_INVALID, "mult/a_iso must not be NULL");
     if (n_samples < 1 || n_steps < 0 || !(dt > 0.0)) return fail(ctx, SPC_ERR_INVALID, "bad sample/step counts");
-    const int slots = sc_slots(sys->n_nuc1, sys->n_nuc2);
-    if (slots < 0) return fail(ctx, SPC_ERR_UNSUPPORTED, "too many nuclei on one electron for the lane-pair split");
+    // lanes per sample: the fewest that hold the nuclei (each lane repeats the
+    // electron work); SPC_SC_LANES=2|4|8 overrides (tuning)
+    int lanes = 0, slots = -1;
+    if (const char *env = std::getenv("SPC_SC_LANES")) {
+        lanes = std::atoi(env);
+        if (lanes != 2 && lanes != 4 && lanes != 8) return fail(ctx, SPC_ERR_INVALID, "SPC_SC_LANES must be 2, 4 or 8");
+        slots = sc_slots(sys->n_nuc1, sys->n_nuc2, lanes);
+    } else {
+        for (int l : {2, 4, 8})
+            if ((slots = sc_slots(sys->n_nuc1, sys->n_nuc2, l)) > 0) {
+                lanes = l;
+                break;
+            }
+    }
+    if (slots < 0) return fail(ctx, SPC_ERR_UNSUPPORTED, "too many nuclei on one electron for the lane split");
     HIP_TRY(hipSetDevice(ctx->device));
     ScParams p{};
     p.nn = nn;
     p.n1 = sys->n_nuc1;
-    p.nl1 = (sys->n_nuc1 + 1) / 2;
+    p.lanes = lanes;
+    p.nl1 = (sys->n_nuc1 + lanes - 1) / lanes;
     p.slots = slots;
     p.dt = dt;
     const double shalf = std::sqrt(0.75);
     std::vector<double> nuc(128, 0.0); // [a | sqrt(I(I+1))] x 64, draw order
-    std::vector<double> cs(4 * slots); // [slot][lane][cos, sin]
-    for (int i = 0; i < 2 * slots; ++i) cs[2 * i] = 1.0, cs[2 * i + 1] = 0.0;
+    std::vector<double> cs(2 * lanes * slots); // [slot][lane][cos, sin]
+    for (int i = 0; i < lanes * slots; ++i) cs[2 * i] = 1.0, cs[2 * i + 1] = 0.0;
     for (int k = 0; k < nn; ++k) {
         const int g = sys->mult[k];
         if (g < 2) return fail(ctx, SPC_ERR_INVALID, "multiplicity must be >= 2");
@@ -855,12 +870,12 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
         const double a = sys->a_iso[k];
         nuc[k] = a;
         nuc[64 + k] = std::sqrt(I * (I + 1.0));
-        // lane-pair slot of nucleus k (the kernel's mapping)
+        // lane and slot of nucleus k (the kernel's mapping)
         const bool e2 = k >= sys->n_nuc1;
         const int j = e2 ? k - sys->n_nuc1 : k;
-        const int slot = (j >> 1) + (e2 ? p.nl1 : 0), lane = j & 1;
-        cs[2 * (2 * slot + lane)] = std::cos(a * shalf * dt);
-        cs[2 * (2 * slot + lane) + 1] = std::sin(a * shalf * dt);
+        const int slot = j / lanes + (e2 ? p.nl1 : 0), lane = j % lanes;
+        cs[2 * (lanes * slot + lane)] = std::cos(a * shalf * dt);
+        cs[2 * (lanes * slot + lane) + 1] = std::sin(a * shalf * dt);
     }
     double nrm = std::sqrt(sys->field_dir[0] * sys->field_dir[0] + sys->field_dir[1] * sys->field_dir[1] +
                            sys->field_dir[2] * sys->field_dir[2]);
@@ -895,13 +910,13 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
     std::vector<int64_t> seeds(2 * n_samples);
     for (int64_t i = 0; i < 2 * n_samples; ++i) seeds[i] = (int64_t)g.next();
     // persistent grid: a fixed block count keeps the summation order independent of the device
-    const int64_t groups = (n_samples + kScThreads / 2 - 1) / (kScThreads / 2);
+    const int64_t groups = (n_samples + kScThreads / lanes - 1) / (kScThreads / lanes);
     const int nblocks = (int)std::min<int64_t>(groups, 1024);
     const int64_t nwaves = (int64_t)nblocks * (kScThreads / 64);
     int64_t *d_seeds = nullptr;
     double *d_part = nullptr, *d_out = nullptr, *d_tab = nullptr;
     if ((rc = ensure(ctx, "sc_seeds", (size_t)2 * n_samples, &d_seeds))) return rc;
-    if ((rc = ensure(ctx, "sc_tab", (size_t)(128 + 64 + 4 * slots), &d_tab))) return rc;
+    if ((rc = ensure(ctx, "sc_tab", (size_t)(128 + 64 + 2 * lanes * slots), &d_tab))) return rc;
     if ((rc = ensure(ctx, "sc_part", (size_t)2 * (n_steps + 1) * nwaves, &d_part))) return rc;
     if ((rc = ensure(ctx, "sc_out", (size_t)2 * (n_steps + 1), &d_out))) return rc;
     std::vector<double> tab(nuc);

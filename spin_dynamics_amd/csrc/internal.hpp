@@ -48,19 +48,21 @@ struct VecInit {
 hipError_t init_vectors(const VecInit &vi, int M, int Mp, int64_t Z, double2 *psi, hipStream_t s);
 
 // -------------------------------------------------------- semiclassical.hip
-constexpr int kScThreads = 256; // threads per block of the semiclassical kernel (two per sample)
+constexpr int kScThreads = 256; // threads per block of the semiclassical kernel
 struct ScParams {
     int nn, n1;          // nuclei, of which the first n1 couple to electron 1
-    int nl1;             // slots per lane for electron-1 nuclei: ceil(n1 / 2)
+    int lanes;           // lanes per sample: 2, 4 or 8
+    int nl1;             // slots per lane for electron-1 nuclei: ceil(n1 / lanes)
     int slots;           // slots per lane (kernel instantiation), from sc_slots
     const double *nuc;   // device [2][64]: a_k (mT), sqrt(I(I+1)) per nucleus in draw order
-    const double *cs;    // device [slots][2 lanes][cos, sin] of a_k sqrt(3)/2 dt (1, 0 for empty slots)
+    const double *cs;    // device [slots][lanes][cos, sin] of a_k sqrt(3)/2 dt (1, 0 for empty slots)
     const double *kjb;   // device [2][4][4]: exp(dt/2 (K+J)) on (S1x,S2x,T32,T23) and (T11,T22,T33,1), row-major
     double w[3];         // electron Zeeman field -B n
     double dt;
 };
-// Lane slots needed for n1 + n2 nuclei split over a lane pair; -1 if too many.
-int sc_slots(int n1, int n2);
+// Slots per lane (2 .. 32 for 2 lanes, 2 .. 12 for 4 or 8) for n1 + n2 nuclei
+// split over `lanes` lanes; -1 if more are needed.
+int sc_slots(int n1, int n2, int lanes);
 // part: 2 (n_steps + 1) nblocks kScThreads/64 doubles (zeroed here); ps, pt:
 // n_steps + 1 values of P_S(t), P_T(t).  The summation order depends on nblocks.
 hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,

@@ -1,5 +1,5 @@
 // semiclassical.hip -- semiclassical (Schulten-Wolynes / Manolopoulos-Hore)
-// singlet correlation function, one sample per lane pair.
+// singlet correlation function, one sample per group of 2, 4 or 8 lanes.
 //
 // Semantics of the reference's src/sc (sc_dynamics.f90:17-100,
 // class_radical_pair.f90:147-251), with its defects removed: electron state
@@ -10,12 +10,16 @@
 // step.  P_S = x_16/4 - tr T12 and P_S(t) = 4 <P_S(0) P_S(t)> over samples.
 //
 // Mapping to CDNA4 (FP64 VALU-bound, all state in VGPRs):
-// - A sample's nuclei are split over a lane pair; lane h of the pair holds the
-//   e1 nuclei j = 2q + h in slots q < nl1 and the e2 nuclei j = 2(q - nl1) + h
-//   in slots q >= nl1 (nl1 = ceil(n1/2) is wave-uniform, so the electron a
-//   slot couples to is a uniform branch).  The pair exchanges its partial
-//   hyperfine fields with one shuffle per component, combined in a fixed order
-//   so both lanes hold bit-identical electron states.
+// - A sample's nuclei are split over a group of L lanes (L = 2; 4 or 8 only
+//   where 2 lanes cannot hold the nuclei: every lane repeats the electron
+//   work, which costs more than the registers a wider split saves -- 0.79 s
+//   against 1.09 s and 1.41 s for 2^20 samples of 48 nuclei, 4001 steps, on
+//   MI355X); lane h of the group holds the
+//   e1 nuclei j = L q + h in slots q < nl1 and the e2 nuclei j = L (q - nl1) + h
+//   in slots q >= nl1 (nl1 = ceil(n1/L) is wave-uniform, so the electron a
+//   slot couples to is a uniform branch).  The group combines its partial
+//   hyperfine fields with a butterfly of shuffles whose additions run in a
+//   fixed order, so every lane holds a bit-identical electron state.
 // - Nuclear spins are stored scaled, I'_k = a_k I_k: the field is the plain
 //   sum of the I'_k, and a rotation commutes with the scaling.  The nuclear
 //   precession angle a_k sqrt(3)/2 dt is constant, so its cos/sin are loaded
@@ -79,7 +83,7 @@ __device__ __forceinline__ Rot make_rot(const double *f, double t)
     return r;
 }
 
-template <int NL>
+template <int NL, int L>
 __device__ __forceinline__ void fields(const double *wz, int nl1, const double (&I)[NL][3], double *w1, double *w2)
 {
     double f1[3] = {0, 0, 0}, f2[3] = {0, 0, 0};
@@ -95,13 +99,22 @@ __device__ __forceinline__ void fields(const double *wz, int nl1, const double (
             f2[2] += I[q][2];
         }
     }
-    // lane-pair combine, low lane's partial first in both lanes
-    const bool low = (threadIdx.x & 1) == 0;
+    // butterfly over the group; at each level the lower half's partial is
+    // added first, so both partners form the same sum
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) {
+        const bool low = (threadIdx.x & m) == 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double o1 = __shfl_xor(f1[d], m), o2 = __shfl_xor(f2[d], m);
+            f1[d] = low ? f1[d] + o1 : o1 + f1[d];
+            f2[d] = low ? f2[d] + o2 : o2 + f2[d];
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double o1 = __shfl_xor(f1[d], 1), o2 = __shfl_xor(f2[d], 1);
-        w1[d] = wz[d] + (low ? f1[d] + o1 : o1 + f1[d]);
-        w2[d] = wz[d] + (low ? f2[d] + o2 : o2 + f2[d]);
+        w1[d] = wz[d] + f1[d];
+        w2[d] = wz[d] + f2[d];
     }
 }
 
@@ -152,20 +165,20 @@ __device__ __forceinline__ void electron_half(double *x, const double *kA, const
 __device__ __forceinline__ double ps_of(const double *x) { return 0.25 * x[15] - (x[6] + x[10] + x[14]); }
 
 // part[st][wave] += sums over the wave's samples of (P_S(0) P_S(t_st),
-// P_S(0) x_16(t_st)), for every sample group the wave processes (kScThreads/2
+// P_S(0) x_16(t_st)), for every sample group the wave processes (kScThreads/L
 // samples per group).
-template <int NL>
+template <int NL, int L>
 __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__restrict__ seeds,
                                                    double *__restrict__ part, int64_t n_samples, int64_t n_steps,
                                                    int64_t n_groups)
 {
-    const int h = threadIdx.x & 1;
+    const int h = threadIdx.x & (L - 1);
     double kA[16], kB[16]; // wave-uniform: scalar registers
 #pragma unroll
     for (int i = 0; i < 16; ++i) kA[i] = p.kjb[i], kB[i] = p.kjb[16 + i];
     double cq[NL], sq[NL]; // this lane's nuclear precession cos/sin per slot
 #pragma unroll
-    for (int q = 0; q < NL; ++q) cq[q] = p.cs[4 * q + 2 * h], sq[q] = p.cs[4 * q + 2 * h + 1];
+    for (int q = 0; q < NL; ++q) cq[q] = p.cs[2 * (L * q + h)], sq[q] = p.cs[2 * (L * q + h) + 1];
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * (kScThreads / kWave);
     const int64_t gwave = (int64_t)blockIdx.x * (kScThreads / kWave) + (threadIdx.x >> 6);
@@ -175,7 +188,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
     const int nl1 = p.nl1;
 
     for (int64_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
-        const int64_t sample = grp * (kScThreads / 2) + (threadIdx.x >> 1);
+        const int64_t sample = grp * (kScThreads / L) + threadIdx.x / L;
         const bool active = sample < n_samples;
         double x[16];
         double I[NL][3];
@@ -193,8 +206,8 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                 const double a = p.nuc[k];
                 const bool e2 = k >= p.n1;
                 const int j = e2 ? k - p.n1 : k;
-                const int slot = (j >> 1) + (e2 ? nl1 : 0);
-                const bool mine = (j & 1) == h;
+                const int slot = j / L + (e2 ? nl1 : 0);
+                const bool mine = (j & (L - 1)) == h;
 #pragma unroll
                 for (int q = 0; q < NL; ++q)
                     if (mine && q == slot) {
@@ -212,7 +225,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
         const double p0 = ps_of(x);
         const bool counts = active && h == 0;
         double w1[3], w2[3];
-        fields<NL>(wz, nl1, I, w1, w2);
+        fields<NL, L>(wz, nl1, I, w1, w2);
         Rot r1 = make_rot(w1, th), r2 = make_rot(w2, th);
 
         for (int64_t st = 0; st <= n_steps; ++st) {
@@ -232,7 +245,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                     else
                         rotate_cs(I[q], h2, z2 ? 1.0 : cq[q], z2 ? 0.0 : sq[q]);
                 }
-                fields<NL>(wz, nl1, I, w1, w2);
+                fields<NL, L>(wz, nl1, I, w1, w2);
                 r1 = make_rot(w1, th);
                 r2 = make_rot(w2, th);
                 electron_half(x, kA, kB, r1, r2);
@@ -275,21 +288,23 @@ __global__ void k_fill_zero(double *__restrict__ a, int64_t n)
         a[i] = 0.0;
 }
 
-template <int NL>
+template <int NL, int L>
 hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
-                  int64_t n_steps, int64_t n_groups, hipStream_t s)
+                  int64_t n_steps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sc<NL>, dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, n_samples, n_steps, n_groups);
+    const int64_t n_groups = (n_samples + kScThreads / L - 1) / (kScThreads / L);
+    hipLaunchKernelGGL((k_sc<NL, L>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, n_samples, n_steps,
+                       n_groups);
     return hipGetLastError();
 }
 
 } // namespace
 
-int sc_slots(int n1, int n2)
+int sc_slots(int n1, int n2, int lanes)
 {
-    const int need = (n1 + 1) / 2 + (n2 + 1) / 2;
+    const int need = (n1 + lanes - 1) / lanes + (n2 + lanes - 1) / lanes;
     for (int nl : {2, 4, 8, 12, 16, 24, 32})
-        if (need <= nl) return nl;
+        if (need <= nl && (lanes == 2 || nl <= 12)) return nl;
     return -1;
 }
 
@@ -297,22 +312,20 @@ hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nbl
                   int64_t n_steps, double *ps, double *pt, hipStream_t s)
 {
     const int64_t nwaves = (int64_t)nblocks * (kScThreads / kWave);
-    const int64_t n_groups = (n_samples + kScThreads / 2 - 1) / (kScThreads / 2);
     const int64_t npart = 2 * (n_steps + 1) * nwaves;
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((npart + 255) / 256, 4096)), dim3(256), 0, s,
                        part, npart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    switch (p.slots) {
-    case 2: e = launch<2>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 4: e = launch<4>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 8: e = launch<8>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 12: e = launch<12>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 16: e = launch<16>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 24: e = launch<24>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    case 32: e = launch<32>(p, seeds, part, nblocks, n_samples, n_steps, n_groups, s); break;
-    default: return hipErrorInvalidValue;
-    }
+#define SPC_SC_CASE(NL, L)                                                                                         \
+    if (p.slots == NL && p.lanes == L) e = launch<NL, L>(p, seeds, part, nblocks, n_samples, n_steps, s);        \
+    else
+    SPC_SC_CASE(2, 2) SPC_SC_CASE(4, 2) SPC_SC_CASE(8, 2) SPC_SC_CASE(12, 2) SPC_SC_CASE(16, 2)
+    SPC_SC_CASE(24, 2) SPC_SC_CASE(32, 2)
+    SPC_SC_CASE(2, 4) SPC_SC_CASE(4, 4) SPC_SC_CASE(8, 4) SPC_SC_CASE(12, 4)
+    SPC_SC_CASE(2, 8) SPC_SC_CASE(4, 8) SPC_SC_CASE(8, 8) SPC_SC_CASE(12, 8)
+    return hipErrorInvalidValue;
+#undef SPC_SC_CASE
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sc_reduce, dim3((unsigned)((n_steps + 1 + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const double2 *>(part), nwaves, n_steps, n_samples, ps, pt);

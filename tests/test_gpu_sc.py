@@ -3,7 +3,7 @@ oracle (orc_semiclassical) and the model's closed forms.
 
 The device and the oracle draw the same samples (same seeds, same xoroshiro
 stream) and integrate them with the same steps; they differ in rounding
-(block-diagonal generator product, field sums split over a lane pair,
+(block-diagonal generator product, field sums split over a lane group,
 device sincos/sqrt).  Over short runs that stays below 1e-10 relative; over
 long runs the trajectories of this chaotic dynamics separate, and the two
 sample means then agree within Monte Carlo error."""
@@ -40,7 +40,7 @@ def test_short_run_matches_oracle(ctx, n_samples):
 
 
 @pytest.mark.parametrize("n1,n2", [(0, 0), (1, 0), (0, 3), (4, 4), (7, 2), (15, 1), (12, 12), (24, 24), (40, 20),
-                                   (63, 0)])
+                                   (63, 0), (63, 1), (33, 31)])
 def test_every_slot_layout_matches_oracle(ctx, n1, n2):
     rng = np.random.default_rng(n1 * 100 + n2)
     a1 = list(rng.uniform(-1, 1, n1))
@@ -49,6 +49,16 @@ def test_every_slot_layout_matches_oracle(ctx, n1, n2):
     m2 = list(rng.choice([2, 3], n2))
     (ps, pt, _, _), (ops, opt) = _both(a1, a2, m1, m2, ctx, B=0.3, J=0.02, kS=0.1, kT=0.3, n_samples=300,
                                       dt=0.04, n_steps_=12)
+    assert max_rel(ps, ops) < 1e-10
+    assert max_rel(pt, opt) < 1e-10
+
+
+@pytest.mark.parametrize("lanes", ["2", "4", "8"])
+def test_every_lane_split_matches_oracle(ctx, monkeypatch, lanes):
+    monkeypatch.setenv("SPC_SC_LANES", lanes)
+    a1, a2 = [0.8, -0.3, 0.5, 0.2, -0.9], [0.6, 0.1, -0.4]
+    (ps, pt, _, _), (ops, opt) = _both(a1, a2, [2, 3, 2, 2, 3], [2, 2, 3], ctx, B=0.2, J=0.03, kS=0.1, kT=0.2,
+                                      n_samples=700, dt=0.05, n_steps_=25)
     assert max_rel(ps, ops) < 1e-10
     assert max_rel(pt, opt) < 1e-10
 
@@ -90,9 +100,9 @@ def test_deterministic_and_seeded(ctx):
 
 def test_rejects_unsupported_layouts(ctx):
     with pytest.raises(_lib.SpinchemError):
-        sc_dynamics.run([0.1] * 63, [0.1], [2] * 63, [2], B=0.1, n_samples=10, dt=0.1, n_steps_=2, ctx=ctx)
-    with pytest.raises(_lib.SpinchemError):
         sc_dynamics.run([0.1] * 40, [0.1] * 25, [2] * 40, [2] * 25, B=0.1, n_samples=10, dt=0.1, n_steps_=2,
                         ctx=ctx)
     with pytest.raises(_lib.SpinchemError):
         sc_dynamics.run([0.1], [], [2], [], B=0.1, n_samples=0, dt=0.1, n_steps_=2, ctx=ctx)
+    with pytest.raises(_lib.SpinchemError):
+        sc_dynamics.run([0.1], [], [1], [], B=0.1, n_samples=4, dt=0.1, n_steps_=2, ctx=ctx)
