@@ -134,6 +134,23 @@ int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, doub
 /* y = H x for M vectors ([M][N] complex each), the spmv of qm_dynamics.f90:452. */
 int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y);
 
+/* Reverse-mode gradient of the final singlet yield Phi_S(T) (the last Simpson
+ * value of observables.f90:96-99 over the sample-averaged P_S of
+ * spc_trace_sample_seeds) with respect to every parameter of H: no
+ * counterpart in the reference.  tol must be 0 (one Krylov step per
+ * rebuild).  Vectors as spc_trace_sample_seeds (vec_kind SUZ or PHASE with
+ * sample_seeds [M][2]) or injected (vec_kind INJECTED, trace_vectors [M][Z]).
+ * The adjoint runs backwards on -H^H with checkpoints every checkpoint_every
+ * steps (0: 64) and recomputation in between.  Outputs (each may be NULL but
+ * Phi_S): P_S [n_out] (n_out = ceiling((t_end+dt)/dt)+1), Phi_S (scalar),
+ * grad_field [3] = d/d(B n), grad_hfc [9 per nucleus, column-major like hfc],
+ * grad_J, grad_D [9, column-major], grad_kS, grad_kT (kS, kT in the units of
+ * spc_spin_system). */
+int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
+                          const double *trace_vectors, int32_t checkpoint_every, double *P_S, int64_t n_out,
+                          double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
+                          double *grad_kS, double *grad_kT);
+
 /* ------------------------------------------------------- semiclassical */
 /* Semiclassical spin system (isotropic couplings, as src/sc uses a_iso). */
 typedef struct spc_sc_system {

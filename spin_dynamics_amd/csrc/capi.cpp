@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <complex>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -33,6 +34,28 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// One Hamiltonian on the device: its parameters, CSR (device basis order) and
+// the same matrix in 4x4 blocks over nuclear indices.
+struct HamDev {
+    DevSystem *dsys = nullptr;
+    int32_t *rowptr = nullptr, *col = nullptr;
+    double2 *val = nullptr;
+    int32_t *browptr = nullptr, *bcol = nullptr;
+    double2 *bval = nullptr;
+    int64_t nnz = 0, nblocks = 0;
+    void release()
+    {
+        hipFree(dsys);
+        hipFree(rowptr);
+        hipFree(col);
+        hipFree(val);
+        hipFree(browptr);
+        hipFree(bcol);
+        hipFree(bval);
+        *this = HamDev{};
+    }
+};
+
 } // namespace
 
 struct spc_ctx {
@@ -45,12 +68,10 @@ struct spc_ctx {
     int64_t N = 0, Z = 0, nnz = 0;
     DevSystem hsys{};
     double kS = 0.0, kT = 0.0;
-    DevSystem *d_sys = nullptr;
-    int32_t *d_rowptr = nullptr, *d_col = nullptr;
-    double2 *d_val = nullptr;
-    int32_t *d_browptr = nullptr, *d_bcol = nullptr; // 4x4 blocks over nuclear indices
-    double2 *d_bval = nullptr;
-    int64_t nblocks = 0;
+    bool iso_applied = false; // the isotropy rule replaced the tensors by trace/3
+    HamDev ham;               // H
+    HamDev adj;               // H' = -H^H (Hermitian part negated), for the adjoint
+    bool adj_valid = false;
     // workspace
     std::map<std::string, DevBuf> bufs;
     // stats
@@ -199,6 +220,59 @@ int check_sim(spc_ctx *ctx, const spc_sim_params *sim)
     return SPC_OK;
 }
 
+void set_ham(KrylovWork &w, const HamDev &h)
+{
+    w.rowptr = h.rowptr;
+    w.col = h.col;
+    w.val = h.val;
+    w.browptr = h.browptr;
+    w.bcol = h.bcol;
+    w.bval = h.bval;
+    w.nnz = h.nnz;
+}
+
+// Assembles H for the parameters S on the device: CSR by count / scan / fill
+// (getHiso_sp + coo_to_csr), then the 4x4 block form.
+int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
+{
+    const int64_t Z = S.Z, N = 4 * Z;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMalloc(&h.dsys, sizeof(DevSystem)));
+    HIP_TRY(hipMemcpyAsync(h.dsys, &S, sizeof(DevSystem), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMalloc(&h.rowptr, sizeof(int32_t) * (N + 1)));
+    int32_t *counts = nullptr;
+    int rc = ensure(ctx, "hcounts", (size_t)N + 1, &counts);
+    if (rc) return rc;
+    HIP_TRY(h_count(h.dsys, N, counts, s));
+    size_t tmp_bytes = 0;
+    HIP_TRY(h_scan(counts, h.rowptr, N, nullptr, tmp_bytes, s));
+    char *tmp = nullptr;
+    if ((rc = ensure(ctx, "scan_tmp", tmp_bytes, &tmp))) return rc;
+    HIP_TRY(h_scan(counts, h.rowptr, N, tmp, tmp_bytes, s));
+    int32_t nnz = 0;
+    HIP_TRY(hipMemcpyAsync(&nnz, h.rowptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMalloc(&h.col, sizeof(int32_t) * std::max(nnz, 1)));
+    HIP_TRY(hipMalloc(&h.val, sizeof(double2) * std::max(nnz, 1)));
+    HIP_TRY(h_fill(h.dsys, N, h.rowptr, h.col, h.val, s));
+    HIP_TRY(hipMalloc(&h.browptr, sizeof(int32_t) * (Z + 1)));
+    HIP_TRY(h_block_count(h.rowptr, h.col, Z, counts, s));
+    size_t tmp2 = 0;
+    HIP_TRY(h_scan(counts, h.browptr, Z, nullptr, tmp2, s));
+    if ((rc = ensure(ctx, "scan_tmp", std::max(tmp2, tmp_bytes), &tmp))) return rc;
+    HIP_TRY(h_scan(counts, h.browptr, Z, tmp, tmp2, s));
+    int32_t nb = 0;
+    HIP_TRY(hipMemcpyAsync(&nb, h.browptr + Z, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMalloc(&h.bcol, sizeof(int32_t) * std::max(nb, 1)));
+    HIP_TRY(hipMalloc(&h.bval, sizeof(double2) * 16 * std::max(nb, 1)));
+    HIP_TRY(h_block_fill(h.rowptr, h.col, h.val, Z, h.browptr, h.bcol, h.bval, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    h.nnz = nnz;
+    h.nblocks = nb;
+    return SPC_OK;
+}
+
 // Propagates M_total trace vectors (chunks of at most max_lanes) and writes
 // the sample-averaged populations; vi.kind selects how vectors are made.
 int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_total,
@@ -236,13 +310,7 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     w.Mp = Mp;
     w.m = m;
     w.n_out = n_out;
-    w.rowptr = ctx->d_rowptr;
-    w.col = ctx->d_col;
-    w.val = ctx->d_val;
-    w.browptr = ctx->d_browptr;
-    w.bcol = ctx->d_bcol;
-    w.bval = ctx->d_bval;
-    w.nnz = ctx->nnz;
+    set_ham(w, ctx->ham);
     w.prof = &ctx->prof;
     ctx->prof.clear();
     w.kS = ctx->kS;
@@ -380,6 +448,130 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
 
 } // namespace
 
+namespace {
+
+// Gauss-Legendre nodes and weights on [0, 1] (4 points) for the integral over
+// s of the Frechet derivative of exp.
+constexpr int kGL = 4;
+const double kGLs[kGL] = {0.5 - 0.43056815579702629, 0.5 - 0.16999052179242813, 0.5 + 0.16999052179242813,
+                          0.5 + 0.43056815579702629};
+const double kGLw[kGL] = {0.17392742256872693, 0.32607257743127307, 0.32607257743127307, 0.17392742256872693};
+
+using cd = std::complex<double>;
+
+// Electron operators on the 4-state block e = 2 s1 + s2 (alpha = 0).
+struct ElectronOps {
+    cd S1[3][4][4], S2[3][4][4], PS[4][4];
+    ElectronOps()
+    {
+        const cd h[3][2][2] = {{{0.0, 0.5}, {0.5, 0.0}},
+                               {{0.0, cd(0, -0.5)}, {cd(0, 0.5), 0.0}},
+                               {{0.5, 0.0}, {0.0, -0.5}}};
+        for (int a = 0; a < 3; ++a)
+            for (int ep = 0; ep < 4; ++ep)
+                for (int e = 0; e < 4; ++e) {
+                    const int s1p = ep >> 1, s2p = ep & 1, s1 = e >> 1, s2 = e & 1;
+                    S1[a][ep][e] = (s2p == s2) ? h[a][s1p][s1] : cd(0);
+                    S2[a][ep][e] = (s1p == s1) ? h[a][s2p][s2] : cd(0);
+                }
+        for (int ep = 0; ep < 4; ++ep)
+            for (int e = 0; e < 4; ++e) PS[ep][e] = 0.0;
+        PS[1][1] = PS[2][2] = 0.5;
+        PS[1][2] = PS[2][1] = -0.5;
+    }
+};
+
+// sum_{e',e} D[e'][e] O[e'][e]
+cd pair_sum(const cd D[4][4], const cd O[4][4])
+{
+    cd s = 0.0;
+    for (int ep = 0; ep < 4; ++ep)
+        for (int e = 0; e < 4; ++e) s += D[ep][e] * O[ep][e];
+    return s;
+}
+
+// Contracts the accumulated bilinear forms S[n][slot][e'][e] (the sum over
+// steps and quadrature nodes of conj(b) a on the neighbour pattern) with
+// dH/dtheta for the Hamiltonian's linear parameters: dF/dtheta = 2 dt Im <S,
+// dH/dtheta>.  field: d/d(B n); A: d/dA_k(a,b) at [9k + 3b + a]; T: d/dT(a,b)
+// (real part of the S1.T.S2 coupling) at [3b + a]; kS, kT through the
+// Haberkorn term -(i/2)(kS P_S + kT P_T).
+void contract_gradient(const DevSystem &sys, const std::vector<double2> &S, double dt, double field[3], double *A,
+                       double T[9], double &kS, double &kT)
+{
+    static const ElectronOps ops;
+    const int nn = sys.nn, nslot = 1 + 2 * nn;
+    const int64_t Z = sys.Z;
+    cd Dd[4][4] = {};                          // slot 0, summed over n
+    std::vector<cd> Dz((size_t)nn * 16, 0.0);  // slot 0 weighted by m_k
+    std::vector<cd> Dp((size_t)nn * 16, 0.0);  // n + stride_k, weighted by the I+ element
+    std::vector<cd> Dm((size_t)nn * 16, 0.0);  // n - stride_k, weighted by the I- element
+    std::vector<int> dig(nn);
+    for (int64_t n = 0; n < Z; ++n) {
+        for (int k = 0; k < nn; ++k) dig[k] = (int)((n / sys.stride[k]) % sys.mult[k]);
+        const double2 *Sn = S.data() + (size_t)n * nslot * 16;
+        for (int q = 0; q < 16; ++q) Dd[q >> 2][q & 3] += cd(Sn[q].x, Sn[q].y);
+        for (int k = 0; k < nn; ++k) {
+            const double I = 0.5 * (sys.mult[k] - 1), m = I - dig[k];
+            for (int q = 0; q < 16; ++q) Dz[16 * k + q] += m * cd(Sn[q].x, Sn[q].y);
+            if (dig[k] < sys.mult[k] - 1) { // column m_c = m - 1, <m|I+|m-1>
+                const double mc = m - 1.0, c = std::sqrt(I * (I + 1.0) - mc * (mc + 1.0));
+                const double2 *Sp = Sn + (1 + 2 * k) * 16;
+                for (int q = 0; q < 16; ++q) Dp[16 * k + q] += c * cd(Sp[q].x, Sp[q].y);
+            }
+            if (dig[k] > 0) { // column m_c = m + 1, <m|I-|m+1>
+                const double mc = m + 1.0, c = std::sqrt(I * (I + 1.0) - mc * (mc - 1.0));
+                const double2 *Sm = Sn + (2 + 2 * k) * 16;
+                for (int q = 0; q < 16; ++q) Dm[16 * k + q] += c * cd(Sm[q].x, Sm[q].y);
+            }
+        }
+    }
+    const double f = 2.0 * dt;
+    cd gI[3];
+    for (int a = 0; a < 3; ++a) {
+        field[a] = f * std::imag(-pair_sum(Dd, ops.S1[a]) - pair_sum(Dd, ops.S2[a]));
+        for (int b = 0; b < 3; ++b) {
+            cd O[4][4];
+            for (int ep = 0; ep < 4; ++ep)
+                for (int e = 0; e < 4; ++e) {
+                    O[ep][e] = 0.0;
+                    for (int x = 0; x < 4; ++x) O[ep][e] += ops.S1[a][ep][x] * ops.S2[b][x][e];
+                }
+            T[3 * b + a] = f * std::imag(pair_sum(Dd, O));
+        }
+    }
+    for (int k = 0; k < nn; ++k) {
+        const auto &Se = k < sys.n1 ? ops.S1 : ops.S2;
+        cd dz[4][4], dp[4][4], dm[4][4];
+        for (int q = 0; q < 16; ++q) {
+            dz[q >> 2][q & 3] = Dz[16 * k + q];
+            dp[q >> 2][q & 3] = Dp[16 * k + q];
+            dm[q >> 2][q & 3] = Dm[16 * k + q];
+        }
+        for (int a = 0; a < 3; ++a) {
+            const cd zp = pair_sum(dp, Se[a]), zm = pair_sum(dm, Se[a]);
+            const cd gx = 0.5 * (zp + zm), gy = cd(0, -0.5) * zp + cd(0, 0.5) * zm, gz = pair_sum(dz, Se[a]);
+            A[9 * k + 0 + a] = f * std::imag(gx);
+            A[9 * k + 3 + a] = f * std::imag(gy);
+            A[9 * k + 6 + a] = f * std::imag(gz);
+        }
+        cd tz = 0.0, tp = 0.0, tm = 0.0;
+        for (int e = 0; e < 4; ++e) tz += dz[e][e], tp += dp[e][e], tm += dm[e][e];
+        gI[0] = 0.5 * (tp + tm);
+        gI[1] = cd(0, -0.5) * tp + cd(0, 0.5) * tm;
+        gI[2] = tz;
+        const double gam = (sys.mult[k] == 3 ? kGamma14N : kGamma1H) / kGammaE;
+        for (int a = 0; a < 3; ++a) field[a] += f * gam * std::imag(gI[a]);
+    }
+    cd PT[4][4];
+    for (int ep = 0; ep < 4; ++ep)
+        for (int e = 0; e < 4; ++e) PT[ep][e] = (ep == e ? 1.0 : 0.0) - ops.PS[ep][e];
+    kS = f * std::imag(cd(0, -0.5) * pair_sum(Dd, ops.PS));
+    kT = f * std::imag(cd(0, -0.5) * pair_sum(Dd, PT));
+}
+
+} // namespace
+
 // ====================================================================== C-ABI
 extern "C" {
 
@@ -416,13 +608,8 @@ void spc_ctx_destroy(spc_ctx *ctx)
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->bufs)
         if (kv.second.p) hipFree(kv.second.p);
-    hipFree(ctx->d_sys);
-    hipFree(ctx->d_rowptr);
-    hipFree(ctx->d_col);
-    hipFree(ctx->d_val);
-    hipFree(ctx->d_browptr);
-    hipFree(ctx->d_bcol);
-    hipFree(ctx->d_bval);
+    ctx->ham.release();
+    ctx->adj.release();
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -492,58 +679,23 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     S.kbar = (sys->kS + 3.0 * sys->kT) / 4.0;
 
     const int64_t N = 4 * Z;
-    hipStream_t s = ctx->stream;
-    if (!ctx->d_sys) HIP_TRY(hipMalloc(&ctx->d_sys, sizeof(DevSystem)));
-    HIP_TRY(hipMemcpyAsync(ctx->d_sys, &S, sizeof(DevSystem), hipMemcpyHostToDevice, s));
-    hipFree(ctx->d_rowptr);
-    ctx->d_rowptr = nullptr;
-    HIP_TRY(hipMalloc(&ctx->d_rowptr, sizeof(int32_t) * (N + 1)));
-    int32_t *counts = nullptr;
-    int rc = ensure(ctx, "hcounts", (size_t)N + 1, &counts);
-    if (rc) return rc;
-    HIP_TRY(h_count(ctx->d_sys, N, counts, s));
-    size_t tmp_bytes = 0;
-    HIP_TRY(h_scan(counts, ctx->d_rowptr, N, nullptr, tmp_bytes, s));
-    char *tmp = nullptr;
-    if ((rc = ensure(ctx, "scan_tmp", tmp_bytes, &tmp))) return rc;
-    HIP_TRY(h_scan(counts, ctx->d_rowptr, N, tmp, tmp_bytes, s));
-    int32_t nnz = 0;
-    HIP_TRY(hipMemcpyAsync(&nnz, ctx->d_rowptr + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    hipFree(ctx->d_col);
-    hipFree(ctx->d_val);
-    ctx->d_col = nullptr;
-    ctx->d_val = nullptr;
-    HIP_TRY(hipMalloc(&ctx->d_col, sizeof(int32_t) * std::max(nnz, 1)));
-    HIP_TRY(hipMalloc(&ctx->d_val, sizeof(double2) * std::max(nnz, 1)));
-    HIP_TRY(h_fill(ctx->d_sys, N, ctx->d_rowptr, ctx->d_col, ctx->d_val, s));
-    // the same matrix in 4x4 blocks over nuclear indices (the product's format)
-    hipFree(ctx->d_browptr);
-    ctx->d_browptr = nullptr;
-    HIP_TRY(hipMalloc(&ctx->d_browptr, sizeof(int32_t) * (Z + 1)));
-    HIP_TRY(h_block_count(ctx->d_rowptr, ctx->d_col, Z, counts, s));
-    size_t tmp2 = 0;
-    HIP_TRY(h_scan(counts, ctx->d_browptr, Z, nullptr, tmp2, s));
-    if ((rc = ensure(ctx, "scan_tmp", std::max(tmp2, tmp_bytes), &tmp))) return rc;
-    HIP_TRY(h_scan(counts, ctx->d_browptr, Z, tmp, tmp2, s));
-    int32_t nb = 0;
-    HIP_TRY(hipMemcpyAsync(&nb, ctx->d_browptr + Z, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    hipFree(ctx->d_bcol);
-    hipFree(ctx->d_bval);
-    ctx->d_bcol = nullptr;
-    ctx->d_bval = nullptr;
-    HIP_TRY(hipMalloc(&ctx->d_bcol, sizeof(int32_t) * std::max(nb, 1)));
-    HIP_TRY(hipMalloc(&ctx->d_bval, sizeof(double2) * 16 * std::max(nb, 1)));
-    HIP_TRY(h_block_fill(ctx->d_rowptr, ctx->d_col, ctx->d_val, Z, ctx->d_browptr, ctx->d_bcol, ctx->d_bval, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    ctx->nblocks = nb;
+    HamDev h;
+    int rc = build_ham(ctx, S, h);
+    if (rc) {
+        h.release();
+        return rc;
+    }
+    ctx->ham.release();
+    ctx->adj.release();
+    ctx->adj_valid = false;
+    ctx->ham = h;
+    ctx->iso_applied = iso;
+    ctx->nnz = h.nnz;
     ctx->hsys = S;
     ctx->kS = sys->kS;
     ctx->kT = sys->kT;
     ctx->N = N;
     ctx->Z = Z;
-    ctx->nnz = nnz;
     ctx->have_sys = true;
     return SPC_OK;
 }
@@ -565,9 +717,9 @@ int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *va
     const int64_t N = ctx->N, Z = ctx->Z;
     std::vector<int32_t> rp(N + 1), cl(std::max<int64_t>(ctx->nnz, 1));
     std::vector<double2> vl(std::max<int64_t>(ctx->nnz, 1));
-    HIP_TRY(hipMemcpy(rp.data(), ctx->d_rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cl.data(), ctx->d_col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(vl.data(), ctx->d_val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rp.data(), ctx->ham.rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cl.data(), ctx->ham.col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vl.data(), ctx->ham.val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
     std::vector<std::pair<int32_t, double2>> row;
     int64_t pos = 0;
     for (int64_t n = 0; n < N; ++n) {
@@ -677,13 +829,7 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     w.M = M;
     w.Mp = Mp;
     w.m = std::max(m, 1);
-    w.rowptr = ctx->d_rowptr;
-    w.col = ctx->d_col;
-    w.val = ctx->d_val;
-    w.browptr = ctx->d_browptr;
-    w.bcol = ctx->d_bcol;
-    w.bval = ctx->d_bval;
-    w.nnz = ctx->nnz;
+    set_ham(w, ctx->ham);
     w.kS = ctx->kS;
     w.kT = ctx->kT;
     w.rpw1 = w.rpw2 = w.rpwp = 4;
@@ -952,6 +1098,271 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
         if (Phi_S) std::copy(phis.begin(), phis.begin() + (n_steps + 1) / 2, Phi_S);
         if (Phi_T) std::copy(phit.begin(), phit.begin() + (n_steps + 1) / 2, Phi_T);
     }
+    return SPC_OK;
+}
+
+int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
+                          const double *trace_vectors, int32_t checkpoint_every, double *P_S, int64_t n_out,
+                          double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
+                          double *grad_kS, double *grad_kT)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
+    int rc = check_sim(ctx, sim);
+    if (rc) return rc;
+    if (sim->tol != 0.0) return fail(ctx, SPC_ERR_UNSUPPORTED, "gradients need integrator_tolerance = 0");
+    const int64_t M = sim->n_samples;
+    if (M < 1) return fail(ctx, SPC_ERR_INVALID, "N_samples must be >= 1");
+    if (!Phi_S) return fail(ctx, SPC_ERR_INVALID, "Phi_S must not be NULL");
+    VecInit vi{};
+    if (sim->vec_kind == SPC_VEC_SUZ || sim->vec_kind == SPC_VEC_PHASE) {
+        if (!sample_seeds) return fail(ctx, SPC_ERR_INVALID, "sample_seeds is NULL");
+        vi.kind = sim->vec_kind;
+    } else if (sim->vec_kind == SPC_VEC_INJECTED) {
+        if (!trace_vectors) return fail(ctx, SPC_ERR_INVALID, "trace_vectors is NULL");
+        vi.kind = 2;
+    } else {
+        return fail(ctx, SPC_ERR_INVALID, "unknown vec_kind");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t n_steps = spc_n_steps(sim->t_end, sim->dt);
+    const int64_t K = steps_taken(sim->t_end, sim->dt);
+    if (P_S && n_out != n_steps + 1)
+        return fail(ctx, SPC_ERR_INVALID,
+                    "output length must be ceiling((t_end+dt)/dt)+1 = " + std::to_string(n_steps + 1));
+    const int64_t nout = n_steps + 1;
+    if (K + 1 > nout) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
+    const int m = sim->n_krylov;
+    const int64_t N = ctx->N, Z = ctx->Z;
+    const int Mp = (int)((M + 63) / 64 * 64);
+    const DevSystem &hs = ctx->hsys;
+    const int nn = hs.nn, nslot = 1 + 2 * nn;
+    const double dt = sim->dt;
+
+    // Simpson weights of the final yield: F = sum_n omega_n sum_lanes psi_n^H P_S psi_n
+    // (observables.f90:96-99, last entry), P_S(t_n) the lane average
+    const int64_t h = nout / 2;
+    std::vector<double> cw(K + 1, 0.0);
+    for (int64_t i = 1; i <= h - 1; ++i)
+        for (int64_t j = 2 * i - 2, c = 0; c < 3; ++c, ++j)
+            if (j <= K) cw[j] += (c == 1 ? 4.0 : 1.0) * dt / 3.0;
+    int64_t ntop = 0; // last step with a nonzero weight
+    for (int64_t n = 0; n <= K; ++n)
+        if (cw[n] != 0.0) ntop = n;
+    const double wscale = ctx->kS / (double)M;
+
+    // memory: basis, 3 work vectors, J b-vectors, checkpoints, a segment of stored a-vectors
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t V = (size_t)N * Mp * sizeof(double2);
+    int64_t L = checkpoint_every > 0 ? checkpoint_every : 64;
+    auto need = [&](int64_t l) { return V * ((size_t)m + 3 + kGL + (size_t)((ntop + l - 1) / l) + (size_t)l * (kGL + 1)); };
+    while (L > 1 && need(L) > free_b * 8 / 10) L /= 2;
+    if (need(L) > free_b * 8 / 10) return fail(ctx, SPC_ERR_OOM, "not enough device memory for the adjoint");
+    const int64_t nseg = std::max<int64_t>(1, (ntop + L - 1) / L);
+
+    if (!ctx->adj_valid) { // H' = -H^H: Hermitian parameters negated, Haberkorn term kept
+        DevSystem Sa = hs;
+        for (int a = 0; a < 3; ++a) Sa.w[a] = -Sa.w[a];
+        for (int k = 0; k < nn; ++k) {
+            Sa.zeta[k] = -Sa.zeta[k];
+            for (int q = 0; q < 9; ++q) Sa.A[k][q] = -Sa.A[k][q];
+        }
+        for (int q = 0; q < 9; ++q) Sa.Tre[q] = -Sa.Tre[q];
+        ctx->adj.release();
+        if ((rc = build_ham(ctx, Sa, ctx->adj))) {
+            ctx->adj.release();
+            return rc;
+        }
+        ctx->adj_valid = true;
+    }
+
+    KrylovWork w{};
+    w.N = N;
+    w.Z = Z;
+    w.M = (int)M;
+    w.Mp = Mp;
+    w.m = m;
+    w.n_out = nout;
+    set_ham(w, ctx->ham);
+    w.prof = &ctx->prof;
+    ctx->prof.clear();
+    w.kS = ctx->kS;
+    w.kT = ctx->kT;
+    const int nch = Mp / 64;
+    w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
+    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
+    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
+    w.u = w.psi;
+    if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
+    const size_t part_need = partials_needed(w);
+    if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
+    w.part_capacity = part_need;
+    if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
+    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, "E", (size_t)m * m * Mp, &w.E))) return rc;
+    if ((rc = ensure(ctx, "c", (size_t)m * Mp, &w.c))) return rc;
+    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, "lane_t", (size_t)Mp, &w.lane_t))) return rc;
+    if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
+    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
+    if ((rc = ensure(ctx, "obs", (size_t)nout * 4 * Mp, &w.obs))) return rc;
+    double2 *lam = nullptr, *bvec = nullptr, *ckpt = nullptr, *store = nullptr, *Cb = nullptr, *Sg = nullptr;
+    double *d_acc = nullptr;
+    if ((rc = ensure(ctx, "adj_lam", (size_t)N * Mp, &lam))) return rc;
+    if ((rc = ensure(ctx, "adj_b", (size_t)kGL * N * Mp, &bvec))) return rc;
+    if ((rc = ensure(ctx, "adj_ckpt", (size_t)nseg * N * Mp, &ckpt))) return rc;
+    if ((rc = ensure(ctx, "adj_store", (size_t)L * (kGL + 1) * N * Mp, &store))) return rc;
+    if ((rc = ensure(ctx, "adj_C", (size_t)kMaxTaus * m * Mp, &Cb))) return rc;
+    if ((rc = ensure(ctx, "adj_S", (size_t)Z * nslot * 16, &Sg))) return rc;
+    if ((rc = ensure(ctx, "acc", (size_t)4 * nout, &d_acc))) return rc;
+    KrylovWork wb = w; // the adjoint Krylov space: H', input lam
+    set_ham(wb, ctx->adj);
+    wb.psi = wb.u = lam;
+    const int64_t NMp = (int64_t)N * Mp;
+    auto stored = [&](int64_t n, int o) { return store + ((n % L) * (kGL + 1) + o) * NMp; };
+
+    hipStream_t s = ctx->stream;
+    int64_t *d_seeds = nullptr;
+    double2 *d_inj = nullptr;
+    if (vi.kind == 2) {
+        if ((rc = ensure(ctx, "inj", (size_t)Mp * Z, &d_inj))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_inj, trace_vectors, sizeof(double2) * (size_t)M * Z, hipMemcpyHostToDevice, s));
+        vi.inj = d_inj;
+    } else {
+        if ((rc = ensure(ctx, "seeds", (size_t)2 * Mp, &d_seeds))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_seeds, sample_seeds, sizeof(int64_t) * 2 * M, hipMemcpyHostToDevice, s));
+        vi.seeds = d_seeds;
+    }
+    double pop[4];
+    theta_of(sim->init_state, vi.theta, pop);
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, s));
+
+    // forward, with checkpoints every L steps and lam = omega P_S psi at ntop
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * nout, s));
+    HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
+    HIP_TRY(init_vectors(vi, (int)M, Mp, Z, w.psi, s));
+    HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, s));
+    HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
+    HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
+    HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
+    {
+        std::vector<double> obs0((size_t)4 * Mp, 0.0);
+        for (int x = 0; x < 4; ++x)
+            for (int r = 0; r < M; ++r) obs0[(size_t)x * Mp + r] = pop[x];
+        HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    for (int64_t n = 0; n < K; ++n) {
+        if (n < ntop && n % L == 0)
+            HIP_TRY(hipMemcpyAsync(ckpt + (n / L) * NMp, w.psi, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
+        if (n == ntop) HIP_TRY(singlet_source(w.psi, cw[ntop] * wscale, lam, Z, Mp, s));
+        HIP_TRY(arnoldi(w, s));
+        HIP_TRY(expm_lanes(w, dt, s));
+        HIP_TRY(step_lanes(w, dt, sim->t_end, 0.0, false, s));
+        HIP_TRY(psi_update(w, true, s));
+    }
+    if (ntop == K) HIP_TRY(singlet_source(w.psi, cw[ntop] * wscale, lam, Z, Mp, s));
+    int32_t flags[4];
+    HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    HIP_TRY(obs_reduce_lanes(w.obs, nout, (int)M, Mp, d_acc, s));
+    std::vector<double> acc((size_t)4 * nout);
+    HIP_TRY(hipMemcpyAsync(acc.data(), d_acc, sizeof(double) * 4 * nout, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (flags[0])
+        return fail(ctx, SPC_ERR_BREAKDOWN,
+                    "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+    std::vector<double> ps(nout);
+    for (int64_t k = 0; k < nout; ++k) ps[k] = acc[k] / (double)M;
+    double Funit = 0.0; // the yield per unit kS
+    for (int64_t n = 0; n <= K; ++n) Funit += cw[n] * ps[n];
+
+    // backward sweep, segment by segment
+    HIP_TRY(hipMemsetAsync(Sg, 0, sizeof(double2) * (size_t)Z * nslot * 16, s));
+    HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
+    if (ctx->kS != 0.0 && ntop > 0) {
+        TauSet tf{}, tb{};
+        tf.n = kGL + 2; // psi_n, a(s_j), psi_{n+1}
+        tf.tau[0] = 0.0, tf.scale[0] = 1.0;
+        for (int j = 0; j < kGL; ++j) tf.tau[1 + j] = kGLs[j] * dt, tf.scale[1 + j] = 1.0;
+        tf.tau[kGL + 1] = dt, tf.scale[kGL + 1] = 1.0;
+        tb.n = kGL + 1; // U^H lam, w_j b(s_j)
+        tb.tau[0] = dt, tb.scale[0] = 1.0;
+        for (int j = 0; j < kGL; ++j) tb.tau[1 + j] = (1.0 - kGLs[j]) * dt, tb.scale[1 + j] = kGLw[j];
+        for (int64_t seg = nseg - 1; seg >= 0; --seg) {
+            const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, ntop);
+            HIP_TRY(hipMemcpyAsync(w.psi, ckpt + seg * NMp, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
+            for (int64_t n = n0; n < n1; ++n) { // recompute: psi_n and a_n(s_j) kept
+                HIP_TRY(arnoldi(w, s));
+                HIP_TRY(expm_coeffs(w, tf, Cb, s));
+                CombineOut out{};
+                out.n = kGL + 2;
+                for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
+                out.y[kGL + 1] = w.psi;
+                HIP_TRY(combine(w.Q, m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+            }
+            for (int64_t n = n1 - 1; n >= n0; --n) { // lam_{n+1} -> lam_n, density of step n
+                HIP_TRY(arnoldi(wb, s));
+                HIP_TRY(expm_coeffs(wb, tb, Cb, s));
+                CombineOut out{};
+                out.n = kGL + 1;
+                out.y[0] = lam;
+                for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
+                HIP_TRY(combine(wb.Q, m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
+                HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(ev1, s));
+    std::vector<double2> Sh((size_t)Z * nslot * 16);
+    HIP_TRY(hipMemcpyAsync(Sh.data(), Sg, sizeof(double2) * Sh.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    ctx->last_ms = ms;
+    ctx->last_rebuilds = K + 2 * ntop;
+    ctx->last_steps = K;
+    if (flags[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
+
+    double gf[3], gT[9], gkS, gkT;
+    std::vector<double> gA((size_t)9 * std::max(nn, 1));
+    contract_gradient(hs, Sh, dt, gf, gA.data(), gT, gkS, gkT);
+    *Phi_S = ctx->kS * Funit;
+    if (P_S)
+        for (int64_t k = 0; k < nout; ++k) P_S[k] = ps[k];
+    if (grad_field)
+        for (int a = 0; a < 3; ++a) grad_field[a] = gf[a];
+    if (grad_hfc) {
+        for (int k = 0; k < nn; ++k) {
+            if (ctx->iso_applied) { // the tensors entered as trace/3
+                const double t = (gA[9 * k + 0] + gA[9 * k + 4] + gA[9 * k + 8]) / 3.0;
+                for (int q = 0; q < 9; ++q) grad_hfc[9 * k + q] = (q % 4 == 0) ? t : 0.0;
+            } else {
+                for (int q = 0; q < 9; ++q) grad_hfc[9 * k + q] = gA[9 * k + q];
+            }
+        }
+    }
+    if (grad_J) *grad_J = 2.0 * (gT[0] + gT[4] + gT[8]);
+    if (grad_D)
+        for (int q = 0; q < 9; ++q) grad_D[q] = gT[q];
+    // kS and kT also set the Haberkorn coupling of S1.T.S2 and kbar; both are
+    // in the -(i/2)(kS P_S + kT P_T) contraction; kS also scales the yield
+    if (grad_kS) *grad_kS = gkS + Funit;
+    if (grad_kT) *grad_kT = gkT;
+    for (int k = 0; k < nn && grad_hfc; ++k)
+        for (int q = 0; q < 9; ++q)
+            if (!std::isfinite(grad_hfc[9 * k + q])) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite gradient");
     return SPC_OK;
 }
 

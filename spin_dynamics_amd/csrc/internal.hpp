@@ -160,5 +160,32 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s); // psi = Q c 
 hipError_t obs_reduce_lanes(const double *obs, int64_t n_out, int M, int Mp, double *acc, hipStream_t s);
 hipError_t spmm_only(KrylovWork &w, hipStream_t s);              // v = H u
 hipError_t expm_single(int m, const double2 *A, double2 *E, hipStream_t s);
+// Times (and scales) for expm_coeffs: C[o] = scale_o beta exp(-i tau_o A) e_1.
+constexpr int kMaxTaus = 8;
+struct TauSet {
+    int n;
+    double tau[kMaxTaus];
+    double scale[kMaxTaus];
+};
+hipError_t expm_coeffs(KrylovWork &w, const TauSet &ts, double2 *C, hipStream_t s); // hess, beta0 -> C[n][m][Mp]
+
+// ---------------------------------------------------------------- adjoint.hip
+// Up to kCombineOut outputs y_o = sum_k C[o][k] q_k (+ omega P_S src for o = 0)
+// from one read of the basis Q ([m][N][Mp]); outputs with y[o] == nullptr
+// skipped; lanes r >= M written as zeros.
+constexpr int kCombineOut = 6;
+struct CombineOut {
+    double2 *y[kCombineOut];
+    int n;
+};
+hipError_t combine(const double2 *Q, int m, const double2 *C, const CombineOut &out, const double2 *src, double omega,
+                   int64_t Z, int M, int Mp, hipStream_t s);
+// lam = omega P_S psi (P_S the singlet projector on each electron block).
+hipError_t singlet_source(const double2 *psi, double omega, double2 *lam, int64_t Z, int Mp, hipStream_t s);
+// S[n][slot][e'][e] += sum_j sum_lanes conj(b_j[4n+e']) a_j[4n_s+e] over the
+// nuclear neighbours n_s of n (slot 0: n; 1+2k: n + stride_k; 2+2k: n - stride_k,
+// where the digit allows), for J vectors a_j, b_j (strided by NMp).
+hipError_t grad_density(const double2 *a, const double2 *b, int J, int64_t NMp, const DevSystem *d_sys,
+                        const DevSystem &h_sys, int M, int Mp, double2 *S, hipStream_t s);
 
 } // namespace spc

@@ -614,6 +614,33 @@ __global__ __launch_bounds__(kBlock) void k_expm(int m, int Mp, int M, double dt
     }
 }
 
+// C[o][k][r] = scale_o beta_r (exp(-i tau_o A_r))_{k,0}: the Krylov coefficients
+// of exp(-i tau_o H) psi_r = Q_r exp(-i tau_o A_r) e_1 beta_r for several times
+// tau_o at once (tau = 0 gives beta e_1).  Grid (Mp, ntau).
+__global__ __launch_bounds__(kBlock) void k_expm_coeffs(int m, int Mp, int M, TauSet ts,
+                                                         const double2 *__restrict__ hess,
+                                                         const double *__restrict__ beta0, double2 *__restrict__ C)
+{
+    extern __shared__ double2 lds[];
+    const int r = blockIdx.x, o = blockIdx.y;
+    const int mm = m * m;
+    double2 *X = lds, *P = X + mm, *E = P + mm, *D = E + mm;
+    double2 *out = C + (int64_t)o * m * Mp;
+    if (r >= M) {
+        for (int k = threadIdx.x; k < m; k += blockDim.x) out[(int64_t)k * Mp + r] = cz(0, 0);
+        return;
+    }
+    const double tau = ts.tau[o];
+    for (int idx = threadIdx.x; idx < mm; idx += blockDim.x) {
+        const double2 h = hess[(int64_t)idx * Mp + r];
+        X[idx] = cz(tau * h.y, -tau * h.x); // (0,-1)*tau*h
+    }
+    __syncthreads();
+    expm_block(m, X, P, E, D);
+    const double f = ts.scale[o] * beta0[r];
+    for (int k = threadIdx.x; k < m; k += blockDim.x) out[(int64_t)k * Mp + r] = cscale(f, E[k]);
+}
+
 } // namespace
 
 size_t expm_lds_bytes(int m) { return 4 * (size_t)m * m * sizeof(double2); }
@@ -821,6 +848,15 @@ hipError_t expm_single(int m, const double2 *A, double2 *E, hipStream_t s)
 {
     const size_t lds = expm_lds_bytes(m);
     hipLaunchKernelGGL(k_expm, dim3(1), dim3(kBlock), lds, s, m, 1, 1, 0.0, A, E, 1);
+    return hipGetLastError();
+}
+
+hipError_t expm_coeffs(KrylovWork &w, const TauSet &ts, double2 *C, hipStream_t s)
+{
+    const size_t lds = expm_lds_bytes(w.m);
+    ProfScope ps(w, PF_EXPM, 0.0, s);
+    hipLaunchKernelGGL(k_expm_coeffs, dim3(w.Mp, ts.n), dim3(kBlock), lds, s, w.m, w.Mp, w.M, ts, w.hess, w.beta0,
+                       C);
     return hipGetLastError();
 }
 

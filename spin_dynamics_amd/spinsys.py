@@ -88,8 +88,9 @@ class SimParam:
         return self.restart is not None
 
 
-def to_c_system(sys: SysParam, B: float) -> "tuple[_lib.SpinSystem, tuple]":
-    """Flattens sys_param for spc_system_set (field magnitude B = -w)."""
+def to_c_system(sys: SysParam, B: float, iso_rule: int = 1) -> "tuple[_lib.SpinSystem, tuple]":
+    """Flattens sys_param for spc_system_set (field magnitude B = -w); iso_rule
+    1 applies the reference's isotropy test (spinsys.f90:237-254)."""
     mult = np.ascontiguousarray(list(sys.e1.g_I) + list(sys.e2.g_I), dtype=np.int32)
     A = [np.asarray(a, float).reshape(3, 3) for a in list(sys.e1.A) + list(sys.e2.A)]
     hfc = np.ascontiguousarray(np.array([a.T.reshape(-1) for a in A]).reshape(-1) if A else np.zeros(1),
@@ -105,7 +106,7 @@ def to_c_system(sys: SysParam, B: float) -> "tuple[_lib.SpinSystem, tuple]":
     s.kS, s.kT, s.B = float(sys.kS), float(sys.kT), float(B)
     for i in range(3):
         s.field_dir[i] = float(sys.field_dir[i])
-    s.iso_rule = 1
+    s.iso_rule = int(iso_rule)
     return s, (mult, hfc)
 
 
