@@ -140,8 +140,10 @@ int spc_spmm(spc_ctx *ctx, int32_t M, const double *x, double *y);
  * counterpart in the reference.  tol must be 0 (one Krylov step per
  * rebuild).  Vectors as spc_trace_sample_seeds (vec_kind SUZ or PHASE with
  * sample_seeds [M][2]) or injected (vec_kind INJECTED, trace_vectors [M][Z]).
- * The adjoint runs backwards on -H^H with checkpoints every checkpoint_every
- * steps (0: 64) and recomputation in between.  Outputs (each may be NULL but
+ * The adjoint runs backwards on -H^H.  The per-step vectors of the last steps
+ * are kept from the forward pass as far as 80% of the free device memory
+ * allows (environment SPC_ADJOINT_KEEP caps their number); earlier steps are
+ * recomputed from checkpoints every checkpoint_every steps (0: 64).  Outputs (each may be NULL but
  * Phi_S): P_S [n_out] (n_out = ceiling((t_end+dt)/dt)+1), Phi_S (scalar),
  * grad_field [3] = d/d(B n), grad_hfc [9 per nucleus, column-major like hfc],
  * grad_J, grad_D [9, column-major], grad_kS, grad_kT (kS, kT in the units of

@@ -1151,15 +1151,30 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         if (cw[n] != 0.0) ntop = n;
     const double wscale = ctx->kS / (double)M;
 
-    // memory: basis, 3 work vectors, J b-vectors, checkpoints, a segment of stored a-vectors
+    // memory: basis, 3 work vectors, J b-vectors, checkpoints every L steps, a
+    // segment of recomputed step vectors (psi_n and the a_n(s_j)), and the step
+    // vectors of the last `keep` steps kept from the first pass (as many as
+    // 80% of the free memory allows; those steps are not recomputed)
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    for (const char *nm : {"adj_store", "adj_ckpt"}) { // our own buffers are reusable
+        auto it = ctx->bufs.find(nm);
+        if (it != ctx->bufs.end()) free_b += it->second.bytes;
+    }
+    const size_t budget = free_b / 10 * 8;
     const size_t V = (size_t)N * Mp * sizeof(double2);
+    const size_t base = V * ((size_t)m + 3 + kGL);
     int64_t L = checkpoint_every > 0 ? checkpoint_every : 64;
-    auto need = [&](int64_t l) { return V * ((size_t)m + 3 + kGL + (size_t)((ntop + l - 1) / l) + (size_t)l * (kGL + 1)); };
-    while (L > 1 && need(L) > free_b * 8 / 10) L /= 2;
-    if (need(L) > free_b * 8 / 10) return fail(ctx, SPC_ERR_OOM, "not enough device memory for the adjoint");
-    const int64_t nseg = std::max<int64_t>(1, (ntop + L - 1) / L);
+    auto need = [&](int64_t l, int64_t kp) {
+        return base + V * ((size_t)((ntop - kp + l - 1) / l) + (size_t)(l + kp) * (kGL + 1));
+    };
+    while (L > 1 && need(L, 0) > budget) L /= 2;
+    if (need(L, 0) > budget) return fail(ctx, SPC_ERR_OOM, "not enough device memory for the adjoint");
+    int64_t keep = std::min<int64_t>(ntop, (int64_t)((budget - need(L, 0)) / (V * (kGL + 1))));
+    if (const char *env = std::getenv("SPC_ADJOINT_KEEP")) // cap on kept steps (tuning and tests)
+        keep = std::min<int64_t>(keep, std::max<int64_t>(0, std::atoll(env)));
+    const int64_t kstart = ntop - keep;         // steps [kstart, ntop) keep their vectors
+    const int64_t nseg = (kstart + L - 1) / L;  // recomputed segments cover [0, kstart)
 
     if (!ctx->adj_valid) { // H' = -H^H: Hermitian parameters negated, Haberkorn term kept
         DevSystem Sa = hs;
@@ -1216,8 +1231,8 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     double *d_acc = nullptr;
     if ((rc = ensure(ctx, "adj_lam", (size_t)N * Mp, &lam))) return rc;
     if ((rc = ensure(ctx, "adj_b", (size_t)kGL * N * Mp, &bvec))) return rc;
-    if ((rc = ensure(ctx, "adj_ckpt", (size_t)nseg * N * Mp, &ckpt))) return rc;
-    if ((rc = ensure(ctx, "adj_store", (size_t)L * (kGL + 1) * N * Mp, &store))) return rc;
+    if ((rc = ensure(ctx, "adj_ckpt", (size_t)std::max<int64_t>(nseg, 1) * N * Mp, &ckpt))) return rc;
+    if ((rc = ensure(ctx, "adj_store", (size_t)(L + keep) * (kGL + 1) * N * Mp, &store))) return rc;
     if ((rc = ensure(ctx, "adj_C", (size_t)kMaxTaus * m * Mp, &Cb))) return rc;
     if ((rc = ensure(ctx, "adj_S", (size_t)Z * nslot * 16, &Sg))) return rc;
     if ((rc = ensure(ctx, "acc", (size_t)4 * nout, &d_acc))) return rc;
@@ -1225,7 +1240,20 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     set_ham(wb, ctx->adj);
     wb.psi = wb.u = lam;
     const int64_t NMp = (int64_t)N * Mp;
-    auto stored = [&](int64_t n, int o) { return store + ((n % L) * (kGL + 1) + o) * NMp; };
+    auto stored = [&](int64_t n, int o) {
+        const int64_t slot = n >= kstart ? L + (n - kstart) : n % L;
+        return store + (slot * (kGL + 1) + o) * NMp;
+    };
+    TauSet tf{}, tb{};
+    tf.n = kGL + 2; // psi_n, a(s_j), psi_{n+1}
+    tf.tau[0] = 0.0, tf.scale[0] = 1.0;
+    for (int j = 0; j < kGL; ++j) tf.tau[1 + j] = kGLs[j] * dt, tf.scale[1 + j] = 1.0;
+    tf.tau[kGL + 1] = dt, tf.scale[kGL + 1] = 1.0;
+    TauSet tk = tf; // first pass: psi_n and a(s_j) only
+    tk.n = kGL + 1;
+    tb.n = kGL + 1; // U^H lam, w_j b(s_j)
+    tb.tau[0] = dt, tb.scale[0] = 1.0;
+    for (int j = 0; j < kGL; ++j) tb.tau[1 + j] = (1.0 - kGLs[j]) * dt, tb.scale[1 + j] = kGLw[j];
 
     hipStream_t s = ctx->stream;
     int64_t *d_seeds = nullptr;
@@ -1261,11 +1289,19 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
+    const bool backward = ctx->kS != 0.0 && ntop > 0;
     for (int64_t n = 0; n < K; ++n) {
-        if (n < ntop && n % L == 0)
+        if (n < kstart && n % L == 0)
             HIP_TRY(hipMemcpyAsync(ckpt + (n / L) * NMp, w.psi, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
         if (n == ntop) HIP_TRY(singlet_source(w.psi, cw[ntop] * wscale, lam, Z, Mp, s));
         HIP_TRY(arnoldi(w, s));
+        if (backward && n >= kstart && n < ntop) {
+            HIP_TRY(expm_coeffs(w, tk, Cb, s));
+            CombineOut out{};
+            out.n = kGL + 1;
+            for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
+            HIP_TRY(combine(w.Q, m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+        }
         HIP_TRY(expm_lanes(w, dt, s));
         HIP_TRY(step_lanes(w, dt, sim->t_end, 0.0, false, s));
         HIP_TRY(psi_update(w, true, s));
@@ -1288,17 +1324,22 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     // backward sweep, segment by segment
     HIP_TRY(hipMemsetAsync(Sg, 0, sizeof(double2) * (size_t)Z * nslot * 16, s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
-    if (ctx->kS != 0.0 && ntop > 0) {
-        TauSet tf{}, tb{};
-        tf.n = kGL + 2; // psi_n, a(s_j), psi_{n+1}
-        tf.tau[0] = 0.0, tf.scale[0] = 1.0;
-        for (int j = 0; j < kGL; ++j) tf.tau[1 + j] = kGLs[j] * dt, tf.scale[1 + j] = 1.0;
-        tf.tau[kGL + 1] = dt, tf.scale[kGL + 1] = 1.0;
-        tb.n = kGL + 1; // U^H lam, w_j b(s_j)
-        tb.tau[0] = dt, tb.scale[0] = 1.0;
-        for (int j = 0; j < kGL; ++j) tb.tau[1 + j] = (1.0 - kGLs[j]) * dt, tb.scale[1 + j] = kGLw[j];
+    auto backward_step = [&](int64_t n) -> int { // lam_{n+1} -> lam_n, and the density of step n
+        HIP_TRY(arnoldi(wb, s));
+        HIP_TRY(expm_coeffs(wb, tb, Cb, s));
+        CombineOut out{};
+        out.n = kGL + 1;
+        out.y[0] = lam;
+        for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
+        HIP_TRY(combine(wb.Q, m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
+        HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
+        return SPC_OK;
+    };
+    if (backward) {
+        for (int64_t n = ntop - 1; n >= kstart; --n)
+            if ((rc = backward_step(n))) return rc;
         for (int64_t seg = nseg - 1; seg >= 0; --seg) {
-            const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, ntop);
+            const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, kstart);
             HIP_TRY(hipMemcpyAsync(w.psi, ckpt + seg * NMp, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
             for (int64_t n = n0; n < n1; ++n) { // recompute: psi_n and a_n(s_j) kept
                 HIP_TRY(arnoldi(w, s));
@@ -1309,16 +1350,8 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
                 out.y[kGL + 1] = w.psi;
                 HIP_TRY(combine(w.Q, m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
             }
-            for (int64_t n = n1 - 1; n >= n0; --n) { // lam_{n+1} -> lam_n, density of step n
-                HIP_TRY(arnoldi(wb, s));
-                HIP_TRY(expm_coeffs(wb, tb, Cb, s));
-                CombineOut out{};
-                out.n = kGL + 1;
-                out.y[0] = lam;
-                for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
-                HIP_TRY(combine(wb.Q, m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
-                HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
-            }
+            for (int64_t n = n1 - 1; n >= n0; --n)
+                if ((rc = backward_step(n))) return rc;
         }
     }
     HIP_TRY(hipEventRecord(ev1, s));
@@ -1331,7 +1364,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
     ctx->last_ms = ms;
-    ctx->last_rebuilds = K + 2 * ntop;
+    ctx->last_rebuilds = K + ntop + kstart;
     ctx->last_steps = K;
     if (flags[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
 

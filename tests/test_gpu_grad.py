@@ -4,7 +4,8 @@ Pins: central finite differences of the device forward (the same trace
 vectors, tol = 0) for every parameter of H; central finite differences of
 the oracle's forward (orc_trace_sampling with injected vectors) for a set of
 parameters, so the gradient is that of the reference algorithm's yield;
-independence of the checkpoint interval; the yield itself equal to the
+independence of the checkpoint interval and of how many final steps keep
+their vectors instead of being recomputed; the yield itself equal to the
 forward run's; and the torch.autograd.Function's chain rule to the
 (symmetric tensor, |B|, theta, phi, J, kS, kT) parameterisation, again
 against finite differences.  Tolerance: 1e-6 relative to the largest
@@ -79,10 +80,13 @@ def _adj_value(g, name, idx):
     return getattr(g, name)
 
 
-def test_yield_matches_forward_and_every_parameter_matches_fd(ctx):
+def test_yield_matches_forward_and_every_parameter_matches_fd(ctx, monkeypatch):
     w = _small()
     g = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)
     F0 = _forward_yield(w.sys, w.sim, w.B, ctx)
+    monkeypatch.setenv("SPC_ADJOINT_KEEP", "0")
+    g0 = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=5, ctx=ctx)
+    assert np.max(np.abs(g0.hfc - g.hfc)) <= 1e-12 * np.max(np.abs(g.hfc))
     assert abs(g.Phi_S - F0) <= 1e-12 * abs(F0)
     fwd = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     assert np.max(np.abs(g.P_S - fwd.P_S)) < 1e-13
@@ -134,10 +138,12 @@ def test_gradient_matches_oracle_finite_differences(ctx):
     assert np.max(np.abs(adj - fd)) <= 1e-6 * np.max(np.abs(fd)), (adj, fd)
 
 
-@pytest.mark.parametrize("every", [1, 7, 64])
-def test_checkpoint_interval_does_not_change_the_gradient(ctx, every):
+@pytest.mark.parametrize("every,keep", [(1, "0"), (7, "0"), (64, "0"), (7, "13"), (64, None)])
+def test_checkpoint_interval_does_not_change_the_gradient(ctx, monkeypatch, every, keep):
     w = _small(T_ns=50.0)
-    ref = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=64, ctx=ctx)
+    ref = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=64, ctx=ctx)  # all steps kept
+    if keep is not None:
+        monkeypatch.setenv("SPC_ADJOINT_KEEP", keep)  # recompute all but the last `keep` steps
     g = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=every, ctx=ctx)
     s = np.max(np.abs(ref.hfc))
     assert np.max(np.abs(g.hfc - ref.hfc)) <= 1e-12 * s
