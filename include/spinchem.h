@@ -153,6 +153,29 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
                           double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
                           double *grad_kS, double *grad_kT);
 
+/* Many field orientations of one radical pair at once: no counterpart in the
+ * reference, which runs one field per call (app/main.f90:48-79).  The system
+ * of spc_system_set (its field magnitude B; its direction is replaced) at
+ * n_orient directions dirs [n][3], with the same M trace vectors (sample_seeds
+ * [M][2], as spc_trace_sample_seeds) for every orientation; tol must be 0 and
+ * M a multiple of 64.  Orientations share one block pattern of H and go into
+ * the lane dimension, orient_per_chunk at a time (0: as many as fit); with
+ * use_graph the step is captured once in a hipGraph and replayed.  Outputs:
+ * Phi_S [n_orient] final singlet yields; P_S [n_orient][n_out] or NULL. */
+int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                          const double *dirs, int32_t orient_per_chunk, int32_t use_graph, double *Phi_S, double *P_S,
+                          int64_t n_out);
+
+/* Autotuner for spc_orientation_batch (no counterpart in the reference):
+ * the smallest Krylov order (down from sim->n_krylov in steps of 2) whose
+ * pilot yields (64 steps, up to 4 orientations) stay within yield_tol of the
+ * requested order's, scaled to the pilot's share of the run, and the
+ * orientations per chunk (power of two) with the least device time per
+ * orientation-step over a short timed run. */
+int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                   const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
+                   double *ms_per_orient_step);
+
 /* ------------------------------------------------------- semiclassical */
 /* Semiclassical spin system (isotropic couplings, as src/sc uses a_iso). */
 typedef struct spc_sc_system {

@@ -72,6 +72,11 @@ struct spc_ctx {
     HamDev ham;               // H
     HamDev adj;               // H' = -H^H (Hermitian part negated), for the adjoint
     bool adj_valid = false;
+    // field basis for batched orientations: a block pattern that holds H at
+    // any field direction, with H(0) and the Zeeman operators d H / d(B n)_a
+    HamDev fpat;
+    double2 *fv0 = nullptr, *fz[3] = {nullptr, nullptr, nullptr};
+    bool fbasis_valid = false;
     // workspace
     std::map<std::string, DevBuf> bufs;
     // stats
@@ -572,6 +577,290 @@ void contract_gradient(const DevSystem &sys, const std::vector<double2> &S, doub
 
 } // namespace
 
+namespace {
+
+// Builds (once per system) the field basis: the block pattern of H at a
+// generic direction, and on it H(B = 0) and Z_a = H(B n = e_a) - H(0)
+// (electron and nuclear Zeeman terms are linear in the field vector).
+int build_field_basis(spc_ctx *ctx)
+{
+    if (ctx->fbasis_valid) return SPC_OK;
+    const DevSystem &hs = ctx->hsys;
+    const double Bmag = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
+    auto with_field = [&](const double b[3]) {
+        DevSystem S = hs;
+        const double nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        for (int a = 0; a < 3; ++a) {
+            S.w[a] = -b[a];
+            S.dir[a] = nb > 0.0 ? b[a] / nb : (a == 2 ? 1.0 : 0.0);
+        }
+        for (int k = 0; k < hs.nn; ++k) S.zeta[k] = nb * (hs.mult[k] == 3 ? kGamma14N : kGamma1H) / kGammaE;
+        return S;
+    };
+    // generic direction: every Zeeman term present
+    const double g0 = std::max(Bmag, 1.0);
+    const double gen[3] = {0.3 * g0, 0.5 * g0, 0.8 * g0};
+    ctx->fpat.release();
+    int rc = build_ham(ctx, with_field(gen), ctx->fpat);
+    if (rc) return rc;
+    const int64_t nb = ctx->fpat.nblocks, Z = ctx->Z;
+    if ((rc = ensure(ctx, "fv0", (size_t)nb * 16, &ctx->fv0))) return rc;
+    const char *names[3] = {"fzx", "fzy", "fzz"};
+    for (int a = 0; a < 3; ++a)
+        if ((rc = ensure(ctx, names[a], (size_t)nb * 16, &ctx->fz[a]))) return rc;
+    double2 *tmp = nullptr;
+    int32_t *missing = nullptr;
+    double *neg = nullptr;
+    if ((rc = ensure(ctx, "ftmp", (size_t)nb * 16, &tmp))) return rc;
+    if ((rc = ensure(ctx, "fmiss", (size_t)1, &missing))) return rc;
+    if ((rc = ensure(ctx, "fneg", (size_t)3, &neg))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemsetAsync(missing, 0, sizeof(int32_t), s));
+    const double m1[3] = {-1.0, 0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(neg, m1, sizeof(m1), hipMemcpyHostToDevice, s));
+    for (int a = -1; a < 3; ++a) {
+        double b[3] = {0.0, 0.0, 0.0};
+        if (a >= 0) b[a] = 1.0;
+        HamDev h;
+        if ((rc = build_ham(ctx, with_field(b), h))) {
+            h.release();
+            return rc;
+        }
+        double2 *dst = a < 0 ? ctx->fv0 : tmp;
+        HIP_TRY(h_block_fill_pattern(h.rowptr, h.col, h.val, Z, ctx->fpat.browptr, ctx->fpat.bcol, dst, missing, s));
+        if (a >= 0) // Z_a = H(e_a) - H(0)
+            HIP_TRY(h_field_values(tmp, ctx->fv0, ctx->fv0, ctx->fv0, neg, nb * 16, 1, ctx->fz[a], s));
+        HIP_TRY(hipStreamSynchronize(s));
+        h.release();
+    }
+    int32_t miss = 0;
+    HIP_TRY(hipMemcpy(&miss, missing, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (miss) return fail(ctx, SPC_ERR_INVALID, "field basis: an entry falls outside the generic block pattern");
+    ctx->fbasis_valid = true;
+    return SPC_OK;
+}
+
+struct BatchPlan {
+    int opc = 0;       // orientations per chunk
+    int m = 0;         // Krylov order
+    bool graph = true; // replay one captured step
+};
+
+// Runs orientations [o0, o0 + n) of dirs in one lane batch (Mo = M lanes per
+// orientation, M a multiple of 64) for K steps; adds each orientation's summed
+// populations to acc[o][4 * n_out].
+int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const int64_t *sample_seeds, int M,
+                          const double *dirs, int o0, int n, bool graph, int64_t K, int64_t nout, double *d_acc,
+                          double *out_ms)
+{
+    const int64_t N = ctx->N, Z = ctx->Z;
+    const int Mp = n * M;
+    const DevSystem &hs = ctx->hsys;
+    const double Bmag = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
+    const int64_t nb = ctx->fpat.nblocks;
+    int rc;
+    KrylovWork w{};
+    w.N = N;
+    w.Z = Z;
+    w.M = Mp;
+    w.Mp = Mp;
+    w.m = m;
+    w.n_out = nout;
+    set_ham(w, ctx->fpat);
+    w.bval_stride = nb * 16;
+    w.chunks_per_bval = M / 64;
+    w.prof = nullptr;
+    w.kS = ctx->kS;
+    w.kT = ctx->kT;
+    const int nch = Mp / 64;
+    w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
+    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
+    double2 *vals = nullptr;
+    double *d_fields = nullptr;
+    int64_t *d_seeds = nullptr;
+    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
+    w.u = w.psi;
+    if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
+    const size_t part_need = partials_needed(w);
+    if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
+    w.part_capacity = part_need;
+    if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
+    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
+    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, "E", (size_t)m * m * Mp, &w.E))) return rc;
+    if ((rc = ensure(ctx, "c", (size_t)m * Mp, &w.c))) return rc;
+    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, "lane_t", (size_t)Mp, &w.lane_t))) return rc;
+    if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
+    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
+    if ((rc = ensure(ctx, "obs", (size_t)nout * 4 * Mp, &w.obs))) return rc;
+    if ((rc = ensure(ctx, "bvals", (size_t)n * nb * 16, &vals))) return rc;
+    if ((rc = ensure(ctx, "bfields", (size_t)3 * n, &d_fields))) return rc;
+    if ((rc = ensure(ctx, "seeds", (size_t)2 * Mp, &d_seeds))) return rc;
+    w.bval = vals;
+    hipStream_t s = ctx->stream;
+    // field vectors B n_o, values of each orientation's H on the pattern
+    std::vector<double> f(3 * n);
+    for (int o = 0; o < n; ++o) {
+        const double *d = dirs + 3 * (o0 + o);
+        const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (!(nd > 0.0)) return fail(ctx, SPC_ERR_INVALID, "orientation direction must be nonzero");
+        for (int a = 0; a < 3; ++a) f[3 * o + a] = Bmag * d[a] / nd;
+    }
+    HIP_TRY(hipMemcpyAsync(d_fields, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], d_fields, nb * 16, n, vals, s));
+    // the same trace vectors for every orientation (main.f90:50 reseeds per field)
+    std::vector<int64_t> sd((size_t)2 * Mp);
+    for (int r = 0; r < Mp; ++r) {
+        sd[2 * r] = sample_seeds[2 * (r % M)];
+        sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
+    }
+    HIP_TRY(hipMemcpyAsync(d_seeds, sd.data(), sizeof(int64_t) * sd.size(), hipMemcpyHostToDevice, s));
+    VecInit vi{};
+    vi.kind = sim->vec_kind;
+    vi.seeds = d_seeds;
+    double pop[4];
+    theta_of(sim->init_state, vi.theta, pop);
+    HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
+    HIP_TRY(init_vectors(vi, Mp, Mp, Z, w.psi, s));
+    HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, s));
+    HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
+    HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
+    HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
+    {
+        std::vector<double> obs0((size_t)4 * Mp);
+        for (int x = 0; x < 4; ++x)
+            for (int r = 0; r < Mp; ++r) obs0[(size_t)x * Mp + r] = pop[x];
+        HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    auto one_step = [&]() -> hipError_t {
+        hipError_t e;
+        if ((e = arnoldi(w, s)) != hipSuccess) return e;
+        if ((e = expm_lanes(w, sim->dt, s)) != hipSuccess) return e;
+        if ((e = step_lanes(w, sim->dt, sim->t_end, 0.0, false, s)) != hipSuccess) return e;
+        return psi_update(w, true, s);
+    };
+    HIP_TRY(hipEventRecord(ev0, s));
+    if (graph && K > 1) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = one_step();
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return hip_fail(ctx, e, "step capture");
+        if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return hip_fail(ctx, e, "hipGraphInstantiate");
+        }
+        for (int64_t k = 0; k < K && e == hipSuccess; ++k) e = hipGraphLaunch(ge, s);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    } else {
+        for (int64_t k = 0; k < K; ++k) HIP_TRY(one_step());
+    }
+    HIP_TRY(hipEventRecord(ev1, s));
+    HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, Mp, d_acc + (size_t)o0 * 4 * nout, s));
+    int32_t flags[4];
+    HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    if (out_ms) *out_ms = ms;
+    if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
+    if (flags[0])
+        return fail(ctx, SPC_ERR_BREAKDOWN,
+                    "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+    return SPC_OK;
+}
+
+// Orientations per chunk that fit `frac` of the free memory (power of two).
+int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout)
+{
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
+    for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part"}) {
+        auto it = ctx->bufs.find(nm);
+        if (it != ctx->bufs.end()) free_b += it->second.bytes;
+    }
+    const size_t per_orient = ((size_t)m + 3) * ctx->N * M * sizeof(double2) + (size_t)nout * 4 * M * sizeof(double) +
+                              (size_t)ctx->fpat.nblocks * 16 * sizeof(double2);
+    int best = 1;
+    for (int c = 1; c <= n_orient; c *= 2)
+        if ((size_t)c * per_orient <= free_b / 10 * 7) best = c;
+    return best;
+}
+
+int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
+                       const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms)
+{
+    const int64_t n_steps = spc_n_steps(sim->t_end, sim->dt);
+    const int64_t K = steps_taken(sim->t_end, sim->dt);
+    const int64_t nout = n_steps + 1;
+    if (K + 1 > nout) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
+    const int M = sim->n_samples;
+    double *d_acc = nullptr;
+    int rc;
+    if ((rc = ensure(ctx, "bacc", (size_t)n_orient * 4 * nout, &d_acc))) return rc;
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * n_orient * 4 * nout, ctx->stream));
+    double tot = 0.0;
+    for (int o0 = 0; o0 < n_orient; o0 += plan.opc) {
+        const int n = std::min(plan.opc, n_orient - o0);
+        double ms = 0.0;
+        if ((rc = run_orientation_chunk(ctx, sim, plan.m, sample_seeds, M, dirs, o0, n, plan.graph, K, nout, d_acc,
+                                        &ms)))
+            return rc;
+        tot += ms;
+    }
+    ps_out.assign((size_t)n_orient * nout, 0.0);
+    std::vector<double> acc((size_t)n_orient * 4 * nout);
+    HIP_TRY(hipMemcpy(acc.data(), d_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
+    for (int o = 0; o < n_orient; ++o)
+        for (int64_t k = 0; k < nout; ++k) {
+            const double v = acc[(size_t)o * 4 * nout + k] / (double)M;
+            if (!std::isfinite(v)) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite population");
+            ps_out[(size_t)o * nout + k] = v;
+        }
+    if (total_ms) *total_ms = tot;
+    return SPC_OK;
+}
+
+int check_batch_args(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
+                     const double *dirs)
+{
+    if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
+    int rc = check_sim(ctx, sim);
+    if (rc) return rc;
+    if (sim->tol != 0.0) return fail(ctx, SPC_ERR_UNSUPPORTED, "batched orientations need integrator_tolerance = 0");
+    if (sim->vec_kind != SPC_VEC_SUZ && sim->vec_kind != SPC_VEC_PHASE)
+        return fail(ctx, SPC_ERR_INVALID, "vec_kind must be SPC_VEC_SUZ or SPC_VEC_PHASE");
+    if (sim->n_samples < 64 || sim->n_samples % 64)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "batched orientations need N_samples a multiple of 64");
+    if (!sample_seeds || !dirs || n_orient < 1) return fail(ctx, SPC_ERR_INVALID, "NULL argument or no orientations");
+    return build_field_basis(ctx);
+}
+
+double final_yield(const double *ps, int64_t nout, double dt, double kS)
+{
+    std::vector<double> z(nout, 0.0), phis(nout / 2 + 1);
+    simpson(nout, ps, z.data(), z.data(), z.data(), dt, kS, 0.0, phis.data(), nullptr);
+    return nout / 2 >= 1 ? phis[nout / 2 - 1] : 0.0;
+}
+
+} // namespace
+
 // ====================================================================== C-ABI
 extern "C" {
 
@@ -610,6 +899,7 @@ void spc_ctx_destroy(spc_ctx *ctx)
         if (kv.second.p) hipFree(kv.second.p);
     ctx->ham.release();
     ctx->adj.release();
+    ctx->fpat.release();
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -688,6 +978,7 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     ctx->ham.release();
     ctx->adj.release();
     ctx->adj_valid = false;
+    ctx->fbasis_valid = false;
     ctx->ham = h;
     ctx->iso_applied = iso;
     ctx->nnz = h.nnz;
@@ -1396,6 +1687,102 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     for (int k = 0; k < nn && grad_hfc; ++k)
         for (int q = 0; q < 9; ++q)
             if (!std::isfinite(grad_hfc[9 * k + q])) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite gradient");
+    return SPC_OK;
+}
+
+int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                          const double *dirs, int32_t orient_per_chunk, int32_t use_graph, double *Phi_S, double *P_S,
+                          int64_t n_out)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_batch_args(ctx, sim, sample_seeds, n_orient, dirs);
+    if (rc) return rc;
+    const int64_t nout = spc_n_steps(sim->t_end, sim->dt) + 1;
+    if (P_S && n_out != nout)
+        return fail(ctx, SPC_ERR_INVALID, "output length must be ceiling((t_end+dt)/dt)+1 = " + std::to_string(nout));
+    BatchPlan plan;
+    plan.m = sim->n_krylov;
+    plan.graph = use_graph != 0;
+    plan.opc = orient_per_chunk > 0 ? std::min<int>(orient_per_chunk, n_orient)
+                                     : max_orient_per_chunk(ctx, plan.m, sim->n_samples, n_orient, nout);
+    std::vector<double> ps;
+    double ms = 0.0;
+    if ((rc = batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, plan, ps, &ms))) return rc;
+    ctx->last_ms = ms;
+    ctx->last_steps = steps_taken(sim->t_end, sim->dt);
+    ctx->last_rebuilds = ctx->last_steps * ((n_orient + plan.opc - 1) / plan.opc);
+    for (int o = 0; o < n_orient; ++o) {
+        if (Phi_S) Phi_S[o] = final_yield(ps.data() + (size_t)o * nout, nout, sim->dt, ctx->kS);
+        if (P_S) std::copy(ps.begin() + (size_t)o * nout, ps.begin() + (size_t)(o + 1) * nout, P_S + (size_t)o * nout);
+    }
+    return SPC_OK;
+}
+
+int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                   const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
+                   double *ms_per_orient_step)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_batch_args(ctx, sim, sample_seeds, n_orient, dirs);
+    if (rc) return rc;
+    const int64_t K = steps_taken(sim->t_end, sim->dt);
+    // Krylov order: the smallest (in steps of 2) whose pilot yields agree with
+    // the requested order's within yield_tol scaled to the pilot length
+    const int64_t pilot = std::min<int64_t>(K, 64);
+    spc_sim_params ps = *sim;
+    ps.t_end = sim->dt * (double)pilot - 0.5 * sim->dt;
+    const int npilot = std::min<int>(n_orient, 4);
+    const int64_t pout = spc_n_steps(ps.t_end, ps.dt) + 1;
+    BatchPlan plan;
+    plan.opc = npilot;
+    plan.m = sim->n_krylov;
+    std::vector<double> ref, cur;
+    if ((rc = batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, ref, nullptr))) return rc;
+    const double tol = yield_tol * (double)std::max<int64_t>(pilot, 1) / (double)std::max<int64_t>(K, 1);
+    int m_best = sim->n_krylov;
+    for (int m = sim->n_krylov - 2; m >= 6; m -= 2) {
+        plan.m = m;
+        ps.n_krylov = m;
+        if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, nullptr)) break;
+        double worst = 0.0;
+        for (int o = 0; o < npilot; ++o) {
+            const double a = final_yield(ref.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
+            const double b = final_yield(cur.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
+            worst = std::max(worst, std::fabs(a - b) / std::max(std::fabs(a), 1e-300));
+        }
+        if (!(worst <= tol)) break;
+        m_best = m;
+    }
+    ctx->err.clear();
+    // lane width: fastest orientation-steps per ms over powers of two that fit
+    spc_sim_params ts = *sim;
+    ts.n_krylov = m_best;
+    const int64_t tsteps = std::min<int64_t>(K, 6);
+    ts.t_end = sim->dt * (double)tsteps - 0.5 * sim->dt;
+    const int64_t tout = spc_n_steps(ts.t_end, ts.dt) + 1;
+    const int cap = max_orient_per_chunk(ctx, m_best, sim->n_samples, n_orient, tout);
+    int best_c = 1;
+    double best_t = 1e300;
+    for (int c = 1; c <= cap; c *= 2) {
+        BatchPlan tp;
+        tp.m = m_best;
+        tp.opc = c;
+        tp.graph = true;
+        std::vector<double> tmp;
+        double ms = 0.0;
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, c, dirs, tp, tmp, &ms))) return rc; // warm
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, c, dirs, tp, tmp, &ms))) return rc;
+        const double per = ms / ((double)c * (double)std::max<int64_t>(tsteps, 1));
+        if (per < best_t * 0.97) {
+            best_t = per;
+            best_c = c;
+        }
+    }
+    if (orient_per_chunk) *orient_per_chunk = best_c;
+    if (n_krylov) *n_krylov = m_best;
+    if (ms_per_orient_step) *ms_per_orient_step = best_t;
     return SPC_OK;
 }
 

@@ -17,6 +17,8 @@
 // hamiltonian.f90:62-73).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -248,6 +250,52 @@ __global__ void k_block_fill(const int32_t *__restrict__ rowptr, const int32_t *
     }
 }
 
+// Values of a CSR into an existing 4x4 block pattern (browptr, bcol) whose
+// blocks cover every CSR entry; pattern positions without an entry get 0.
+__global__ void k_block_fill_pattern(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                     const double2 *__restrict__ val, int64_t Z, const int32_t *__restrict__ browptr,
+                                     const int32_t *__restrict__ bcol, double2 *__restrict__ bval,
+                                     int32_t *__restrict__ missing)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Z) return;
+    const int b0 = browptr[n], b1 = browptr[n + 1];
+    for (int b = b0; b < b1; ++b)
+        for (int q = 0; q < 16; ++q) bval[(int64_t)b * 16 + q] = cz(0, 0);
+    for (int e = 0; e < 4; ++e)
+        for (int p = rowptr[4 * n + e]; p < rowptr[4 * n + e + 1]; ++p) {
+            const int c = col[p], cb = c >> 2;
+            int b = b0;
+            while (b < b1 && bcol[b] != cb) ++b;
+            if (b == b1) {
+                missing[0] = 1;
+                continue;
+            }
+            bval[(int64_t)b * 16 + 4 * e + (c & 3)] = val[p];
+        }
+}
+
+// out[o][i] = v0[i] + sum_a f[o][a] za[i] over the nb*16 block values.
+__global__ void k_field_values(const double2 *__restrict__ v0, const double2 *__restrict__ zx,
+                               const double2 *__restrict__ zy, const double2 *__restrict__ zz,
+                               const double *__restrict__ f, int64_t count, int n_fields, double2 *__restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < count * n_fields; idx += stride) {
+        const int64_t o = idx / count, i = idx % count;
+        const double fx = f[3 * o], fy = f[3 * o + 1], fz = f[3 * o + 2];
+        double2 v = v0[i];
+        const double2 x = zx[i], y = zy[i], z = zz[i];
+        v.x = fma(fx, x.x, v.x);
+        v.y = fma(fx, x.y, v.y);
+        v.x = fma(fy, y.x, v.x);
+        v.y = fma(fy, y.y, v.y);
+        v.x = fma(fz, z.x, v.x);
+        v.y = fma(fz, z.y, v.y);
+        out[idx] = v;
+    }
+}
+
 } // namespace
 
 hipError_t h_block_count(const int32_t *rowptr, const int32_t *col, int64_t Z, int32_t *bcounts, hipStream_t s)
@@ -282,6 +330,24 @@ hipError_t h_fill(const DevSystem *d_sys, int64_t N, const int32_t *rowptr, int3
 {
     int blocks = (int)((N + 255) / 256);
     hipLaunchKernelGGL(k_h_fill, dim3(blocks), dim3(256), 0, s, d_sys, N, rowptr, col, val);
+    return hipGetLastError();
+}
+
+hipError_t h_block_fill_pattern(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
+                                const int32_t *browptr, const int32_t *bcol, double2 *bval, int32_t *missing,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_block_fill_pattern, dim3((unsigned)((Z + 255) / 256)), dim3(256), 0, s, rowptr, col, val, Z,
+                       browptr, bcol, bval, missing);
+    return hipGetLastError();
+}
+
+hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *zy, const double2 *zz,
+                          const double *fields, int64_t count, int n_fields, double2 *out, hipStream_t s)
+{
+    const int64_t total = count * n_fields;
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_field_values, dim3(blocks), dim3(256), 0, s, v0, zx, zy, zz, fields, count, n_fields, out);
     return hipGetLastError();
 }
 

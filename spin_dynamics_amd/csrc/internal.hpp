@@ -36,6 +36,14 @@ hipError_t h_fill(const DevSystem *d_sys, int64_t N, const int32_t *rowptr, int3
 hipError_t h_block_count(const int32_t *rowptr, const int32_t *col, int64_t Z, int32_t *bcounts, hipStream_t s);
 hipError_t h_block_fill(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
                         const int32_t *browptr, int32_t *bcol, double2 *bval, hipStream_t s);
+// The values of another CSR on an existing block pattern (missing[0] set to 1
+// if an entry has no block).
+hipError_t h_block_fill_pattern(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
+                                const int32_t *browptr, const int32_t *bcol, double2 *bval, int32_t *missing,
+                                hipStream_t s);
+// out[o] = v0 + sum_a fields[o][a] z_a for n_fields field vectors (count values each).
+hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *zy, const double2 *zz,
+                          const double *fields, int64_t count, int n_fields, double2 *out, hipStream_t s);
 
 // ------------------------------------------------------------- sampling.hip
 struct VecInit {
@@ -123,6 +131,8 @@ struct KrylovWork {
     const int32_t *browptr; // the same H in 4x4 blocks over nuclear indices
     const int32_t *bcol;
     const double2 *bval;     // [nblocks][4][4]
+    int64_t bval_stride;     // values per lane group (0: one H for all lanes)
+    int chunks_per_bval;     // 64-lane chunks sharing one value set
     double2 *Q;      // [m][N][Mp]
     double2 *u;      // [N][Mp]
     double2 *v;      // [N][Mp]
@@ -158,6 +168,9 @@ hipError_t gram_projectors(KrylovWork &w, hipStream_t s);         // Q -> gram
 hipError_t step_lanes(KrylovWork &w, double dt, double t_end, double tol, bool gram, hipStream_t s);
 hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s); // psi = Q c (+ observables)
 hipError_t obs_reduce_lanes(const double *obs, int64_t n_out, int M, int Mp, double *acc, hipStream_t s);
+// acc[g][x*n_out + k] += sum over lanes g*group .. g*group+M_g-1 (M_g = M valid lanes of each group)
+hipError_t obs_reduce_groups(const double *obs, int64_t n_out, int groups, int group, int M_g, int Mp, double *acc,
+                             hipStream_t s);
 hipError_t spmm_only(KrylovWork &w, hipStream_t s);              // v = H u
 hipError_t expm_single(int m, const double2 *A, double2 *E, hipStream_t s);
 // Times (and scales) for expm_coeffs: C[o] = scale_o beta exp(-i tau_o A) e_1.

@@ -108,13 +108,15 @@ __device__ __forceinline__ void block_reduce(double2 (&acc)[NACC > 0 ? NACC : 1]
 // its nuclear index, loading each neighbouring nuclear index's 4 rows once.
 // Block values are wave-uniform (scalar loads).  Also c = u^H v, d = |u|^2.
 __global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ browptr, const int32_t *__restrict__ bcol,
-                                                  const double2 *__restrict__ bval, const double2 *__restrict__ u,
+                                                  const double2 *__restrict__ bvals, const double2 *__restrict__ u,
                                                   double2 *__restrict__ v, double2 *__restrict__ part, int64_t Z,
-                                                  int Mp, int rpw, int nrb)
+                                                  int Mp, int rpw, int nrb, int64_t bval_stride, int chunks_per_bval)
 {
     __shared__ double2 red[(kWaves - 1) * 8 * kWave];
     int rb, ch;
     block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
+    // lane chunks may carry their own H (orientations batched in lanes)
+    const double2 *__restrict__ bval = bvals + (int64_t)(ch / chunks_per_bval) * bval_stride;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
@@ -789,7 +791,7 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
     const int nrb1 = nrb_for(w.Z, w.rpw1);
     ProfScope ps(w, PF_PASS1, 2 * vec_bytes(w) + csr_bytes(w), s);
     hipLaunchKernelGGL(k_spmm, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.browptr, w.bcol, w.bval, w.u, w.v,
-                       w.part, w.Z, w.Mp, w.rpw1, nrb1);
+                       w.part, w.Z, w.Mp, w.rpw1, nrb1, w.bval_stride, std::max(w.chunks_per_bval, 1));
     return hipGetLastError();
 }
 
@@ -916,6 +918,30 @@ __global__ void k_obs_lanes(const double *__restrict__ obs, int64_t n_out, int M
     acc[idx] = s;
 }
 } // namespace
+
+namespace {
+__global__ void k_obs_groups(const double *__restrict__ obs, int64_t n_out, int groups, int group, int M_g, int Mp,
+                             double *__restrict__ acc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 4 * n_out * groups) return;
+    const int g = (int)(idx / (4 * n_out));
+    const int64_t xk = idx % (4 * n_out);
+    const int x = (int)(xk / n_out);
+    const int64_t k = xk % n_out;
+    double s = acc[idx];
+    for (int r = 0; r < M_g; ++r) s += obs[(k * 4 + x) * Mp + (int64_t)g * group + r];
+    acc[idx] = s;
+}
+} // namespace
+
+hipError_t obs_reduce_groups(const double *obs, int64_t n_out, int groups, int group, int M_g, int Mp, double *acc,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_obs_groups, dim3((unsigned)ceil_div(4 * n_out * groups, 256)), dim3(256), 0, s, obs, n_out,
+                       groups, group, M_g, Mp, acc);
+    return hipGetLastError();
+}
 
 hipError_t obs_reduce_lanes(const double *obs, int64_t n_out, int M, int Mp, double *acc, hipStream_t s)
 {

@@ -1,0 +1,105 @@
+"""GPU batched field orientations (spc_orientation_batch) and the autotuner.
+
+Pins: every orientation's P_S against the oracle run at that orientation with
+the same trace vectors (orc_trace_sampling, 1e-10 relative); against the
+plain per-orientation device runs; the captured-graph replay equal to the
+eager launches, bit for bit; results independent of how orientations are
+chunked into lanes; the autotuned Krylov order keeps the yields within the
+requested tolerance."""
+import numpy as np
+import pytest
+
+from helpers import max_rel, oracle_system
+from oracle import pyoracle
+from spin_dynamics_amd import _lib, batch, dynamics, gradients, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(M=64, T_ns=30.0, m=14):
+    w = workloads.small_anisotropic(m1=(3, 2), m2=(2, 2), M=M, m=m, T_ns=T_ns)
+    return w
+
+
+def test_each_orientation_matches_oracle(ctx):
+    w = _case()
+    dirs = batch.fibonacci_sphere(6)
+    phi, ps = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=4, with_populations=True,
+                                       ctx=ctx)
+    Z = w.sys.Z1 * w.sys.Z2
+    for o, d in enumerate(dirs):
+        s = oracle_system(w.sys, w.B)
+        s.direction = tuple(d)
+        h = pyoracle.build_general_csr(s)
+        r = pyoracle.trace_sampling(h, Z, M=w.sim.N_samples, m=w.sim.N_krylov, t_end=w.sim.t_end, dt=w.sim.dt,
+                                    vec_kind=1, seed=w.sim.seed)
+        assert max_rel(ps[o], r["P_S"]) < 1e-10
+        _, phis, _ = pyoracle.get_kinetics(r["P_S"], r["P_T0"], r["P_Tp"], r["P_Tm"], w.sim.dt, w.sys.kS, w.sys.kT)
+        assert abs(phi[o] - phis[-1]) <= 1e-10 * abs(phis[-1])
+
+
+def test_matches_plain_per_orientation_runs(ctx):
+    import copy
+    w = _case(T_ns=25.0)
+    dirs = batch.fibonacci_sphere(5)
+    phi = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, ctx=ctx)
+    for o, d in enumerate(dirs):
+        s = copy.deepcopy(w.sys)
+        s.field_dir = tuple(d)
+        res = dynamics.trace_sampling_para(s, w.sim, B=w.B, ctx=ctx)
+        res.get_kinetics(w.sim.dt, s.kS, s.kT)
+        assert abs(phi[o] - res.Phi_S[-1]) <= 1e-12 * abs(res.Phi_S[-1])
+
+
+def test_graph_replay_equals_eager_launches(ctx):
+    w = _case(T_ns=20.0)
+    dirs = batch.fibonacci_sphere(3)
+    a, pa = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, graph=True, with_populations=True, ctx=ctx)
+    b, pb = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, graph=False, with_populations=True, ctx=ctx)
+    c, pc = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, graph=True, with_populations=True, ctx=ctx)
+    assert np.array_equal(pa, pb) and np.array_equal(pa, pc)
+
+
+@pytest.mark.parametrize("opc", [1, 2, 3, 7])
+def test_chunking_does_not_change_results(ctx, opc):
+    w = _case(T_ns=15.0)
+    dirs = batch.fibonacci_sphere(7)
+    ref = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=7, ctx=ctx)
+    got = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=opc, ctx=ctx)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_more_trace_vectors_and_su_z(ctx):
+    w = _case(M=128, T_ns=12.0)
+    w.sim.vec_kind = "SUZ"
+    dirs = batch.fibonacci_sphere(3)
+    phi = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, ctx=ctx)
+    import copy
+    for o, d in enumerate(dirs):
+        s = copy.deepcopy(w.sys)
+        s.field_dir = tuple(d)
+        res = dynamics.trace_sampling_para(s, w.sim, B=w.B, ctx=ctx)
+        res.get_kinetics(w.sim.dt, s.kS, s.kT)
+        assert abs(phi[o] - res.Phi_S[-1]) <= 1e-12 * abs(res.Phi_S[-1])
+
+
+def test_autotuner_keeps_the_yield(ctx):
+    w = _case(T_ns=40.0, m=20)
+    dirs = batch.fibonacci_sphere(8)
+    opc, m, ms = batch.tune(w.sys, w.sim, dirs, B=w.B, yield_tol=1e-10, ctx=ctx)
+    assert 1 <= opc <= 8 and (opc & (opc - 1)) == 0
+    assert 6 <= m <= 20 and ms > 0
+    ref = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, ctx=ctx)
+    w.sim.N_krylov = m
+    got = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=opc, ctx=ctx)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-10
+
+
+def test_rejects_unsupported_requests(ctx):
+    w = _case(M=32)
+    with pytest.raises(_lib.SpinchemError):
+        batch.orientation_yields(w.sys, w.sim, batch.fibonacci_sphere(2), B=w.B, ctx=ctx)
+    w = _case()
+    w.sim.tol = 1e-6
+    with pytest.raises(_lib.SpinchemError):
+        batch.orientation_yields(w.sys, w.sim, batch.fibonacci_sphere(2), B=w.B, ctx=ctx)
