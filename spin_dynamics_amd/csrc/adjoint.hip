@@ -32,9 +32,10 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // waves take nuclear indices base + wave, base + wave + 4, ...  The per-lane
 // coefficients of the 16 lanes sit in LDS ([o][k][lane]), read as broadcasts.
 template <int MK>
-__global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ Q, const double2 *__restrict__ C,
-                                                     CombineOut out, const double2 *__restrict__ src, double omega,
-                                                     int64_t Z, int M, int Mp, int64_t rows_per_block)
+__global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ Q, int64_t qs,
+                                                     const double2 *__restrict__ C, CombineOut out,
+                                                     const double2 *__restrict__ src, double omega, int64_t Z, int M,
+                                                     int Mp, int64_t rows_per_block)
 {
     __shared__ double2 cs[kCombineOut][MK][kGroup];
     const int ngroups = Mp / kGroup;
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ 
         cs[o][k][l] = C[((int64_t)o * MK + k) * Mp + lg * kGroup + l];
     }
     __syncthreads();
-    const int64_t NMp = 4 * Z * (int64_t)Mp;
+    const int64_t NMp = qs; // basis vector stride
     const int64_t n0 = rb * rows_per_block, n1 = n0 + rows_per_block < Z ? n0 + rows_per_block : Z;
     const bool live = r < M;
     for (int64_t n = n0 + wave; n < n1; n += 4) {
@@ -165,8 +166,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
     }
 }
 
-using CombineFn = void (*)(const double2 *, const double2 *, CombineOut, const double2 *, double, int64_t, int, int,
-                           int64_t);
+using CombineFn = void (*)(const double2 *, int64_t, const double2 *, CombineOut, const double2 *, double, int64_t,
+                           int, int, int64_t);
 
 template <size_t... I>
 constexpr auto make_combine(std::index_sequence<I...>)
@@ -177,16 +178,16 @@ const auto kCombine = make_combine(std::make_index_sequence<kMaxKrylov>{});
 
 } // namespace
 
-hipError_t combine(const double2 *Q, int m, const double2 *C, const CombineOut &out, const double2 *src, double omega,
-                   int64_t Z, int M, int Mp, hipStream_t s)
+hipError_t combine(const double2 *Q, int64_t qs, int m, const double2 *C, const CombineOut &out, const double2 *src,
+                   double omega, int64_t Z, int M, int Mp, hipStream_t s)
 {
     if (m < 1 || m > kMaxKrylov || out.n < 1 || out.n > kCombineOut || Mp % kGroup) return hipErrorInvalidValue;
     const int ngroups = Mp / kGroup;
     // about 2048 blocks: rows per block a multiple of the 4 waves
     int64_t rpb = std::max<int64_t>(4, ceil_div(Z * ngroups, 2048) / 4 * 4);
     const int64_t nrb = ceil_div(Z, rpb);
-    hipLaunchKernelGGL(kCombine[m - 1], dim3((unsigned)(nrb * ngroups)), dim3(kBlock), 0, s, Q, C, out, src, omega,
-                       Z, M, Mp, rpb);
+    hipLaunchKernelGGL(kCombine[m - 1], dim3((unsigned)(nrb * ngroups)), dim3(kBlock), 0, s, Q, qs, C, out, src,
+                       omega, Z, M, Mp, rpb);
     return hipGetLastError();
 }
 

@@ -328,7 +328,8 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     const bool need_gram = sim->tol > 0.0;
 
-    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    w.qs = basis_stride(N, Mp);
+    if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
@@ -679,7 +680,8 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     double2 *vals = nullptr;
     double *d_fields = nullptr;
     int64_t *d_seeds = nullptr;
-    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    w.qs = basis_stride(N, Mp);
+    if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
@@ -1499,7 +1501,8 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
     w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
-    if ((rc = ensure(ctx, "Q", (size_t)m * N * Mp, &w.Q))) return rc;
+    w.qs = basis_stride(N, Mp);
+    if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
@@ -1591,7 +1594,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
             CombineOut out{};
             out.n = kGL + 1;
             for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
-            HIP_TRY(combine(w.Q, m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+            HIP_TRY(combine(w.Q, qstride(w), m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
         }
         HIP_TRY(expm_lanes(w, dt, s));
         HIP_TRY(step_lanes(w, dt, sim->t_end, 0.0, false, s));
@@ -1622,7 +1625,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         out.n = kGL + 1;
         out.y[0] = lam;
         for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
-        HIP_TRY(combine(wb.Q, m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
+        HIP_TRY(combine(wb.Q, qstride(wb), m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
         HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
         return SPC_OK;
     };
@@ -1639,7 +1642,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
                 out.n = kGL + 2;
                 for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
                 out.y[kGL + 1] = w.psi;
-                HIP_TRY(combine(w.Q, m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+                HIP_TRY(combine(w.Q, qstride(w), m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
             }
             for (int64_t n = n1 - 1; n >= n0; --n)
                 if ((rc = backward_step(n))) return rc;

@@ -51,6 +51,17 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b)
     return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
 }
 
+// Non-temporal (streaming) load / store of one complex value.
+__device__ __forceinline__ double2 ntload(const double2 *p)
+{
+    return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ void ntstore(double2 *p, double2 v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // ----------------------------------------------------------- xoroshiro128+
 // Same stream as the reference generator (m_random.f90:90-198, 280-340).
 struct Rng {

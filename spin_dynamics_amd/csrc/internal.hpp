@@ -133,7 +133,8 @@ struct KrylovWork {
     const double2 *bval;     // [nblocks][4][4]
     int64_t bval_stride;     // values per lane group (0: one H for all lanes)
     int chunks_per_bval;     // 64-lane chunks sharing one value set
-    double2 *Q;      // [m][N][Mp]
+    double2 *Q;      // [m] basis vectors [N][Mp], qs elements apart (qstride)
+    int64_t qs;      // basis vector stride in elements (0: N*Mp)
     double2 *u;      // [N][Mp]
     double2 *v;      // [N][Mp]
     double2 *psi;    // [N][Mp]
@@ -161,6 +162,11 @@ struct KrylovWork {
     Prof *prof;            // optional kernel timing
 };
 
+inline int64_t qstride(const KrylovWork &w) { return w.qs > 0 ? w.qs : w.N * (int64_t)w.Mp; }
+// Stride between the basis vectors of an N x Mp basis: N*Mp plus a pad that
+// staggers the vectors over HBM channels and banks (vectors exactly 2^k-ish
+// apart put every stream of the update pass on the same bank at once).
+int64_t basis_stride(int64_t N, int Mp);
 size_t partials_needed(const KrylovWork &w);
 hipError_t arnoldi(KrylovWork &w, hipStream_t s);               // psi -> Q, hess, beta0
 hipError_t expm_lanes(KrylovWork &w, double dt, hipStream_t s);   // hess -> E
@@ -191,8 +197,8 @@ struct CombineOut {
     double2 *y[kCombineOut];
     int n;
 };
-hipError_t combine(const double2 *Q, int m, const double2 *C, const CombineOut &out, const double2 *src, double omega,
-                   int64_t Z, int M, int Mp, hipStream_t s);
+hipError_t combine(const double2 *Q, int64_t qs, int m, const double2 *C, const CombineOut &out, const double2 *src,
+                   double omega, int64_t Z, int M, int Mp, hipStream_t s);
 // lam = omega P_S psi (P_S the singlet projector on each electron block).
 hipError_t singlet_source(const double2 *psi, double omega, double2 *lam, int64_t Z, int Mp, hipStream_t s);
 // S[n][slot][e'][e] += sum_j sum_lanes conj(b_j[4n+e']) a_j[4n_s+e] over the

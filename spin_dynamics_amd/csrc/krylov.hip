@@ -26,6 +26,7 @@
 // The last update's dots correct column m-1.  The result is the reference's
 // Hessenberg matrix A (qm_dynamics.f90:432-467) to rounding.
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <utility>
 
@@ -320,7 +321,8 @@ template <int NQ, bool LAST>
 __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q, const double2 *__restrict__ Qre,
                                                     double2 *__restrict__ qout, double2 *__restrict__ u,
                                                     const double2 *__restrict__ v, const double2 *__restrict__ coef,
-                                                    double2 *__restrict__ part, int64_t Z, int Mp, int npb, int nrb)
+                                                    double2 *__restrict__ part, int64_t Z, int Mp, int npb, int nrb,
+                                                    int64_t qs)
 {
     constexpr int NC = 2 * NQ + 3;
     constexpr int NA = NQ + 1;
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
-    const int64_t NMp = 4 * Z * (int64_t)Mp;
+    const int64_t NMp = qs; // basis vector stride
     for (int idx = threadIdx.x; idx < NC * kWave; idx += kBlock)
         sc[idx] = coef[(int64_t)(idx / kWave) * Mp + ch * kWave + idx % kWave];
     __syncthreads();
@@ -350,17 +352,25 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
         // of being held in registers across the loop
         __asm__ volatile("" ::: "memory");
         const int64_t off = (4 * n + wave) * Mp + r;
-        const double2 uu = u[off], vv = v[off];
+        // u and v are read once and the new rows written once: non-temporal,
+        // so L2 keeps the basis rows the dot phase below reads again
+        const double2 uu = ntload(u + off), vv = ntload(v + off);
+        // every load of the row is issued before the first use (the compiler
+        // otherwise may trade the loads in flight for occupancy and wait on
+        // each one in turn, which starves this bandwidth-bound pass)
+        double2 xq[NQ > 0 ? NQ : 1];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) xq[k] = Q[k * NMp + off];
+        __asm__ volatile("" ::: "memory");
         double2 qi = cmul(cqu, uu);
         double2 un = cfma(cuv, vv, cmul(cuu, uu));
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            const double2 x = Q[k * NMp + off];
-            qi = cfma(sc[k * kWave + lane], x, qi);
-            un = cfma(sc[(NQ + k) * kWave + lane], x, un);
+            qi = cfma(sc[k * kWave + lane], xq[k], qi);
+            un = cfma(sc[(NQ + k) * kWave + lane], xq[k], un);
         }
-        qout[off] = qi;
-        if constexpr (!LAST) u[off] = un;
+        ntstore(qout + off, qi);
+        if constexpr (!LAST) ntstore(u + off, un);
         sun[wave * kWave + lane] = un;
         sqi[wave * kWave + lane] = qi;
         __syncthreads();
@@ -369,18 +379,20 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
         for (int e = 0; e < 4; ++e) w4[e] = sun[e * kWave + lane];
         const double2 dun = csub(w4[1], w4[2]);
         const int64_t base = (4 * n) * Mp + r;
+        double2 xd[NK][4]; // the dot phase's rows, all loads issued first as above
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int k = wave + kWaves * j;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                xd[j][e] = k < NQ ? ntload(Qre + k * NMp + base + e * Mp) : sqi[e * kWave + lane];
+        }
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int j = 0; j < NK; ++j) {
             const int k = wave + kWaves * j;
             if (k > NQ) break;
-            double2 x[4];
-            if (k < NQ) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) x[e] = Qre[k * NMp + base + e * Mp];
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) x[e] = sqi[e * kWave + lane];
-            }
+            const double2 *x = xd[j];
             double2 t = aa[j];
 #pragma unroll
             for (int e = 0; e < 4; ++e) t = cfma_conj(x[e], w4[e], t);
@@ -406,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
 template <int MK, bool OBS>
 __global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, const double2 *__restrict__ cvec,
                                                  double2 *__restrict__ psi, double2 *__restrict__ part, int64_t Z,
-                                                 int Mp, int rpw, int nrb)
+                                                 int Mp, int rpw, int nrb, int64_t qs)
 {
     __shared__ double2 red[(kWaves - 1) * 8 * kWave];
     int rb, ch;
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, c
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_uniform(threadIdx.x / kWave);
     const int r = ch * kWave + lane;
-    const int64_t NMp = 4 * Z * (int64_t)Mp;
+    const int64_t NMp = qs;
     double2 cc[MK];
 #pragma unroll
     for (int k = 0; k < MK; ++k) cc[k] = cvec[(int64_t)k * Mp + r];
@@ -716,7 +728,7 @@ __global__ void k_step(int m, int M, int Mp, double dt, double t_end, double tol
 // (qm_dynamics.f90:469-498), whose P_S/P_T0 differ from G_X only by
 // anti-Hermitian parts.  Used when tol > 0 (several steps per rebuild).
 __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, double2 *__restrict__ part, int m,
-                                                 int64_t Z, int Mp, int64_t rows_per_block)
+                                                 int64_t Z, int Mp, int64_t rows_per_block, int64_t qs)
 {
     const int nch = Mp / kWave;
     const int ch = blockIdx.x % nch;
@@ -724,7 +736,7 @@ __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, d
     const int xi = blockIdx.y; // x*m + i
     const int x = xi / m, i = xi % m;
     const int r = ch * kWave + threadIdx.x;
-    const int64_t NMp = 4 * Z * (int64_t)Mp;
+    const int64_t NMp = qs;
     double2 acc[kMaxKrylov];
     for (int j = 0; j < m; ++j) acc[j] = cz(0, 0);
     const double h = 0.70710678118654752440;
@@ -747,8 +759,8 @@ __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, d
 
 // ---------------------------------------------------------- dispatch tables
 using UpdateFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, const double2 *, const double2 *,
-                          double2 *, int64_t, int, int, int);
-using PsiFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int, int);
+                          double2 *, int64_t, int, int, int, int64_t);
+using PsiFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int, int, int64_t);
 
 template <size_t... I>
 constexpr auto make_update(std::index_sequence<I...>, std::false_type)
@@ -797,6 +809,16 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 
 } // namespace
 
+int64_t basis_stride(int64_t N, int Mp)
+{
+    static int64_t pad = -1;
+    if (pad < 0) {
+        const char *e = std::getenv("SPC_BASIS_PAD_BYTES");
+        pad = e ? std::max<long long>(0, std::atoll(e)) : 0;
+    }
+    return N * (int64_t)Mp + (pad + 15) / 16;
+}
+
 size_t partials_needed(const KrylovWork &w)
 {
     const size_t nrb1 = nrb_for(w.Z, w.rpw1), nrb2 = ceil_div(w.Z, w.rpw2), nrbp = nrb_for(w.Z, w.rpwp);
@@ -826,7 +848,8 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
         {
             ProfScope ps(w, PF_PASS2, (i + (last ? 3 : 4)) * vec_bytes(w), s);
             hipLaunchKernelGGL(last ? kUpdateLast[i] : kUpdate[i], dim3(nrb2 * nch), dim3(kBlock), 0, s, w.Q, w.Q,
-                               w.Q + (int64_t)i * w.N * w.Mp, w.u, w.v, w.coef, w.part, w.Z, w.Mp, w.rpw2, nrb2);
+                               w.Q + (int64_t)i * qstride(w), w.u, w.v, w.coef, w.part, w.Z, w.Mp, w.rpw2, nrb2,
+                               qstride(w));
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = reduce(w, last ? i + 1 : 2 * (i + 1), nrb2, w.dots, s)) != hipSuccess) return e;
@@ -872,7 +895,8 @@ hipError_t gram_projectors(KrylovWork &w, hipStream_t s)
     nrb = (int)ceil_div(w.Z, rows);
     {
         ProfScope ps(w, PF_GRAM, 0.0, s);
-        hipLaunchKernelGGL(k_gram, dim3(nrb * nch, 4 * w.m), dim3(kWave), 0, s, w.Q, w.part, w.m, w.Z, w.Mp, rows);
+        hipLaunchKernelGGL(k_gram, dim3(nrb * nch, 4 * w.m), dim3(kWave), 0, s, w.Q, w.part, w.m, w.Z, w.Mp, rows,
+                           qstride(w));
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -895,7 +919,7 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s)
     {
         ProfScope ps(w, PF_PSI, (w.m + 1) * vec_bytes(w), s);
         hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, w.c, w.psi, w.part, w.Z, w.Mp, w.rpwp,
-                           nrbp);
+                           nrbp, qstride(w));
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !observe) return e;
