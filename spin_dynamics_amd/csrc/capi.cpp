@@ -43,8 +43,13 @@ struct HamDev {
     int32_t *browptr = nullptr, *bcol = nullptr;
     double2 *bval = nullptr;
     int64_t nnz = 0, nblocks = 0;
+    KronDev *kron = nullptr; // Kronecker form (when H has it exactly; else null)
+    double2 *kdiag = nullptr, *kgen = nullptr;
     void release()
     {
+        hipFree(kron);
+        hipFree(kdiag);
+        hipFree(kgen);
         hipFree(dsys);
         hipFree(rowptr);
         hipFree(col);
@@ -225,6 +230,12 @@ int check_sim(spc_ctx *ctx, const spc_sim_params *sim)
     return SPC_OK;
 }
 
+bool kron_disabled()
+{
+    const char *e = std::getenv("SPC_SPMM");
+    return e && std::string(e) == "blocks";
+}
+
 void set_ham(KrylovWork &w, const HamDev &h)
 {
     w.rowptr = h.rowptr;
@@ -234,6 +245,42 @@ void set_ham(KrylovWork &w, const HamDev &h)
     w.bcol = h.bcol;
     w.bval = h.bval;
     w.nnz = h.nnz;
+    const bool kr = h.kron && !kron_disabled();
+    w.kron = kr ? h.kron : nullptr;
+    w.kdiag = kr ? h.kdiag : nullptr;
+    w.kgen = kr ? h.kgen : nullptr;
+    w.kdiag_stride = w.kgen_stride = 0;
+}
+
+KronDev kron_of(const DevSystem &S)
+{
+    KronDev k{};
+    k.nn = S.nn;
+    for (int i = 0; i < S.nn; ++i) {
+        k.stride[i] = (int32_t)S.stride[i];
+        k.mult[i] = S.mult[i];
+        k.elec[i] = i < S.n1 ? 0 : 1;
+    }
+    return k;
+}
+
+// Kronecker tables of h (nsets value sets of its block pattern); on a
+// mismatch with the block form the tables are dropped (the block product is
+// used).  diag/gen: nsets * Z * 16 and nsets * nn * 24 values.
+int make_kron(spc_ctx *ctx, const KronDev &hk, KronDev *dk, const HamDev &h, const double2 *bval, int nsets,
+              double2 *diag, double2 *gen, bool &ok)
+{
+    int32_t *bad = nullptr;
+    int rc = ensure(ctx, "kron_bad", (size_t)1, &bad);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    HIP_TRY(kron_tables(dk, hk, h.browptr, h.bcol, bval, h.nblocks, nsets, ctx->Z ? ctx->Z : 1, diag, gen, bad, s));
+    int32_t b = 0;
+    HIP_TRY(hipMemcpyAsync(&b, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ok = b == 0;
+    return SPC_OK;
 }
 
 // Assembles H for the parameters S on the device: CSR by count / scan / fill
@@ -275,6 +322,26 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     HIP_TRY(hipStreamSynchronize(s));
     h.nnz = nnz;
     h.nblocks = nb;
+    // the Kronecker form used by the product
+    const KronDev hk = kron_of(S);
+    HIP_TRY(hipMalloc(&h.kron, sizeof(KronDev)));
+    HIP_TRY(hipMemcpyAsync(h.kron, &hk, sizeof(KronDev), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMalloc(&h.kdiag, sizeof(double2) * 16 * Z));
+    HIP_TRY(hipMalloc(&h.kgen, sizeof(double2) * 24 * std::max(S.nn, 1)));
+    int32_t *bad = nullptr;
+    if ((rc = ensure(ctx, "kron_bad", (size_t)1, &bad))) return rc;
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    HIP_TRY(kron_tables(h.kron, hk, h.browptr, h.bcol, h.bval, nb, 1, Z, h.kdiag, h.kgen, bad, s));
+    int32_t b = 0;
+    HIP_TRY(hipMemcpyAsync(&b, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (b) { // not of the Kronecker form: the block product is used
+        hipFree(h.kron);
+        hipFree(h.kdiag);
+        hipFree(h.kgen);
+        h.kron = nullptr;
+        h.kdiag = h.kgen = nullptr;
+    }
     return SPC_OK;
 }
 
@@ -715,6 +782,21 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     }
     HIP_TRY(hipMemcpyAsync(d_fields, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], d_fields, nb * 16, n, vals, s));
+    w.kron = nullptr;
+    if (ctx->fpat.kron && !kron_disabled()) { // per-orientation Kronecker tables
+        double2 *kd = nullptr, *kg = nullptr;
+        if ((rc = ensure(ctx, "bkdiag", (size_t)n * Z * 16, &kd))) return rc;
+        if ((rc = ensure(ctx, "bkgen", (size_t)n * std::max(hs.nn, 1) * 24, &kg))) return rc;
+        bool ok = false;
+        if ((rc = make_kron(ctx, kron_of(hs), ctx->fpat.kron, ctx->fpat, vals, n, kd, kg, ok))) return rc;
+        if (ok) {
+            w.kron = ctx->fpat.kron;
+            w.kdiag = kd;
+            w.kgen = kg;
+            w.kdiag_stride = Z * 16;
+            w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
+        }
+    }
     // the same trace vectors for every orientation (main.f90:50 reseeds per field)
     std::vector<int64_t> sd((size_t)2 * Mp);
     for (int r = 0; r < Mp; ++r) {

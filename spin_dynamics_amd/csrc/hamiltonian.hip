@@ -296,6 +296,114 @@ __global__ void k_field_values(const double2 *__restrict__ v0, const double2 *__
     }
 }
 
+// ------------------------------------------------------- Kronecker tables
+__device__ int find_block(const int32_t *__restrict__ browptr, const int32_t *__restrict__ bcol, int64_t n, int64_t c)
+{
+    for (int b = browptr[n]; b < browptr[n + 1]; ++b)
+        if (bcol[b] == c) return b;
+    return -1;
+}
+
+// the 2x2 electron matrix of a flip block B of the nucleus on electron el
+__device__ void kron_2x2(const double2 *B, int el, double2 *M)
+{
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) M[2 * i + j] = el == 0 ? B[4 * (2 * i) + 2 * j] : B[4 * i + j];
+}
+
+__global__ void k_kron_diag(const int32_t *__restrict__ browptr, const int32_t *__restrict__ bcol,
+                            const double2 *__restrict__ bval, int64_t nb, int nsets, int64_t Z,
+                            double2 *__restrict__ diag, int32_t *__restrict__ bad)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Z * nsets) return;
+    const int64_t set = idx / Z, n = idx % Z;
+    const int b = find_block(browptr, bcol, n, n);
+    double2 *D = diag + (set * Z + n) * 16;
+    for (int q = 0; q < 16; ++q) D[q] = b >= 0 ? bval[(set * nb + b) * 16 + q] : cz(0, 0);
+    if (b < 0) bad[0] = 1;
+}
+
+__global__ void k_kron_gen(const KronDev *__restrict__ kd, const int32_t *__restrict__ browptr,
+                           const int32_t *__restrict__ bcol, const double2 *__restrict__ bval, int64_t nb, int nsets,
+                           double2 *__restrict__ gen)
+{
+    const int nn = kd->nn;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x; // (set, k, dir, d)
+    if (idx >= nsets * nn * 6) return;
+    const int set = idx / (nn * 6), k = (idx / 6) % nn, dir = (idx / 3) % 2, d = idx % 3;
+    double2 M[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+    const int g = kd->mult[k];
+    const int64_t st = kd->stride[k];
+    const bool valid = dir == 0 ? d < g - 1 : (d > 0 && d < g);
+    if (valid) {
+        const int64_t n = d * st, c = dir == 0 ? n + st : n - st;
+        const int b = find_block(browptr, bcol, n, c);
+        if (b >= 0) kron_2x2(bval + (set * nb + b) * 16, kd->elec[k], M);
+    }
+    double2 *out = gen + (((int64_t)set * nn + k) * 6 + dir * 3 + d) * 4;
+    for (int q = 0; q < 4; ++q) out[q] = M[q];
+}
+
+// Every block of every row must be the diagonal block or c M (x) 1 of one
+// nucleus as tabulated, and every tabulated nonzero flip must be present.
+__global__ void k_kron_check(const KronDev *__restrict__ kd, const int32_t *__restrict__ browptr,
+                             const int32_t *__restrict__ bcol, const double2 *__restrict__ bval, int64_t nb,
+                             int nsets, int64_t Z, const double2 *__restrict__ gen, int32_t *__restrict__ bad)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Z * nsets) return;
+    const int64_t set = idx / Z, n = idx % Z;
+    const int nn = kd->nn;
+    const double2 *G = gen + (int64_t)set * nn * 24;
+    for (int b = browptr[n]; b < browptr[n + 1]; ++b) {
+        const int64_t c = bcol[b];
+        if (c == n) continue;
+        int kk = -1, dir = 0, d = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int64_t st = kd->stride[k];
+            const int dn = (int)((n / st) % kd->mult[k]);
+            if (c == n + st && dn < kd->mult[k] - 1) kk = k, dir = 0, d = dn;
+            if (c == n - st && dn > 0) kk = k, dir = 1, d = dn;
+        }
+        if (kk < 0) {
+            bad[0] = 1;
+            return;
+        }
+        const double2 *B = bval + (set * nb + b) * 16;
+        const double2 *M = G + ((int64_t)kk * 6 + dir * 3 + d) * 4;
+        const int el = kd->elec[kk];
+        for (int e = 0; e < 4; ++e)
+            for (int f = 0; f < 4; ++f) {
+                const int a = e >> 1, bb = e & 1, a2 = f >> 1, b2 = f & 1;
+                double2 want = cz(0, 0);
+                if (el == 0 && bb == b2) want = M[2 * a + a2];
+                if (el == 1 && a == a2) want = M[2 * bb + b2];
+                const double2 got = B[4 * e + f];
+                if (got.x != want.x || got.y != want.y) {
+                    bad[0] = 1;
+                    return;
+                }
+            }
+    }
+    for (int k = 0; k < nn; ++k) {
+        const int64_t st = kd->stride[k];
+        const int g = kd->mult[k];
+        const int dn = (int)((n / st) % g);
+        for (int dir = 0; dir < 2; ++dir) {
+            const bool valid = dir == 0 ? dn < g - 1 : dn > 0;
+            if (!valid) continue;
+            const double2 *M = G + ((int64_t)k * 6 + dir * 3 + dn) * 4;
+            bool nz = false;
+            for (int q = 0; q < 4; ++q) nz = nz || M[q].x != 0.0 || M[q].y != 0.0;
+            if (nz && find_block(browptr, bcol, n, dir == 0 ? n + st : n - st) < 0) {
+                bad[0] = 1;
+                return;
+            }
+        }
+    }
+}
+
 } // namespace
 
 hipError_t h_block_count(const int32_t *rowptr, const int32_t *col, int64_t Z, int32_t *bcounts, hipStream_t s)
@@ -348,6 +456,26 @@ hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *z
     const int64_t total = count * n_fields;
     const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 8192);
     hipLaunchKernelGGL(k_field_values, dim3(blocks), dim3(256), 0, s, v0, zx, zy, zz, fields, count, n_fields, out);
+    return hipGetLastError();
+}
+
+hipError_t kron_tables(const KronDev *d_kron, const KronDev &h_kron, const int32_t *browptr, const int32_t *bcol,
+                       const double2 *bval, int64_t nb, int nsets, int64_t Z, double2 *diag, double2 *gen,
+                       int32_t *bad, hipStream_t s)
+{
+    const int64_t rows = Z * nsets;
+    hipLaunchKernelGGL(k_kron_diag, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, browptr, bcol, bval, nb,
+                       nsets, Z, diag, bad);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int ng = nsets * h_kron.nn * 6;
+    if (ng > 0) {
+        hipLaunchKernelGGL(k_kron_gen, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, d_kron, browptr, bcol,
+                           bval, nb, nsets, gen);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_kron_check, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, d_kron, browptr, bcol,
+                       bval, nb, nsets, Z, gen, bad);
     return hipGetLastError();
 }
 

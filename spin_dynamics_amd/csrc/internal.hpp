@@ -24,6 +24,16 @@ struct DevSystem {
     double kbar;             // (kS + 3 kT)/4
 };
 
+// Nuclear digits of the Kronecker product form: H rows 4n + e couple n only to
+// itself (4x4 block) and to n +- stride_k, through c(m) M (x) 1 on the
+// electron of nucleus k (M a 2x2 matrix, c the ladder element).
+struct KronDev {
+    int nn;
+    int32_t stride[64];
+    int32_t mult[64];
+    int32_t elec[64]; // 0: electron 1 (spin s1 of e = 2 s1 + s2), 1: electron 2
+};
+
 // ---------------------------------------------------------- hamiltonian.hip
 // counts[N]: entries per row; then rowptr by exclusive scan; then fill.
 hipError_t h_count(const DevSystem *d_sys, int64_t N, int32_t *counts, hipStream_t s);
@@ -41,6 +51,14 @@ hipError_t h_block_fill(const int32_t *rowptr, const int32_t *col, const double2
 hipError_t h_block_fill_pattern(const int32_t *rowptr, const int32_t *col, const double2 *val, int64_t Z,
                                 const int32_t *browptr, const int32_t *bcol, double2 *bval, int32_t *missing,
                                 hipStream_t s);
+// Kronecker tables from the block form: diag[n] = block (n, n); gen[k][dir][d]
+// = the 2x2 matrix of the block from row n (digit d) to n + stride_k (dir 0)
+// or n - stride_k (dir 1), taken at the row whose other digits are 0.  bad[0]
+// is set if any block of H differs from that form (then the tables are not
+// used).  nsets value sets (stride nb*16 in bval) give nsets table sets.
+hipError_t kron_tables(const KronDev *d_kron, const KronDev &h_kron, const int32_t *browptr, const int32_t *bcol,
+                       const double2 *bval, int64_t nb, int nsets, int64_t Z, double2 *diag, double2 *gen,
+                       int32_t *bad, hipStream_t s);
 // out[o] = v0 + sum_a fields[o][a] z_a for n_fields field vectors (count values each).
 hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *zy, const double2 *zz,
                           const double *fields, int64_t count, int n_fields, double2 *out, hipStream_t s);
@@ -133,6 +151,13 @@ struct KrylovWork {
     const double2 *bval;     // [nblocks][4][4]
     int64_t bval_stride;     // values per lane group (0: one H for all lanes)
     int chunks_per_bval;     // 64-lane chunks sharing one value set
+    // Kronecker form of the same H (used by the product when set): per nuclear
+    // index the 4x4 diagonal block, per nucleus the 2x2 electron matrices of
+    // its flips (see kron_tables); strides per lane group as bval_stride
+    const struct KronDev *kron;
+    const double2 *kdiag;    // [Z][16]
+    const double2 *kgen;     // [nn][2][3][4]
+    int64_t kdiag_stride, kgen_stride;
     double2 *Q;      // [m] basis vectors [N][Mp], qs elements apart (qstride)
     int64_t qs;      // basis vector stride in elements (0: N*Mp)
     double2 *u;      // [N][Mp]

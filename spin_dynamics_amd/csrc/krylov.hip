@@ -168,6 +168,125 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ bro
     block_reduce<2>(acc, red, part, rb, r, lane, wave, Mp);
 }
 
+// The same product from the Kronecker form of H: per nuclear index n the 4x4
+// diagonal block and, for each nucleus k, the flips to n -+ stride_k, each a
+// 2x2 matrix on one electron's spin (half the FMAs of dense 4x4 blocks, and
+// the tables are a few KB: scalar cache).  Columns are taken in ascending
+// order, n - stride_0 < ... < n < ... < n + stride_0, and within a block in
+// ascending order, so the sums equal the block kernel's bit for bit.  The
+// neighbour rows of 4 nuclei are loaded before any is used.
+__device__ __forceinline__ void kron_apply(const double2 *__restrict__ M, int el, const double2 (&x)[4], double2 (&y)[4])
+{
+    if (el == 0) { // M on s1: y[2a+b] += M[a][0] x[b] + M[a][1] x[2+b]
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                y[2 * a + b] = cfma(M[2 * a], x[b], y[2 * a + b]);
+                y[2 * a + b] = cfma(M[2 * a + 1], x[2 + b], y[2 * a + b]);
+            }
+    } else { // M on s2: y[2a+b] += M[b][0] x[2a] + M[b][1] x[2a+1]
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                y[2 * a + b] = cfma(M[2 * b], x[2 * a], y[2 * a + b]);
+                y[2 * a + b] = cfma(M[2 * b + 1], x[2 * a + 1], y[2 * a + b]);
+            }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict__ kd, const double2 *__restrict__ diags,
+                                                       const double2 *__restrict__ gens, int64_t dstride,
+                                                       int64_t gstride, int chunks_per_set,
+                                                       const double2 *__restrict__ u, double2 *__restrict__ v,
+                                                       double2 *__restrict__ part, int64_t Z, int Mp, int rpw, int nrb)
+{
+    __shared__ double2 red[(kWaves - 1) * 8 * kWave];
+    int rb, ch;
+    block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int r = ch * kWave + lane;
+    const int set = ch / chunks_per_set;
+    const double2 *__restrict__ diag = diags + (int64_t)set * dstride;
+    const double2 *__restrict__ gen = gens + (int64_t)set * gstride;
+    const int nn = kd->nn;
+    double2 acc[2] = {cz(0, 0), cz(0, 0)};
+    const int64_t n0 = ((int64_t)rb * kWaves + wave) * rpw;
+    const int64_t n1 = n0 + rpw < Z ? n0 + rpw : Z;
+    for (int64_t n = n0; n < n1; ++n) {
+        const int nu = (int)n;
+        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+        // neighbours n - stride_k, k ascending
+        for (int k0 = 0; k0 < nn; k0 += 4) {
+            double2 xn[4][4];
+            int dd[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + j;
+                dd[j] = 0;
+                if (k < nn) {
+                    const int st = kd->stride[k];
+                    dd[j] = (nu / st) % kd->mult[k];
+                    if (dd[j] > 0) {
+                        const int64_t cb = (int64_t)(nu - st) * 4 * Mp + r;
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) xn[j][f] = u[cb + f * Mp];
+                    }
+                }
+            }
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (dd[j] > 0) kron_apply(gen + ((int64_t)(k0 + j) * 6 + 3 + dd[j]) * 4, kd->elec[k0 + j], xn[j], y);
+        }
+        // the diagonal block
+        double2 x0[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) x0[f] = u[(4 * n + f) * Mp + r];
+        {
+            const double2 *D = diag + n * 16;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) y[e] = cfma(D[4 * e + f], x0[f], y[e]);
+        }
+        // neighbours n + stride_k, k descending
+        for (int k0 = nn - 1; k0 >= 0; k0 -= 4) {
+            double2 xn[4][4];
+            int dd[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 - j;
+                dd[j] = -1;
+                if (k >= 0) {
+                    const int st = kd->stride[k];
+                    const int d = (nu / st) % kd->mult[k];
+                    if (d < kd->mult[k] - 1) {
+                        dd[j] = d;
+                        const int64_t cb = (int64_t)(nu + st) * 4 * Mp + r;
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) xn[j][f] = u[cb + f * Mp];
+                    }
+                }
+            }
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (dd[j] >= 0) kron_apply(gen + ((int64_t)(k0 - j) * 6 + dd[j]) * 4, kd->elec[k0 - j], xn[j], y);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t off = (4 * n + e) * Mp + r;
+            v[off] = y[e];
+            acc[0] = cfma_conj(x0[e], y[e], acc[0]);
+            acc[1].x += cabs2(x0[e]);
+        }
+    }
+    block_reduce<2>(acc, red, part, rb, r, lane, wave, Mp);
+}
+
 // Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].
 __global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ part, double2 *__restrict__ out,
                                                     int nacc, int nrb, int Mp)
@@ -802,6 +921,12 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 {
     const int nrb1 = nrb_for(w.Z, w.rpw1);
     ProfScope ps(w, PF_PASS1, 2 * vec_bytes(w) + csr_bytes(w), s);
+    if (w.kron) {
+        hipLaunchKernelGGL(k_spmm_kron, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
+                           w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
+                           w.Mp, w.rpw1, nrb1);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_spmm, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.browptr, w.bcol, w.bval, w.u, w.v,
                        w.part, w.Z, w.Mp, w.rpw1, nrb1, w.bval_stride, std::max(w.chunks_per_bval, 1));
     return hipGetLastError();

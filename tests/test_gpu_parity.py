@@ -149,3 +149,26 @@ def test_errors_are_reported(ctx):
     with pytest.raises(_lib.SpinchemError) as ei:
         dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "config2_like"])
+def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
+    # the Kronecker-form product (diagonal blocks + 2x2 flip tables) and the
+    # 4x4-block product sum the same terms in the same order: bitwise equal
+    if case == "iso":
+        w = workloads.config1(M=8, T_ns=20.0)
+    elif case == "aniso_mixed":
+        w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3), M=8, T_ns=20.0)
+    else:
+        w = workloads.small_anisotropic(m1=(3, 3, 2, 2), m2=(2, 2), M=70, m=16, T_ns=15.0)
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(5, H.N)) + 1j * rng.normal(size=(5, H.N))
+    a = H.spmm(X)
+    ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    monkeypatch.setenv("SPC_SPMM", "blocks")
+    b = H.spmm(X)
+    rb = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ra.P_S, rb.P_S) and np.array_equal(ra.P_Tp, rb.P_Tp)
+    assert max_rel(a, X @ H.scipy().toarray().T) < 1e-13
