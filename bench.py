@@ -75,6 +75,75 @@ def cpu_baseline(w, target_s: float):
                       f"m={w.sim.N_krylov}, MGS Arnoldi + Pade expm + convert_to_krylov, {t:.1f} s on {_cpu_model()}"}
 
 
+def side_workloads(ctx, a):
+    """The other measured paths, each on a bounded sample (stated in the output):
+    grad/fwd device-time ratio on the 36 864 x 64 system, batched orientations
+    against the plain per-orientation loop per dimension class, and the
+    semiclassical kernel at 2^20 samples x 48 nuclei x 4001 steps."""
+    import copy
+    from spin_dynamics_amd import batch, dynamics, gradients, sc_dynamics, workloads
+    out = {}
+    # adjoint gradient of the yield vs the forward run (step-uniform cost at tol = 0)
+    w4 = workloads.config2(M=64, m=24, T_ns=float(a.extra_steps) - 0.5)
+    dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)  # warm
+    dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)
+    fms, _, steps = dynamics.last_run_stats(ctx)
+    g = gradients.yield_gradient(w4.sys, w4.sim, B=w4.B, ctx=ctx)
+    out["grad_fwd"] = {"ratio": round(g.device_ms / fms, 3), "forward_ms": round(fms, 1),
+                       "gradient_ms": round(g.device_ms, 1), "steps": steps,
+                       "sample": f"N=36864 (2x14N+6x1H | 4x1H, anisotropic), M=64, m=24, dt=1 ns, {steps} steps, "
+                                 f"gradient w.r.t. 78 parameters; T=5 us (5001 steps) costs ~{5001 / steps:.0f}x"}
+    # batched orientations: one pair per dimension class, 96 Fibonacci orientations
+    steps_b = 8
+    dirs = batch.fibonacci_sphere(96)
+    pairs, seen = [], set()
+    for p in batch.radical_pairs():
+        if p.N not in seen:
+            seen.add(p.N)
+            pairs.append(p)
+    classes = {}
+    t_plain = t_batch = 0.0
+    counts = {}
+    for p in batch.radical_pairs():
+        counts[p.N] = counts.get(p.N, 0) + 1
+    for p in sorted(pairs, key=lambda q: q.N):
+        sim = batch.config3_sim(T_ns=float(steps_b) - 0.5)
+        plain = 0.0
+        for d in dirs[:2]:
+            s = copy.deepcopy(p.sys)
+            s.field_dir = tuple(d)
+            dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
+            ms, _, st = dynamics.last_run_stats(ctx)
+            plain += ms / (2 * st)
+        # the order is tuned for the full run length (10 us: the pilot's tolerance is
+        # scaled to it), the timing uses a few steps
+        opc, m, _ = batch.tune(p.sys, batch.config3_sim(), dirs, B=0.05, ctx=ctx)
+        s2 = copy.deepcopy(sim)
+        s2.N_krylov = m
+        batch.orientation_yields(p.sys, s2, dirs, B=0.05, orient_per_chunk=opc, ctx=ctx)
+        ms, _, st = dynamics.last_run_stats(ctx)
+        per = ms / (96 * st)
+        classes[str(p.N)] = {"plain_ms_per_orientation_step": round(plain, 4),
+                             "batched_ms_per_orientation_step": round(per, 4), "speedup": round(plain / per, 2),
+                             "orient_per_chunk": opc, "krylov": m, "pairs": counts[p.N]}
+        t_plain += counts[p.N] * plain
+        t_batch += counts[p.N] * per
+    out["batched_orientations"] = {
+        "speedup_weighted": round(t_plain / t_batch, 2), "classes": classes,
+        "sample": f"first pair of each dimension class, 96 orientations, M=64, {steps_b} steps; plain = one "
+                  f"spc_trace_sample per orientation (m=24); batched = spc_orientation_batch with the order and "
+                  f"width spc_batch_tune picks for the 10 us run (1e-10 yield tolerance); weighted by the 48 "
+                  f"pairs' classes"}
+    # semiclassical
+    w5 = workloads.config5()
+    sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **{**w5.kwargs(), "n_steps_": 10})  # warm
+    sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **w5.kwargs())
+    ms, _, st = dynamics.last_run_stats(ctx)
+    out["semiclassical"] = {"value": w5.n_samples * st / (ms * 1e-3), "unit": "sample·steps/s", "ms": round(ms, 1),
+                            "sample": f"{w5.n_samples} samples, 24 + 24 nuclei, {st} steps of 0.5 ns (T = 2 us)"}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +154,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the gradient, batched-orientation and semiclassical measurements")
+    ap.add_argument("--extra-steps", type=int, default=64, help="steps of the gradient/forward comparison")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
@@ -151,6 +223,7 @@ def main(argv=None):
                                      "ms_per_step": loop_ms / taken}}
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(w, a.cpu_seconds)
+    extras = {} if a.no_extras else side_workloads(ctx, a)
     line = {"metric": METRIC, "value": value, "unit": "state·steps/s", "n_gpus": 1, "steps": int(taken),
             "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
@@ -161,6 +234,7 @@ def main(argv=None):
             "roofline": roofline, "cpu_baseline": cpu}
     if families:
         line["kernel_families"] = families
+    line.update(extras)
     print(json.dumps(line))
     return 0
 

@@ -196,6 +196,7 @@ __device__ __forceinline__ void kron_apply(const double2 *__restrict__ M, int el
     }
 }
 
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict__ kd, const double2 *__restrict__ diags,
                                                        const double2 *__restrict__ gens, int64_t dstride,
                                                        int64_t gstride, int chunks_per_set,
@@ -219,11 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
         const int nu = (int)n;
         double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
         // neighbours n - stride_k, k ascending
-        for (int k0 = 0; k0 < nn; k0 += 4) {
-            double2 xn[4][4];
-            int dd[4];
+        for (int k0 = 0; k0 < nn; k0 += G) {
+            double2 xn[G][4];
+            int dd[G];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < G; ++j) {
                 const int k = k0 + j;
                 dd[j] = 0;
                 if (k < nn) {
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
             }
             __asm__ volatile("" ::: "memory");
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < G; ++j)
                 if (dd[j] > 0) kron_apply(gen + ((int64_t)(k0 + j) * 6 + 3 + dd[j]) * 4, kd->elec[k0 + j], xn[j], y);
         }
         // the diagonal block
@@ -253,11 +254,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
                 for (int f = 0; f < 4; ++f) y[e] = cfma(D[4 * e + f], x0[f], y[e]);
         }
         // neighbours n + stride_k, k descending
-        for (int k0 = nn - 1; k0 >= 0; k0 -= 4) {
-            double2 xn[4][4];
-            int dd[4];
+        for (int k0 = nn - 1; k0 >= 0; k0 -= G) {
+            double2 xn[G][4];
+            int dd[G];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < G; ++j) {
                 const int k = k0 - j;
                 dd[j] = -1;
                 if (k >= 0) {
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
             }
             __asm__ volatile("" ::: "memory");
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < G; ++j)
                 if (dd[j] >= 0) kron_apply(gen + ((int64_t)(k0 - j) * 6 + dd[j]) * 4, kd->elec[k0 - j], xn[j], y);
         }
 #pragma unroll
@@ -922,7 +923,13 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
     const int nrb1 = nrb_for(w.Z, w.rpw1);
     ProfScope ps(w, PF_PASS1, 2 * vec_bytes(w) + csr_bytes(w), s);
     if (w.kron) {
-        hipLaunchKernelGGL(k_spmm_kron, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
+        static int group = -1;
+        if (group < 0) {
+            const char *e = std::getenv("SPC_KRON_GROUP");
+            group = e ? std::min(4, std::max(2, std::atoi(e))) : 3; // 3: fastest on MI355X (187 us vs 218 for 4)
+        }
+        auto fn = group == 2 ? k_spmm_kron<2> : group == 3 ? k_spmm_kron<3> : k_spmm_kron<4>;
+        hipLaunchKernelGGL(fn, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
                            w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
                            w.Mp, w.rpw1, nrb1);
         return hipGetLastError();
