@@ -4,7 +4,7 @@ module class_observables
     use, intrinsic :: iso_fortran_env, only: dp => real64
     implicit none
     private
-    public :: observables
+    public :: observables, write_data
 
     type observables
         real(dp), allocatable :: P_S(:), P_T0(:), P_Tp(:), P_Tm(:)

@@ -7,6 +7,7 @@ program main
     use spc_random
     use class_observables
     use dynamics
+    use sc_dynamics
     implicit none
 
     character(len=600)        :: arg
@@ -45,7 +46,9 @@ program main
             call trace_sampling(sys, sim, rng, res, folder)
         case ('exact_dynamics')
             call exact_dynamics(sys, sim, res, folder)
-        case ('symmetry_dynamics', 'semi_classical')
+        case ('semi_classical')
+            call semi_classical(sim, sys, res, folder)
+        case ('symmetry_dynamics')
             print *, 'method not yet implemented'
         case default
             stop 'Error: Invalid simulation type entered.'

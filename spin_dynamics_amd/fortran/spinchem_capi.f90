@@ -16,6 +16,13 @@ module spinchem_capi
         integer(c_int32_t) :: iso_rule = 1
     end type
 
+    type, bind(C), public :: spc_sc_system
+        integer(c_int32_t) :: n_nuc1 = 0, n_nuc2 = 0
+        type(c_ptr)        :: mult = c_null_ptr
+        type(c_ptr)        :: a_iso = c_null_ptr
+        real(c_double)     :: B = 0, field_dir(3) = [0d0, 0d0, 1d0], J = 0, kS = 0, kT = 0
+    end type
+
     type, bind(C), public :: spc_sim_params
         integer(c_int32_t) :: n_samples = 0, n_krylov = 0
         real(c_double)     :: t_end = 0, dt = 0, tol = 0
@@ -23,7 +30,7 @@ module spinchem_capi
     end type
 
     public :: spc_ctx_create, spc_ctx_destroy, spc_last_error, spc_system_set, spc_n_steps, &
-              spc_trace_sample, spc_trace_sample_seeds, spc_exact_trace, spc_expm_small, &
+              spc_trace_sample, spc_trace_sample_seeds, spc_exact_trace, spc_expm_small, spc_semiclassical_run, &
               spc_ini_open, spc_ini_close, spc_ini_num_sections, spc_ini_has_section, &
               spc_ini_count_values, spc_ini_get_string, spc_ini_get_doubles, spc_ini_get_int64s, &
               c_str, f_str, device_context, check
@@ -135,6 +142,17 @@ module spinchem_capi
         integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
+        end function
+        integer(c_int) function spc_semiclassical_run(ctx, sys, n_samples, seed, dt, n_steps, ps, pt, phis, phit) &
+                bind(C, name='spc_semiclassical_run')
+            import :: c_int, c_ptr, c_int64_t, c_double, spc_sc_system
+            type(c_ptr), value :: ctx
+            type(spc_sc_system), intent(in) :: sys
+            integer(c_int64_t), value :: n_samples
+            integer(c_int64_t), intent(in) :: seed(2)
+            real(c_double), value :: dt
+            integer(c_int64_t), value :: n_steps
+            real(c_double), intent(out) :: ps(*), pt(*), phis(*), phit(*)
         end function
     end interface
 

@@ -71,3 +71,22 @@ def test_driver_exact_dynamics(tmp_path):
     got = load(out / ".050")
     assert np.max(np.abs(got["S_prob"] - ref["P_S"])) < 1e-10
     assert np.max(np.abs(got["total_prob"] - sum(ref[k] for k in ("P_S", "P_T0", "P_Tp", "P_Tm")))) < 1e-10
+
+
+def test_driver_semi_classical_matches_oracle(tmp_path):
+    # simulation_type = semi_classical (unreachable in the reference driver):
+    # isotropic parts of the tensors, N_samples classical samples from the seed
+    w = workloads.config1(M=3000, T_ns=20.0)
+    out = run_driver(w, tmp_path, sim_type="semi_classical", fields=[0.05])
+    folder = out / ".050"
+    got = {k: np.loadtxt(os.path.join(folder, f"{k}.data")) for k in ("S_prob", "T_prob", "S_yield", "T_yield")}
+    a1 = [np.trace(a) / 3 for a in w.sys.e1.A]
+    a2 = [np.trace(a) / 3 for a in w.sys.e2.A]
+    n_steps = pyoracle.n_steps(w.sim.t_end, w.sim.dt)
+    ps, pt = pyoracle.semiclassical(a1, a2, w.sys.e1.g_I, w.sys.e2.g_I, B=0.05, J=w.sys.J, kS=w.sys.kS, kT=w.sys.kT,
+                                    n_samples=3000, dt=w.sim.dt, n_steps_=n_steps, seed=w.sim.seed, with_pt=True)
+    assert len(got["S_prob"]) == n_steps + 1
+    assert np.max(np.abs(got["S_prob"] - ps)) < 1e-10 * np.max(np.abs(ps))
+    assert np.max(np.abs(got["T_prob"] - pt)) < 1e-10 * max(np.max(np.abs(pt)), 1e-3)
+    _, phis, phit = pyoracle.get_kinetics(ps, pt, np.zeros_like(ps), np.zeros_like(ps), w.sim.dt, w.sys.kS, w.sys.kT)
+    assert abs(got["S_yield"][-1] - phis[-1]) <= 1e-10 * abs(phis[-1])
