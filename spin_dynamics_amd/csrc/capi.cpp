@@ -45,6 +45,7 @@ struct HamDev {
     int64_t nnz = 0, nblocks = 0;
     KronDev *kron = nullptr; // Kronecker form (when H has it exactly; else null)
     double2 *kdiag = nullptr, *kgen = nullptr;
+    int kq = 0; // in-wave nuclei of the Kronecker product (kron_inwave)
     void release()
     {
         hipFree(kron);
@@ -250,6 +251,7 @@ void set_ham(KrylovWork &w, const HamDev &h)
     w.kdiag = kr ? h.kdiag : nullptr;
     w.kgen = kr ? h.kgen : nullptr;
     w.kdiag_stride = w.kgen_stride = 0;
+    w.kq = kr ? h.kq : 0;
 }
 
 KronDev kron_of(const DevSystem &S)
@@ -324,6 +326,7 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     h.nblocks = nb;
     // the Kronecker form used by the product
     const KronDev hk = kron_of(S);
+    h.kq = kron_inwave(hk);
     HIP_TRY(hipMalloc(&h.kron, sizeof(KronDev)));
     HIP_TRY(hipMemcpyAsync(h.kron, &hk, sizeof(KronDev), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMalloc(&h.kdiag, sizeof(double2) * 16 * Z));
@@ -793,6 +796,7 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
             w.kron = ctx->fpat.kron;
             w.kdiag = kd;
             w.kgen = kg;
+            w.kq = kron_inwave(kron_of(hs));
             w.kdiag_stride = Z * 16;
             w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
         }

@@ -34,6 +34,15 @@ struct KronDev {
     int32_t elec[64]; // 0: electron 1 (spin s1 of e = 2 s1 + s2), 1: electron 2
 };
 
+// How many of the last nuclei (at most 3) are spin-1/2 with strides 1, 2, 4:
+// the product can then give a wave 2^q consecutive nuclear indices.
+inline int kron_inwave(const KronDev &k)
+{
+    int q = 0;
+    while (q < 3 && q < k.nn && k.mult[k.nn - 1 - q] == 2 && k.stride[k.nn - 1 - q] == (1 << q)) ++q;
+    return q;
+}
+
 // ---------------------------------------------------------- hamiltonian.hip
 // counts[N]: entries per row; then rowptr by exclusive scan; then fill.
 hipError_t h_count(const DevSystem *d_sys, int64_t N, int32_t *counts, hipStream_t s);
@@ -158,6 +167,7 @@ struct KrylovWork {
     const double2 *kdiag;    // [Z][16]
     const double2 *kgen;     // [nn][2][3][4]
     int64_t kdiag_stride, kgen_stride;
+    int kq;                  // trailing spin-1/2 nuclei of strides 1, 2, 4 usable in-wave (0..3)
     double2 *Q;      // [m] basis vectors [N][Mp], qs elements apart (qstride)
     int64_t qs;      // basis vector stride in elements (0: N*Mp)
     double2 *u;      // [N][Mp]

@@ -173,8 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ bro
 // 2x2 matrix on one electron's spin (half the FMAs of dense 4x4 blocks, and
 // the tables are a few KB: scalar cache).  Columns are taken in ascending
 // order, n - stride_0 < ... < n < ... < n + stride_0, and within a block in
-// ascending order, so the sums equal the block kernel's bit for bit.  The
-// neighbour rows of 4 nuclei are loaded before any is used.
+// ascending order, so the sums equal the block kernel's bit for bit.
 __device__ __forceinline__ void kron_apply(const double2 *__restrict__ M, int el, const double2 (&x)[4], double2 (&y)[4])
 {
     if (el == 0) { // M on s1: y[2a+b] += M[a][0] x[b] + M[a][1] x[2+b]
@@ -192,6 +191,69 @@ __device__ __forceinline__ void kron_apply(const double2 *__restrict__ M, int el
             for (int b = 0; b < 2; ++b) {
                 y[2 * a + b] = cfma(M[2 * b], x[2 * a], y[2 * a + b]);
                 y[2 * a + b] = cfma(M[2 * b + 1], x[2 * a + 1], y[2 * a + b]);
+            }
+    }
+}
+
+// Existing neighbours of nuclear index n as bit masks over the nuclei k <
+// kmax: minus (digit d_k > 0), plus (d_k < mult_k - 1), one (d_k = 1; with
+// the direction it fixes d_k for spin-1/2 and spin-1 nuclei).  n is uniform.
+struct NbrMasks {
+    uint64_t minus, plus, one;
+};
+__device__ __forceinline__ NbrMasks nbr_masks(const KronDev *__restrict__ kd, int n, int kmax)
+{
+    NbrMasks m{0, 0, 0};
+    for (int k = 0; k < kmax; ++k) {
+        const int mu = kd->mult[k];
+        const int d = (n / kd->stride[k]) % mu;
+        if (d > 0) m.minus |= 1ull << k;
+        if (d < mu - 1) m.plus |= 1ull << k;
+        if (d == 1) m.one |= 1ull << k;
+    }
+    return m;
+}
+
+// y += the flips of the existing neighbours in `mask`, G at a time: minus
+// neighbours n - stride_k in ascending k, plus neighbours n + stride_k in
+// descending k (ascending columns either way).  The G rows of a batch are
+// loaded before any is used and with no branch around the loads (a slot past
+// the last neighbour reads row n itself and is not applied): loads under
+// branches made the compiler wait for each before issuing the next.  base is
+// the offset of this thread's element of row 4n; rows of index n + s are at
+// base + s * M4.
+template <int G, bool MINUS>
+__device__ __forceinline__ void kron_neighbours(const KronDev *__restrict__ kd, const double2 *__restrict__ gen,
+                                                const double2 *__restrict__ u, int64_t base, int64_t M4, int Mp,
+                                                uint64_t mask, uint64_t one, double2 (&y)[4])
+{
+    while (mask) {
+        double2 xn[G][4];
+        int ks[G], sh[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            ks[j] = -1;
+            sh[j] = 0;
+            if (mask) {
+                const int k = MINUS ? __builtin_ctzll(mask) : 63 - __builtin_clzll(mask);
+                mask &= ~(1ull << k);
+                ks[j] = k;
+                sh[j] = MINUS ? -kd->stride[k] : kd->stride[k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int64_t cb = base + (int64_t)sh[j] * M4;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) xn[j][f] = u[cb + f * Mp];
+        }
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (ks[j] >= 0) {
+                const int k = ks[j];
+                const int d = ((one >> k) & 1) ? 1 : (MINUS ? 2 : 0);
+                kron_apply(gen + ((int64_t)k * 6 + (MINUS ? 3 : 0) + d) * 4, kd->elec[k], xn[j], y);
             }
     }
 }
@@ -218,34 +280,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
     const int64_t n1 = n0 + rpw < Z ? n0 + rpw : Z;
     for (int64_t n = n0; n < n1; ++n) {
         const int nu = (int)n;
-        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
-        // neighbours n - stride_k, k ascending
-        for (int k0 = 0; k0 < nn; k0 += G) {
-            double2 xn[G][4];
-            int dd[G];
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const int k = k0 + j;
-                dd[j] = 0;
-                if (k < nn) {
-                    const int st = kd->stride[k];
-                    dd[j] = (nu / st) % kd->mult[k];
-                    if (dd[j] > 0) {
-                        const int64_t cb = (int64_t)(nu - st) * 4 * Mp + r;
-#pragma unroll
-                        for (int f = 0; f < 4; ++f) xn[j][f] = u[cb + f * Mp];
-                    }
-                }
-            }
-            __asm__ volatile("" ::: "memory");
-#pragma unroll
-            for (int j = 0; j < G; ++j)
-                if (dd[j] > 0) kron_apply(gen + ((int64_t)(k0 + j) * 6 + 3 + dd[j]) * 4, kd->elec[k0 + j], xn[j], y);
-        }
-        // the diagonal block
         double2 x0[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) x0[f] = u[(4 * n + f) * Mp + r];
+        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+        const NbrMasks nm = nbr_masks(kd, nu, nn);
+        const int64_t base = 4 * n * Mp + r, M4 = 4 * (int64_t)Mp;
+        kron_neighbours<G, true>(kd, gen, u, base, M4, Mp, nm.minus, nm.one, y);
+        // the diagonal block
         {
             const double2 *D = diag + n * 16;
 #pragma unroll
@@ -253,30 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
 #pragma unroll
                 for (int f = 0; f < 4; ++f) y[e] = cfma(D[4 * e + f], x0[f], y[e]);
         }
-        // neighbours n + stride_k, k descending
-        for (int k0 = nn - 1; k0 >= 0; k0 -= G) {
-            double2 xn[G][4];
-            int dd[G];
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const int k = k0 - j;
-                dd[j] = -1;
-                if (k >= 0) {
-                    const int st = kd->stride[k];
-                    const int d = (nu / st) % kd->mult[k];
-                    if (d < kd->mult[k] - 1) {
-                        dd[j] = d;
-                        const int64_t cb = (int64_t)(nu + st) * 4 * Mp + r;
-#pragma unroll
-                        for (int f = 0; f < 4; ++f) xn[j][f] = u[cb + f * Mp];
-                    }
-                }
-            }
-            __asm__ volatile("" ::: "memory");
-#pragma unroll
-            for (int j = 0; j < G; ++j)
-                if (dd[j] >= 0) kron_apply(gen + ((int64_t)(k0 - j) * 6 + dd[j]) * 4, kd->elec[k0 - j], xn[j], y);
-        }
+        kron_neighbours<G, false>(kd, gen, u, base, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t off = (4 * n + e) * Mp + r;
@@ -286,6 +305,145 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
         }
     }
     block_reduce<2>(acc, red, part, rb, r, lane, wave, Mp);
+}
+
+// The same Kronecker product on lane slices of W = 64 >> Q trace vectors, for
+// an H whose last Q nuclei are spin-1/2 (strides 1, 2, .., 2^(Q-1)).  A wave
+// takes the 2^Q consecutive nuclear indices n = (qi << Q) + g (g = lane group
+// of W threads) for its W lanes: the digits of the other nuclei are the same
+// for the whole wave (scalar), the last Q nuclei flip between the wave's own
+// indices, and the wave's diagonal blocks (contiguous) are staged in LDS.
+// When Mp / W is a multiple of 8, XCD x takes the slices 8s + x, s = 0, 1, ..,
+// each over all rows: its L2 then holds the neighbours of 4 MiB / (64 W)
+// nuclear indices instead of a quarter of that at W = 64.  Sums are taken in
+// the order of k_spmm_kron (bit-identical products).
+template <int G, int Q>
+__global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__restrict__ kd,
+                                                              const double2 *__restrict__ diags,
+                                                              const double2 *__restrict__ gens, int64_t dstride,
+                                                              int64_t gstride, int chunks_per_set,
+                                                              const double2 *__restrict__ u, double2 *__restrict__ v,
+                                                              double2 *__restrict__ part, int64_t Z, int Mp, int rpw,
+                                                              int nrb)
+{
+    constexpr int W = kWave >> Q;
+    constexpr int NG = 1 << Q;
+    __shared__ double2 red[kWaves - 1][2][W];
+    __shared__ double2 dl[kWaves][NG * 16];
+    __shared__ double2 xl[kWaves][NG][4][W]; // the wave's own rows, for the in-wave flips
+    const int nsl = Mp / W;
+    int rb, sl;
+    if ((nsl & 7) == 0) {
+        const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+        sl = (j / nrb) * 8 + x;
+        rb = j % nrb;
+    } else {
+        sl = blockIdx.x / nrb;
+        rb = blockIdx.x % nrb;
+    }
+    const int t = threadIdx.x & (kWave - 1);
+    const int lane = t & (W - 1), g = t >> (6 - Q);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int r = sl * W + lane;
+    const int set = (sl * W / kWave) / chunks_per_set;
+    const double2 *__restrict__ diag = diags + (int64_t)set * dstride;
+    const double2 *__restrict__ gen = gens + (int64_t)set * gstride;
+    const int nn = kd->nn, nuni = nn - Q; // nuclei 0 .. nuni-1 have wave-uniform digits
+    const int64_t M4 = 4 * (int64_t)Mp;
+    const int64_t toff = (int64_t)g * M4 + r; // this thread's offset from row 4 (qi << Q)
+    double2 acc[2] = {cz(0, 0), cz(0, 0)};
+    const int64_t Zq = Z >> Q;
+    const int64_t q0 = ((int64_t)rb * kWaves + wave) * rpw;
+    const int64_t q1 = q0 + rpw < Zq ? q0 + rpw : Zq;
+    for (int64_t qi = q0; qi < q1; ++qi) {
+        const int nb = (int)(qi << Q);
+        const int64_t rowb = (int64_t)nb * M4;
+#pragma unroll
+        for (int i = 0; i < (NG * 16 + kWave - 1) / kWave; ++i)
+            if (i * kWave + t < NG * 16) dl[wave][i * kWave + t] = diag[(int64_t)nb * 16 + i * kWave + t];
+        double2 x0[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) x0[f] = u[rowb + toff + f * Mp];
+        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+        // neighbours n - stride_k of the uniform nuclei, k ascending
+        const NbrMasks nm = nbr_masks(kd, nb, nuni);
+        kron_neighbours<G, true>(kd, gen, u, rowb + toff, M4, Mp, nm.minus, nm.one, y);
+        // the in-wave nuclei: every index has exactly one neighbour per nucleus,
+        // g ^ 2^j (below it when bit j of g is set, above it otherwise), whose
+        // rows the partner lane group holds: exchanged through LDS
+#pragma unroll
+        for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = Q - 1; j >= 0; --j) { // k = nn-1-j ascending
+            const int k = nn - 1 - j;
+            if ((g >> j) & 1) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
+                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, kd->elec[k], xp, y);
+            }
+        }
+        {
+            // one block row at a time (4 LDS reads in registers, not 16)
+            const double2 *D = &dl[wave][g * 16];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                double2 de[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) de[f] = D[4 * e + f];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) y[e] = cfma(de[f], x0[f], y[e]);
+                __asm__ volatile("" ::: "memory");
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < Q; ++j) { // k = nn-1-j descending
+            const int k = nn - 1 - j;
+            if (!((g >> j) & 1)) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
+                kron_apply(gen + ((int64_t)k * 6) * 4, kd->elec[k], xp, y);
+            }
+        }
+        // neighbours n + stride_k of the uniform nuclei, k descending
+        kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[rowb + toff + e * Mp] = y[e];
+            acc[0] = cfma_conj(x0[e], y[e], acc[0]);
+            acc[1].x += cabs2(x0[e]);
+        }
+        // the next index's diagonal blocks overwrite dl only after every lane read it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // sum over the wave's lane groups (a + b = b + a: every group holds the same sum), then the waves
+#pragma unroll
+    for (int s = W; s < kWave; s <<= 1)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double2 o = cz(__shfl_xor(acc[j].x, s), __shfl_xor(acc[j].y, s));
+            acc[j] = cadd(acc[j], o);
+        }
+    if (wave > 0 && g == 0) {
+        red[wave - 1][0][lane] = acc[0];
+        red[wave - 1][1][lane] = acc[1];
+    }
+    __syncthreads();
+    if (wave == 0 && g == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double2 s = acc[j];
+#pragma unroll
+            for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][j][lane]);
+            part[((int64_t)rb * 2 + j) * Mp + r] = s;
+        }
+    }
 }
 
 // Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].
@@ -918,6 +1076,28 @@ hipError_t reduce(const KrylovWork &w, int nacc, int nrb, double2 *out, hipStrea
     return hipGetLastError();
 }
 
+using KronFn = void (*)(const KronDev *, const double2 *, const double2 *, int64_t, int64_t, int, const double2 *,
+                        double2 *, double2 *, int64_t, int, int, int);
+
+// Lane-slice depth of the Kronecker product: 2^Q nuclear indices per wave on
+// 64 >> Q lanes.  Default 2 where the system allows it (N = 36864, M = 256 on
+// MI355X: 160 us per product at Q = 2, 167 at 1, 202 at 3, 212 at 0);
+// SPC_KRON_Q overrides it (read at every launch), capped by the system.
+int kron_slices(const KrylovWork &w)
+{
+    if (!w.kron) return 0;
+    int q = 2;
+    if (const char *e = std::getenv("SPC_KRON_Q")) q = std::max(0, std::atoi(e));
+    return std::min(q, w.kq);
+}
+
+// Row blocks of the product's partial sums (the reduce after it needs them).
+int spmm_nrb(const KrylovWork &w)
+{
+    const int q = kron_slices(w);
+    return nrb_for(w.Z >> q, w.rpw1);
+}
+
 hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 {
     const int nrb1 = nrb_for(w.Z, w.rpw1);
@@ -926,7 +1106,19 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
         static int group = -1;
         if (group < 0) {
             const char *e = std::getenv("SPC_KRON_GROUP");
-            group = e ? std::min(4, std::max(2, std::atoi(e))) : 3; // 3: fastest on MI355X (187 us vs 218 for 4)
+            group = e ? std::min(4, std::max(2, std::atoi(e))) : 2; // 2: fastest at Q = 2 (160 us vs 165 for 3)
+        }
+        const int q = kron_slices(w);
+        if (q > 0) {
+            static const KronFn fns[3][3] = {
+                {k_spmm_kron_slices<2, 1>, k_spmm_kron_slices<3, 1>, k_spmm_kron_slices<4, 1>},
+                {k_spmm_kron_slices<2, 2>, k_spmm_kron_slices<3, 2>, k_spmm_kron_slices<4, 2>},
+                {k_spmm_kron_slices<2, 3>, k_spmm_kron_slices<3, 3>, k_spmm_kron_slices<4, 3>}};
+            const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave;
+            hipLaunchKernelGGL(fns[q - 1][group - 2], dim3(nrb * nsl), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
+                               w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
+                               w.Mp, w.rpw1, nrb);
+            return hipGetLastError();
         }
         auto fn = group == 2 ? k_spmm_kron<2> : group == 3 ? k_spmm_kron<3> : k_spmm_kron<4>;
         hipLaunchKernelGGL(fn, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
@@ -965,7 +1157,7 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
     // u_0 = psi: w.u aliases w.psi (the update overwrites it in place; psi is
     // rebuilt from Q c after the step)
     const int nch = w.Mp / kWave;
-    const int nrb1 = nrb_for(w.Z, w.rpw1), nrb2 = (int)ceil_div(w.Z, w.rpw2);
+    const int nrb1 = spmm_nrb(w), nrb2 = (int)ceil_div(w.Z, w.rpw2);
     hipError_t e = hipSuccess;
     for (int i = 0; i < w.m; ++i) {
         if ((e = spmm_launch(w, s)) != hipSuccess) return e;

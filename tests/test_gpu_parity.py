@@ -151,24 +151,37 @@ def test_errors_are_reported(ctx):
     assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "config2_like"])
+@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "trailing1", "config2_like"])
 def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
     # the Kronecker-form product (diagonal blocks + 2x2 flip tables) and the
-    # 4x4-block product sum the same terms in the same order: bitwise equal
+    # 4x4-block product sum the same terms in the same order: bitwise equal,
+    # also when a wave takes 2^q consecutive nuclear indices on 64 / 2^q lanes
+    # (q is capped by the trailing spin-1/2 nuclei: 3 for iso and config2_like,
+    # 0 for aniso_mixed, 1 for trailing1)
     if case == "iso":
         w = workloads.config1(M=8, T_ns=20.0)
     elif case == "aniso_mixed":
         w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3), M=8, T_ns=20.0)
+    elif case == "trailing1":
+        w = workloads.small_anisotropic(m1=(2, 3), m2=(3, 2), M=8, T_ns=20.0)
     else:
         w = workloads.small_anisotropic(m1=(3, 3, 2, 2), m2=(2, 2), M=70, m=16, T_ns=15.0)
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
     rng = np.random.default_rng(1)
     X = rng.normal(size=(5, H.N)) + 1j * rng.normal(size=(5, H.N))
-    a = H.spmm(X)
-    ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    Xw = rng.normal(size=(130, H.N)) + 1j * rng.normal(size=(130, H.N))  # 192 lanes: 3 chunks
     monkeypatch.setenv("SPC_SPMM", "blocks")
-    b = H.spmm(X)
+    b, bw = H.spmm(X), H.spmm(Xw)
     rb = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
-    assert np.array_equal(a, b)
-    assert np.array_equal(ra.P_S, rb.P_S) and np.array_equal(ra.P_Tp, rb.P_Tp)
-    assert max_rel(a, X @ H.scipy().toarray().T) < 1e-13
+    monkeypatch.delenv("SPC_SPMM")
+    for q in ("0", "1", "2", "3"):
+        monkeypatch.setenv("SPC_KRON_Q", q)
+        a = H.spmm(X)
+        assert np.array_equal(a, b), q
+        assert np.array_equal(H.spmm(Xw), bw), q
+        ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+        # the product's dot products are summed in another order for q > 0
+        assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, q
+        if q == "0":
+            assert np.array_equal(ra.P_S, rb.P_S) and np.array_equal(ra.P_Tp, rb.P_Tp)
+    assert max_rel(b, X @ H.scipy().toarray().T) < 1e-13
