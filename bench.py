@@ -79,7 +79,8 @@ def side_workloads(ctx, a):
     """The other measured paths, each on a bounded sample (stated in the output):
     grad/fwd device-time ratio on the 36 864 x 64 system, batched orientations
     against the plain per-orientation loop per dimension class, and the
-    semiclassical kernel at 2^20 samples x 48 nuclei x 4001 steps."""
+    semiclassical kernel at 2^20 samples x 48 nuclei x 4001 steps, with and
+    without its field derivative."""
     import copy
     from spin_dynamics_amd import batch, dynamics, gradients, sc_dynamics, workloads
     out = {}
@@ -139,8 +140,15 @@ def side_workloads(ctx, a):
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **{**w5.kwargs(), "n_steps_": 10})  # warm
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **w5.kwargs())
     ms, _, st = dynamics.last_run_stats(ctx)
+    # d/dB at fixed direction: the same samples with their forward-mode tangents
+    kw5 = w5.kwargs()
+    sc_dynamics.run_tangent(w5.a1, w5.a2, w5.m1, w5.m2, dfield=w5.direction, ctx=ctx, **{**kw5, "n_steps_": 10})
+    sc_dynamics.run_tangent(w5.a1, w5.a2, w5.m1, w5.m2, dfield=w5.direction, ctx=ctx, **kw5)
+    ms_t, _, _ = dynamics.last_run_stats(ctx)
     out["semiclassical"] = {"value": w5.n_samples * st / (ms * 1e-3), "unit": "sample·steps/s", "ms": round(ms, 1),
-                            "sample": f"{w5.n_samples} samples, 24 + 24 nuclei, {st} steps of 0.5 ns (T = 2 us)"}
+                            "grad_B_ms": round(ms_t, 1), "grad_B_over_forward": round(ms_t / ms, 2),
+                            "sample": f"{w5.n_samples} samples, 24 + 24 nuclei, {st} steps of 0.5 ns (T = 2 us); "
+                                      f"grad_B = the run with d/dB tangents (spc_semiclassical_tangent)"}
     return out
 
 

@@ -198,6 +198,19 @@ typedef struct spc_sc_system {
 int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2],
                           double dt, int64_t n_steps, double *P_S, double *P_T, double *Phi_S, double *Phi_T);
 
+/* Derivative of the semiclassical run with respect to the field vector, along
+ * dfield (mT per unit): no counterpart in the reference.  The same samples as
+ * spc_semiclassical_run (same seed), each carrying the forward-mode tangent of
+ * its classical state through every precession, so dP_S, dP_T are the exact
+ * derivatives of the sample averages P_S, P_T (which are also returned) and
+ * dPhi_S, dPhi_T those of the Simpson yields.  dfield = the unit field
+ * direction gives d/dB at fixed direction.  P_T, Phi_*, dP_T, dPhi_* may be
+ * NULL. */
+int spc_semiclassical_tangent(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2],
+                              double dt, int64_t n_steps, const double dfield[3], double *P_S, double *P_T,
+                              double *dP_S, double *dP_T, double *Phi_S, double *Phi_T, double *dPhi_S,
+                              double *dPhi_T);
+
 /* Last-run timing of the propagation loop: milliseconds of device time and
  * number of Krylov rebuilds, for benchmarks. */
 int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps);

@@ -1353,10 +1353,15 @@ static void sc_generator(double kS, double kT, double J, double *G)
     S(12, 1, -h), S(12, 4, h), S(13, 2, -h), S(13, 5, h), S(14, 1, h), S(14, 4, -h);
 }
 
-int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2], double dt,
-                          int64_t n_steps, double *P_S, double *P_T, double *Phi_S, double *Phi_T)
+// The semiclassical run, with the tangent along the field vector dfield when
+// dfield is given (dP_S, dP_T and dPhi_S, dPhi_T then receive the derivatives).
+static int sc_common(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2], double dt,
+                     int64_t n_steps, const double *dfield, double *P_S, double *P_T, double *dP_S, double *dP_T,
+                     double *Phi_S, double *Phi_T, double *dPhi_S, double *dPhi_T)
 {
     if (!ctx || !sys || !seed || !P_S) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
+    const bool tan = dfield != nullptr;
+    if (tan && !dP_S) return fail(ctx, SPC_ERR_INVALID, "dP_S must not be NULL");
     const int nn = sys->n_nuc1 + sys->n_nuc2;
     if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || nn > 64) return fail(ctx, SPC_ERR_UNSUPPORTED, "at most 64 nuclei");
     if (nn > 0 && (!sys->mult || !sys->a_iso)) return fail(ctx, SPC_ERR_INVALID, "mult/a_iso must not be NULL");
@@ -1367,10 +1372,10 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
     if (const char *env = std::getenv("SPC_SC_LANES")) {
         lanes = std::atoi(env);
         if (lanes != 2 && lanes != 4 && lanes != 8) return fail(ctx, SPC_ERR_INVALID, "SPC_SC_LANES must be 2, 4 or 8");
-        slots = sc_slots(sys->n_nuc1, sys->n_nuc2, lanes);
+        slots = sc_slots(sys->n_nuc1, sys->n_nuc2, lanes, tan);
     } else {
         for (int l : {2, 4, 8})
-            if ((slots = sc_slots(sys->n_nuc1, sys->n_nuc2, l)) > 0) {
+            if ((slots = sc_slots(sys->n_nuc1, sys->n_nuc2, l, tan)) > 0) {
                 lanes = l;
                 break;
             }
@@ -1406,6 +1411,7 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
                            sys->field_dir[2] * sys->field_dir[2]);
     if (!(nrm > 0.0)) return fail(ctx, SPC_ERR_INVALID, "field direction must be nonzero");
     for (int a = 0; a < 3; ++a) p.w[a] = -sys->B * sys->field_dir[a] / nrm;
+    for (int a = 0; a < 3; ++a) p.dw[a] = tan ? -dfield[a] : 0.0; // w = -(field vector)
     // exp(dt/2 (K + J)) with the device Pade kernel (as a complex matrix)
     double G[256];
     sc_generator(sys->kS, sys->kT, sys->J, G);
@@ -1439,11 +1445,12 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
     const int nblocks = (int)std::min<int64_t>(groups, 1024);
     const int64_t nwaves = (int64_t)nblocks * (kScThreads / 64);
     int64_t *d_seeds = nullptr;
-    double *d_part = nullptr, *d_out = nullptr, *d_tab = nullptr;
+    double *d_part = nullptr, *d_out = nullptr, *d_tab = nullptr, *d_partd = nullptr;
     if ((rc = ensure(ctx, "sc_seeds", (size_t)2 * n_samples, &d_seeds))) return rc;
     if ((rc = ensure(ctx, "sc_tab", (size_t)(128 + 64 + 2 * lanes * slots), &d_tab))) return rc;
     if ((rc = ensure(ctx, "sc_part", (size_t)2 * (n_steps + 1) * nwaves, &d_part))) return rc;
-    if ((rc = ensure(ctx, "sc_out", (size_t)2 * (n_steps + 1), &d_out))) return rc;
+    if ((rc = ensure(ctx, "sc_out", (size_t)4 * (n_steps + 1), &d_out))) return rc;
+    if (tan && (rc = ensure(ctx, "sc_partd", (size_t)2 * (n_steps + 1) * nwaves, &d_partd))) return rc;
     std::vector<double> tab(nuc);
     tab.insert(tab.end(), kjb.begin(), kjb.end());
     tab.insert(tab.end(), cs.begin(), cs.end());
@@ -1456,11 +1463,17 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s));
-    HIP_TRY(sc_run(p, d_seeds, d_part, nblocks, n_samples, n_steps, d_out, d_out + n_steps + 1, s));
+    const int64_t ns1 = n_steps + 1;
+    HIP_TRY(sc_run(p, d_seeds, d_part, nblocks, n_samples, n_steps, d_out, d_out + ns1, s, d_partd, d_out + 2 * ns1,
+                   d_out + 3 * ns1));
     HIP_TRY(hipEventRecord(e1, s));
-    std::vector<double> pt(n_steps + 1);
-    HIP_TRY(hipMemcpyAsync(P_S, d_out, sizeof(double) * (n_steps + 1), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(pt.data(), d_out + n_steps + 1, sizeof(double) * (n_steps + 1), hipMemcpyDeviceToHost, s));
+    std::vector<double> pt(ns1), dpt(tan ? ns1 : 0);
+    HIP_TRY(hipMemcpyAsync(P_S, d_out, sizeof(double) * ns1, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pt.data(), d_out + ns1, sizeof(double) * ns1, hipMemcpyDeviceToHost, s));
+    if (tan) {
+        HIP_TRY(hipMemcpyAsync(dP_S, d_out + 2 * ns1, sizeof(double) * ns1, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(dpt.data(), d_out + 3 * ns1, sizeof(double) * ns1, hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
@@ -1477,7 +1490,35 @@ int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samp
         if (Phi_S) std::copy(phis.begin(), phis.begin() + (n_steps + 1) / 2, Phi_S);
         if (Phi_T) std::copy(phit.begin(), phit.begin() + (n_steps + 1) / 2, Phi_T);
     }
+    if (tan) {
+        // the Simpson yields are linear in the populations: their tangents are
+        // the yields of the tangent populations
+        if (dP_T) std::copy(dpt.begin(), dpt.end(), dP_T);
+        if (dPhi_S || dPhi_T) {
+            std::vector<double> zero(ns1, 0.0), phis(ns1 / 2 + 1), phit(ns1 / 2 + 1);
+            simpson(ns1, dP_S, dpt.data(), zero.data(), zero.data(), dt, sys->kS, sys->kT, phis.data(), phit.data());
+            if (dPhi_S) std::copy(phis.begin(), phis.begin() + ns1 / 2, dPhi_S);
+            if (dPhi_T) std::copy(phit.begin(), phit.begin() + ns1 / 2, dPhi_T);
+        }
+    }
     return SPC_OK;
+}
+
+int spc_semiclassical_run(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2], double dt,
+                          int64_t n_steps, double *P_S, double *P_T, double *Phi_S, double *Phi_T)
+{
+    return sc_common(ctx, sys, n_samples, seed, dt, n_steps, nullptr, P_S, P_T, nullptr, nullptr, Phi_S, Phi_T,
+                     nullptr, nullptr);
+}
+
+int spc_semiclassical_tangent(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, const int64_t seed[2],
+                              double dt, int64_t n_steps, const double dfield[3], double *P_S, double *P_T,
+                              double *dP_S, double *dP_T, double *Phi_S, double *Phi_T, double *dPhi_S,
+                              double *dPhi_T)
+{
+    if (!dfield) return fail(ctx, SPC_ERR_INVALID, "dfield must not be NULL");
+    return sc_common(ctx, sys, n_samples, seed, dt, n_steps, dfield, P_S, P_T, dP_S, dP_T, Phi_S, Phi_T, dPhi_S,
+                     dPhi_T);
 }
 
 int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,

@@ -93,15 +93,20 @@ struct ScParams {
     const double *cs;    // device [slots][lanes][cos, sin] of a_k sqrt(3)/2 dt (1, 0 for empty slots)
     const double *kjb;   // device [2][4][4]: exp(dt/2 (K+J)) on (S1x,S2x,T32,T23) and (T11,T22,T33,1), row-major
     double w[3];         // electron Zeeman field -B n
+    double dw[3];        // its tangent (the tangent kernel only)
     double dt;
 };
-// Slots per lane (2 .. 32 for 2 lanes, 2 .. 12 for 4 or 8) for n1 + n2 nuclei
-// split over `lanes` lanes; -1 if more are needed.
-int sc_slots(int n1, int n2, int lanes);
+// Slots per lane (2 .. 32 for 2 lanes, 2 .. 12 for 4 or 8, and 2 .. 8 for the
+// tangent kernel, which carries twice the state) for n1 + n2 nuclei split over
+// `lanes` lanes; -1 if more are needed.
+int sc_slots(int n1, int n2, int lanes, bool tangent = false);
 // part: 2 (n_steps + 1) nblocks kScThreads/64 doubles (zeroed here); ps, pt:
 // n_steps + 1 values of P_S(t), P_T(t).  The summation order depends on nblocks.
+// With partd (same size) the tangent kernel also gives dps, dpt: their
+// derivatives for the field moving along p.dw.
 hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
-                  int64_t n_steps, double *ps, double *pt, hipStream_t s);
+                  int64_t n_steps, double *ps, double *pt, hipStream_t s, double *partd = nullptr,
+                  double *dps = nullptr, double *dpt = nullptr);
 
 // --------------------------------------------------------------- krylov.hip
 // Kernel families timed by the optional event profiler (spc_profile_*).

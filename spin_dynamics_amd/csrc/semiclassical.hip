@@ -164,13 +164,122 @@ __device__ __forceinline__ void electron_half(double *x, const double *kA, const
 
 __device__ __forceinline__ double ps_of(const double *x) { return 0.25 * x[15] - (x[6] + x[10] + x[14]); }
 
+// ---- forward tangents (d/d eps of the state for a field w(eps) = w + eps dw)
+// A rotation by the rotation vector phi (angle |phi| about phi/|phi|) varies
+// as d R(phi) = [om]x R(phi), om = J(phi) dphi with the left Jacobian of SO(3)
+// J(phi) = 1 + (1 - cos t)/t^2 [phi]x + (t - sin t)/t^3 [phi]x^2, t = |phi|; so
+// a rotated vector v' = R v has tangent R dv + om x v'.
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// om for the electron precession about field w for time t (phi = w t), with
+// r = make_rot(w, t); the Jacobian's coefficients by their series below
+// |phi| = 0.05 (the closed forms cancel there).
+__device__ __forceinline__ void rot_jac(const double *w, const double *dw, double t, const Rot &r, double *om)
+{
+    const double th2 = (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) * t * t;
+    double A, B;
+    if (th2 < 0.0025) {
+        A = 0.5 - th2 * (1.0 / 24 - th2 * (1.0 / 720 - th2 * (1.0 / 40320)));
+        B = 1.0 / 6 - th2 * (1.0 / 120 - th2 * (1.0 / 5040 - th2 * (1.0 / 362880)));
+    } else {
+        const double th = sqrt(th2);
+        A = (1.0 - r.c) / th2;
+        B = (th - r.s) / (th2 * th);
+    }
+    double c1[3], c2[3];
+    cross3(w, dw, c1);
+    cross3(w, c1, c2);
+    const double a = A * t * t, b = B * t * t * t;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) om[d] = t * dw[d] + a * c1[d] + b * c2[d];
+}
+
+// v <- R v and dv <- R dv + om x (R v)
+__device__ __forceinline__ void rotate_tan(double *v, double *dv, const Rot &r, const double *om)
+{
+    rotate_cs(v, r.h, r.c, r.s);
+    rotate_cs(dv, r.h, r.c, r.s);
+    double c[3];
+    cross3(om, v, c);
+    dv[0] += c[0];
+    dv[1] += c[1];
+    dv[2] += c[2];
+}
+
+// electron_half on (x, dx): the linear step acts on both; every rotation
+// carries its tangent rule, in electron_half's order
+__device__ __forceinline__ void electron_half_tan(double *x, double *dx, const double *kA, const double *kB,
+                                                  const Rot &r1, const Rot &r2, const double *om1,
+                                                  const double *om2)
+{
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        double *z = pass ? dx : x;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            double y[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) s = fma(kA[4 * i + j], z[kSpinBlk[b][j]], s);
+                y[i] = s;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[kSpinBlk[b][i]] = y[i];
+        }
+        double y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s = fma(kB[4 * i + j], z[kDiagBlk[j]], s);
+            y[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[kDiagBlk[i]] = y[i];
+    }
+    rotate_tan(x + 0, dx + 0, r1, om1);
+    rotate_tan(x + 3, dx + 3, r2, om2);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) rotate_tan(x + 6 + 3 * b, dx + 6 + 3 * b, r1, om1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v[3] = {x[6 + a], x[9 + a], x[12 + a]}, dv[3] = {dx[6 + a], dx[9 + a], dx[12 + a]};
+        rotate_tan(v, dv, r2, om2);
+        x[6 + a] = v[0], x[9 + a] = v[1], x[12 + a] = v[2];
+        dx[6 + a] = dv[0], dx[9 + a] = dv[1], dx[12 + a] = dv[2];
+    }
+}
+
+// Tangent of the nuclear axis h = S/|S| (n = |S|): dh = (dS - h (h.dS)) / n;
+// u = dh and v = h x dh give a nucleus rotated by angle a about h the
+// om = sin(a) u + (1 - cos(a)) v (J(a h) a dh with dh orthogonal to h).
+__device__ __forceinline__ void axis_tan(const double *S, const double *dS, const double *h, double n, bool z,
+                                         double *u, double *v)
+{
+    const double hd = h[0] * dS[0] + h[1] * dS[1] + h[2] * dS[2];
+    const double inv = z ? 0.0 : 1.0 / n;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = (dS[d] - h[d] * hd) * inv;
+    cross3(h, u, v);
+}
+
 // part[st][wave] += sums over the wave's samples of (P_S(0) P_S(t_st),
 // P_S(0) x_16(t_st)), for every sample group the wave processes (kScThreads/L
 // samples per group).
-template <int NL, int L>
+// With TAN the kernel also carries the tangent (dx, dI) of the state for the
+// field direction p.dw and adds (P_S(0) dP_S(t), P_S(0) dx_16(t)) to
+// partd[st][wave].
+template <int NL, int L, bool TAN>
 __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__restrict__ seeds,
-                                                   double *__restrict__ part, int64_t n_samples, int64_t n_steps,
-                                                   int64_t n_groups)
+                                                   double *__restrict__ part, double *__restrict__ partd,
+                                                   int64_t n_samples, int64_t n_steps, int64_t n_groups)
 {
     const int h = threadIdx.x & (L - 1);
     double kA[16], kB[16]; // wave-uniform: scalar registers
@@ -184,6 +293,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
     const int64_t gwave = (int64_t)blockIdx.x * (kScThreads / kWave) + (threadIdx.x >> 6);
     const double shalf = 0.86602540378443864676; // sqrt(3/4)
     const double wz[3] = {p.w[0], p.w[1], p.w[2]};
+    const double dwz[3] = {p.dw[0], p.dw[1], p.dw[2]};
     const double th = 0.5 * p.dt;
     const int nl1 = p.nl1;
 
@@ -227,10 +337,25 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
         double w1[3], w2[3];
         fields<NL, L>(wz, nl1, I, w1, w2);
         Rot r1 = make_rot(w1, th), r2 = make_rot(w2, th);
+        // tangents: zero at t = 0 (the initial state does not depend on the field)
+        double dx[TAN ? 16 : 1], dI[TAN ? NL : 1][3], dw1[3], dw2[3], om1[3], om2[3];
+        if constexpr (TAN) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) dx[i] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NL; ++q) dI[q][0] = dI[q][1] = dI[q][2] = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) dw1[d] = dw2[d] = dwz[d];
+            rot_jac(w1, dw1, th, r1, om1);
+            rot_jac(w2, dw2, th, r2, om2);
+        }
 
         for (int64_t st = 0; st <= n_steps; ++st) {
             if (st > 0) {
-                electron_half(x, kA, kB, r1, r2);
+                if constexpr (TAN)
+                    electron_half_tan(x, dx, kA, kB, r1, r2, om1, om2);
+                else
+                    electron_half(x, kA, kB, r1, r2);
                 // nuclear full step about S_e / |S_e| (angle a_k sqrt(3)/2 dt)
                 const double n1 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
                 const double n2 = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
@@ -238,17 +363,45 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                 const double d1 = z1 ? 1.0 : n1, d2 = z2 ? 1.0 : n2;
                 const double h1[3] = {x[0] / d1, x[1] / d1, x[2] / d1};
                 const double h2[3] = {x[3] / d2, x[4] / d2, x[5] / d2};
+                if constexpr (TAN) {
+                    double u1[3], v1[3], u2[3], v2[3];
+                    axis_tan(x + 0, dx + 0, h1, n1, z1, u1, v1);
+                    axis_tan(x + 3, dx + 3, h2, n2, z2, u2, v2);
 #pragma unroll
-                for (int q = 0; q < NL; ++q) {
-                    if (q < nl1)
-                        rotate_cs(I[q], h1, z1 ? 1.0 : cq[q], z1 ? 0.0 : sq[q]);
-                    else
-                        rotate_cs(I[q], h2, z2 ? 1.0 : cq[q], z2 ? 0.0 : sq[q]);
+                    for (int q = 0; q < NL; ++q) {
+                        const bool e1 = q < nl1;
+                        const bool zz = e1 ? z1 : z2;
+                        const double c = zz ? 1.0 : cq[q], sn = zz ? 0.0 : sq[q];
+                        const double *hh = e1 ? h1 : h2, *uu = e1 ? u1 : u2, *vv = e1 ? v1 : v2;
+                        rotate_cs(I[q], hh, c, sn);
+                        rotate_cs(dI[q], hh, c, sn);
+                        double om[3], cr[3];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) om[d] = sn * uu[d] + (1.0 - c) * vv[d];
+                        cross3(om, I[q], cr);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) dI[q][d] += cr[d];
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NL; ++q) {
+                        if (q < nl1)
+                            rotate_cs(I[q], h1, z1 ? 1.0 : cq[q], z1 ? 0.0 : sq[q]);
+                        else
+                            rotate_cs(I[q], h2, z2 ? 1.0 : cq[q], z2 ? 0.0 : sq[q]);
+                    }
                 }
                 fields<NL, L>(wz, nl1, I, w1, w2);
                 r1 = make_rot(w1, th);
                 r2 = make_rot(w2, th);
-                electron_half(x, kA, kB, r1, r2);
+                if constexpr (TAN) {
+                    fields<NL, L>(dwz, nl1, dI, dw1, dw2);
+                    rot_jac(w1, dw1, th, r1, om1);
+                    rot_jac(w2, dw2, th, r2, om2);
+                    electron_half_tan(x, dx, kA, kB, r1, r2, om1, om2);
+                } else {
+                    electron_half(x, kA, kB, r1, r2);
+                }
             }
             double v = counts ? p0 * ps_of(x) : 0.0, u = counts ? p0 * x[15] : 0.0;
 #pragma unroll
@@ -260,6 +413,19 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                 double2 *pp = reinterpret_cast<double2 *>(part) + st * nwaves + gwave;
                 const double2 old = *pp;
                 *pp = make_double2(old.x + v, old.y + u);
+            }
+            if constexpr (TAN) {
+                double vd = counts ? p0 * ps_of(dx) : 0.0, ud = counts ? p0 * dx[15] : 0.0;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    vd += __shfl_xor(vd, o);
+                    ud += __shfl_xor(ud, o);
+                }
+                if (lane == 0) {
+                    double2 *pp = reinterpret_cast<double2 *>(partd) + st * nwaves + gwave;
+                    const double2 old = *pp;
+                    *pp = make_double2(old.x + vd, old.y + ud);
+                }
             }
         }
     }
@@ -288,47 +454,65 @@ __global__ void k_fill_zero(double *__restrict__ a, int64_t n)
         a[i] = 0.0;
 }
 
-template <int NL, int L>
-hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
-                  int64_t n_steps, hipStream_t s)
+template <int NL, int L, bool TAN>
+hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, double *partd, int nblocks,
+                  int64_t n_samples, int64_t n_steps, hipStream_t s)
 {
     const int64_t n_groups = (n_samples + kScThreads / L - 1) / (kScThreads / L);
-    hipLaunchKernelGGL((k_sc<NL, L>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, n_samples, n_steps,
-                       n_groups);
+    hipLaunchKernelGGL((k_sc<NL, L, TAN>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd, n_samples,
+                       n_steps, n_groups);
     return hipGetLastError();
 }
 
 } // namespace
 
-int sc_slots(int n1, int n2, int lanes)
+int sc_slots(int n1, int n2, int lanes, bool tangent)
 {
     const int need = (n1 + lanes - 1) / lanes + (n2 + lanes - 1) / lanes;
     for (int nl : {2, 4, 8, 12, 16, 24, 32})
-        if (need <= nl && (lanes == 2 || nl <= 12)) return nl;
+        if (need <= nl && (tangent ? nl <= 8 : (lanes == 2 || nl <= 12))) return nl;
     return -1;
 }
 
 hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nblocks, int64_t n_samples,
-                  int64_t n_steps, double *ps, double *pt, hipStream_t s)
+                  int64_t n_steps, double *ps, double *pt, hipStream_t s, double *partd, double *dps, double *dpt)
 {
     const int64_t nwaves = (int64_t)nblocks * (kScThreads / kWave);
     const int64_t npart = 2 * (n_steps + 1) * nwaves;
+    const bool tan = partd != nullptr;
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((npart + 255) / 256, 4096)), dim3(256), 0, s,
                        part, npart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (tan) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((npart + 255) / 256, 4096)), dim3(256), 0,
+                           s, partd, npart);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
 #define SPC_SC_CASE(NL, L)                                                                                         \
-    if (p.slots == NL && p.lanes == L) e = launch<NL, L>(p, seeds, part, nblocks, n_samples, n_steps, s);        \
+    if (!tan && p.slots == NL && p.lanes == L)                                                                     \
+        e = launch<NL, L, false>(p, seeds, part, partd, nblocks, n_samples, n_steps, s);                          \
+    else
+#define SPC_SC_TCASE(NL, L)                                                                                        \
+    if (tan && p.slots == NL && p.lanes == L)                                                                      \
+        e = launch<NL, L, true>(p, seeds, part, partd, nblocks, n_samples, n_steps, s);                           \
     else
     SPC_SC_CASE(2, 2) SPC_SC_CASE(4, 2) SPC_SC_CASE(8, 2) SPC_SC_CASE(12, 2) SPC_SC_CASE(16, 2)
     SPC_SC_CASE(24, 2) SPC_SC_CASE(32, 2)
     SPC_SC_CASE(2, 4) SPC_SC_CASE(4, 4) SPC_SC_CASE(8, 4) SPC_SC_CASE(12, 4)
     SPC_SC_CASE(2, 8) SPC_SC_CASE(4, 8) SPC_SC_CASE(8, 8) SPC_SC_CASE(12, 8)
+    SPC_SC_TCASE(2, 2) SPC_SC_TCASE(4, 2) SPC_SC_TCASE(8, 2)
+    SPC_SC_TCASE(2, 4) SPC_SC_TCASE(4, 4) SPC_SC_TCASE(8, 4)
+    SPC_SC_TCASE(2, 8) SPC_SC_TCASE(4, 8) SPC_SC_TCASE(8, 8)
     return hipErrorInvalidValue;
 #undef SPC_SC_CASE
+#undef SPC_SC_TCASE
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sc_reduce, dim3((unsigned)((n_steps + 1 + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const double2 *>(part), nwaves, n_steps, n_samples, ps, pt);
+    if ((e = hipGetLastError()) != hipSuccess || !tan) return e;
+    hipLaunchKernelGGL(k_sc_reduce, dim3((unsigned)((n_steps + 1 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(partd), nwaves, n_steps, n_samples, dps, dpt);
     return hipGetLastError();
 }
 

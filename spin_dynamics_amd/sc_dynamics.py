@@ -49,6 +49,93 @@ def run(a_iso1, a_iso2, mult1, mult2, *, B: float, n_samples: int, dt: float, n_
     return ps, pt, phis[:n_out // 2], phit[:n_out // 2]
 
 
+def run_tangent(a_iso1, a_iso2, mult1, mult2, *, B: float, dfield, n_samples: int, dt: float, n_steps_: int,
+                direction=(0, 0, 1), J: float = 0.0, kS: float = 0.0, kT: float = 0.0, seed=(42, 99),
+                ctx: Optional[_lib.Context] = None):
+    """run() together with the derivatives of P_S, P_T, Phi_S, Phi_T for the
+    field vector B n moving along dfield (spc_semiclassical_tangent): returns
+    (ps, pt, phis, phit), (dps, dpt, dphis, dphit)."""
+    ctx = ctx or context()
+    s, keep = sc_system(a_iso1, a_iso2, mult1, mult2, B=B, direction=direction, J=J, kS=kS, kT=kT)
+    n_out = n_steps_ + 1
+    nh = max(n_out // 2, 1)
+    ps, pt, dps, dpt = (np.zeros(n_out) for _ in range(4))
+    phis, phit, dphis, dphit = (np.zeros(nh) for _ in range(4))
+    sd = np.ascontiguousarray(seed, dtype=np.int64)
+    df = np.ascontiguousarray(dfield, dtype=np.float64)
+    ctx.check(ctx.lib.spc_semiclassical_tangent(ctx.h, s, int(n_samples), _lib.ptr(sd, _lib._i64p), float(dt),
+                                                int(n_steps_), _lib.ptr(df), _lib.ptr(ps), _lib.ptr(pt),
+                                                _lib.ptr(dps), _lib.ptr(dpt), _lib.ptr(phis), _lib.ptr(phit),
+                                                _lib.ptr(dphis), _lib.ptr(dphit)))
+    del keep
+    h = n_out // 2
+    return (ps, pt, phis[:h], phit[:h]), (dps, dpt, dphis[:h], dphit[:h])
+
+
+def _direction(theta: float, phi: float) -> np.ndarray:
+    return np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)])
+
+
+class ScYieldProblem:
+    """A semiclassical yield Phi_S(T) as a function of the field (|B|, theta,
+    phi): the remaining parameters and the samples are fixed (same seed, so
+    the estimator is a smooth function of the field)."""
+
+    def __init__(self, a_iso1, a_iso2, mult1, mult2, *, n_samples: int, dt: float, n_steps_: int, J: float = 0.0,
+                 kS: float = 0.0, kT: float = 0.0, seed=(42, 99), ctx: Optional[_lib.Context] = None):
+        self.args = (list(a_iso1), list(a_iso2), list(mult1), list(mult2))
+        self.kw = dict(n_samples=n_samples, dt=dt, n_steps_=n_steps_, J=J, kS=kS, kT=kT, seed=seed)
+        self.ctx = ctx
+
+    def value(self, B: float, theta: float, phi: float) -> float:
+        _, _, phis, _ = run(*self.args, B=B, direction=_direction(theta, phi), ctx=self.ctx, **self.kw)
+        return float(phis[-1])
+
+    def derivative(self, B: float, theta: float, phi: float, dfield) -> tuple:
+        """(Phi_S(T), its derivative along the field-vector direction dfield)."""
+        (_, _, phis, _), (_, _, dphis, _) = run_tangent(*self.args, B=B, dfield=dfield,
+                                                        direction=_direction(theta, phi), ctx=self.ctx, **self.kw)
+        return float(phis[-1]), float(dphis[-1])
+
+    def gradient(self, B: float, theta: float, phi: float, need=(True, True, True)) -> tuple:
+        """Phi_S(T) and d/d(|B|, theta, phi): one tangent run per component
+        asked for (d/dB along n, d/dtheta along B dn/dtheta, d/dphi along
+        B dn/dphi)."""
+        n = _direction(theta, phi)
+        dirs = (n, B * np.array([np.cos(theta) * np.cos(phi), np.cos(theta) * np.sin(phi), -np.sin(theta)]),
+                B * np.array([-np.sin(theta) * np.sin(phi), np.sin(theta) * np.cos(phi), 0.0]))
+        F, g = None, [0.0, 0.0, 0.0]
+        for i in range(3):
+            if need[i]:
+                F, g[i] = self.derivative(B, theta, phi, dirs[i])
+        if F is None:
+            F = self.value(B, theta, phi)
+        return F, g
+
+
+def semiclassical_yield_function():
+    """A torch.autograd.Function F = Phi_S(T)(B, theta, phi; problem) whose
+    backward uses the forward-mode tangents of spc_semiclassical_tangent (one
+    run per field component that needs a gradient)."""
+    import torch
+
+    class _ScYield(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, B, theta, phi, problem):
+            vals = [float(t.detach().cpu()) for t in (B, theta, phi)]
+            need = ctx.needs_input_grad[:3]
+            F, g = problem.gradient(*vals, need=need) if any(need) else (problem.value(*vals), [0.0] * 3)
+            ctx.save_for_backward(torch.tensor(g, dtype=torch.float64))
+            return torch.tensor(F, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gout):
+            (g,) = ctx.saved_tensors
+            return gout * g[0], gout * g[1], gout * g[2], None
+
+    return _ScYield
+
+
 def semi_classical(sys: SysParam, sim: SimParam, seed=None, out: Optional[str] = None, *, B: Optional[float] = None,
                    ctx: Optional[_lib.Context] = None) -> Observables:
     """sc_dynamics.f90:17-100 with the field B (default -sys.e1.w): N_samples
