@@ -471,39 +471,67 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ p
 }
 
 // ------------------------------------------------------------ coefficients
-// Column i of the Arnoldi process for 64 lanes per block, 4 threads per lane
-// (q = thread / 64 takes the indices k = q mod 4).
+// Column i of the Arnoldi process for kCoefLanes lanes per block, 32 threads
+// per lane (thread q takes the index k = q; i < kMaxKrylov = 32).  The
+// per-lane dots and the Hessenberg columns 0..i-2 are staged in LDS first
+// (columns packed, rows <= col + 1), so the O(i^2) algebra reads LDS rather
+// than chains of global loads.
 //   d1: [c, d] of u_i from the product;  d2: [a_0..a_{i-1}, s_0..s_{i-1}]
 //   from the previous update.  Writes coef = [cq_0..cq_{i-1}, cu_0..cu_{i-1},
 //   cqu, cuu, cuv], tprev = t_i, Hessenberg column i-1, beta0 (i = 0).
+constexpr int kCoefLanes = 8;
 __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, double kS, double kT,
                                                     const double2 *__restrict__ d1, const double2 *__restrict__ d2,
                                                     double2 *__restrict__ coef, double2 *__restrict__ tprev,
                                                     double2 *__restrict__ hess, double *__restrict__ beta0,
                                                     int32_t *__restrict__ flags)
 {
-    __shared__ double2 sb[kMaxKrylov][kWave];
-    __shared__ double red[kWaves][kWave];
-    __shared__ double2 redc[kWaves][kWave];
-    __shared__ double sbeta[kWave];
-    __shared__ double2 sgi[kWave], ss[kWave];
-    const int lane = threadIdx.x & (kWave - 1), q = threadIdx.x / kWave;
-    const int r = blockIdx.x * kWave + lane;
+    constexpr int NL = kCoefLanes, TP = kBlock / kCoefLanes;
+    constexpr int kPack = kMaxKrylov * (kMaxKrylov + 3) / 2;
+    static_assert(TP >= kMaxKrylov, "one thread per index k");
+    __shared__ double2 sh[NL][kPack];
+    __shared__ double2 sa[NL][kMaxKrylov], sst[NL][kMaxKrylov], sb[NL][kMaxKrylov];
+    __shared__ double red[TP][NL];
+    __shared__ double2 redc[TP][NL];
+    __shared__ double sbeta[NL];
+    __shared__ double2 sgi[NL], ss[NL];
+    const int l = threadIdx.x % NL, q = threadIdx.x / NL;
+    const int r = blockIdx.x * NL + l;
     const bool active = r < M;
     const int64_t ld = Mp;
-    auto A = [&](int k) { return d2[(int64_t)k * ld + r]; };
-    auto S = [&](int k) { return d2[(int64_t)(i + k) * ld + r]; };
+    auto off = [](int col) { return col * (col + 3) / 2; };
     auto H = [&](int row, int col) -> double2 & { return hess[((int64_t)col * m + row) * ld + r]; };
 
+    if (q < i) {
+        sa[l][q] = d2[(int64_t)q * ld + r];
+        sst[l][q] = d2[(int64_t)(i + q) * ld + r];
+    }
+    const int ne = i >= 2 ? off(i - 1) : 0; // packed entries of columns 0..i-2
+    // six loads in flight per thread; every slot loads (clamped index): loads
+    // under branches are issued one at a time
+    constexpr int kBatch = 6;
+    for (int u0 = 0; u0 * TP < ne; u0 += kBatch) {
+        double2 hv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = min(q + TP * (u0 + u), ne - 1);
+            int col = (int)((sqrt(9.0 + 8.0 * e) - 3.0) * 0.5); // then one rounding correction
+            col += off(col + 1) <= e ? 1 : 0;
+            col -= off(col) > e ? 1 : 0;
+            hv[u] = H(e - off(col), col);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (q + TP * (u0 + u) < ne) sh[l][q + TP * (u0 + u)] = hv[u];
+    }
+    __syncthreads();
+
     // beta_i^2 = d - |a|^2
-    double p = 0.0;
-    if (active)
-        for (int k = q; k < i; k += kWaves) p += cabs2(A(k));
-    red[q][lane] = p;
+    red[q][l] = active && q < i ? cabs2(sa[l][q]) : 0.0;
     __syncthreads();
     if (q == 0) {
         double beta2 = active ? d1[ld + r].x : 1.0;
-        for (int w = 0; w < kWaves; ++w) beta2 -= red[w][lane];
+        for (int w = 0; w < TP; ++w) beta2 -= red[w][l];
         double beta = 0.0;
         if (active) {
             if (!(beta2 > 0.0) || !isfinite(beta2))
@@ -512,65 +540,72 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, 
                 beta = sqrt(beta2);
             if (i == 0) beta0[r] = beta;
         }
-        sbeta[lane] = beta;
+        sbeta[l] = beta;
     }
     __syncthreads();
-    const double beta = sbeta[lane];
+    const double beta = sbeta[l];
     const bool ok = active && beta > 0.0;
     const double ib = ok ? 1.0 / beta : 0.0;
     // finalise column i-1: h(k,i-1) = t_{i-1}[k] + a_k, h(i,i-1) = beta
     if (ok && i > 0) {
-        for (int k = q; k < i; k += kWaves) H(k, i - 1) = cadd(tprev[(int64_t)k * ld + r], A(k));
-        if (q == 0) H(i, i - 1) = cz(beta, 0.0);
+        if (q < i) {
+            const double2 h = cadd(tprev[(int64_t)q * ld + r], sa[l][q]);
+            H(q, i - 1) = h;
+            sh[l][off(i - 1) + q] = h;
+        }
+        if (q == 0) {
+            H(i, i - 1) = cz(beta, 0.0);
+            sh[l][off(i - 1) + i] = cz(beta, 0.0);
+        }
     }
     __syncthreads();
-    // b_k and g_k for k = q mod 4; g_i kept for the last coefficient
+    // b_k and g_k for k = q; g_i kept for the last coefficient
     double2 abp = cz(0, 0);
-    if (ok) {
-        for (int k = q; k <= i; k += kWaves) {
-            double2 g = cz(0, 0); // g_k = sum_{j >= k-1} h(k,j) a_j
-            for (int j = k > 0 ? k - 1 : 0; j < i; ++j) g = cfma(H(k, j), A(j), g);
-            if (k == i) {
-                sgi[lane] = g;
-                continue;
-            }
+    if (ok && q <= i) {
+        const int k = q;
+        double2 g = cz(0, 0); // g_k = sum_{j >= k-1} h(k,j) a_j
+        for (int j = k > 0 ? k - 1 : 0; j < i; ++j) g = cfma(sh[l][off(j) + k], sa[l][j], g);
+        if (k == i) {
+            sgi[l] = g;
+        } else {
             double2 b = cz(0, 0); // (H q_k)^H u_i
             const int lmax = k + 1 < i ? k + 1 : i - 1;
-            for (int l = 0; l <= lmax; ++l) b = cfma_conj(H(l, k), A(l), b);
+            for (int ll = 0; ll <= lmax; ++ll) b = cfma_conj(sh[l][off(k) + ll], sa[l][ll], b);
             if (k == i - 1) b.x += beta * beta; // h(i,i-1) q_i^H u_i = beta^2
-            const double2 a = A(k), s = S(k);
+            const double2 a = sa[l][k], sv = sst[l][k];
             // - i (kT a_k + (kS - kT) s_k)
-            const double2 ka = cz(kT * a.x + (kS - kT) * s.x, kT * a.y + (kS - kT) * s.y);
+            const double2 ka = cz(kT * a.x + (kS - kT) * sv.x, kT * a.y + (kS - kT) * sv.y);
             b = cadd(b, cz(ka.y, -ka.x));
-            sb[k][lane] = b;
+            sb[l][k] = b;
             tprev[(int64_t)k * ld + r] = cscale(ib, csub(b, g));
             coef[(int64_t)k * ld + r] = cscale(-ib, a);
             abp = cfma_conj(a, b, abp);
         }
     }
-    redc[q][lane] = abp;
+    redc[q][l] = abp;
     __syncthreads();
     if (q == 0) {
-        double2 s = cz(0, 0);
+        double2 sv = cz(0, 0);
         if (ok) {
             double2 ab = cz(0, 0);
-            for (int w = 0; w < kWaves; ++w) ab = cadd(ab, redc[w][lane]);
+            for (int w = 0; w < TP; ++w) ab = cadd(ab, redc[w][l]);
             const double2 c = d1[r];
-            const double2 gi = sgi[lane];
+            const double2 gi = sgi[l];
             const double2 ti = cscale(ib, csub(cscale(ib, csub(c, ab)), gi));
             tprev[(int64_t)i * ld + r] = ti;
-            s = cadd(cscale(ib, gi), ti);
+            sv = cadd(cscale(ib, gi), ti);
         }
-        ss[lane] = s;
+        ss[l] = sv;
         coef[(int64_t)(2 * i) * ld + r] = cz(ib, 0.0);
-        coef[(int64_t)(2 * i + 1) * ld + r] = cscale(-ib, s);
+        coef[(int64_t)(2 * i + 1) * ld + r] = cscale(-ib, sv);
         coef[(int64_t)(2 * i + 2) * ld + r] = cz(ib, 0.0);
     }
     __syncthreads();
-    const double2 s = ss[lane];
-    for (int k = q; k < i; k += kWaves) { // cu_k = -(b_k - s a_k)/beta
+    const double2 sv = ss[l];
+    if (q < i) { // cu_k = -(b_k - s a_k)/beta
+        const int k = q;
         double2 cu = cz(0, 0);
-        if (ok) cu = cscale(-ib, csub(sb[k][lane], cmul(s, A(k))));
+        if (ok) cu = cscale(-ib, csub(sb[l][k], cmul(sv, sa[l][k])));
         coef[(int64_t)(i + k) * ld + r] = cu;
         if (!ok) coef[(int64_t)k * ld + r] = cz(0, 0);
     }
@@ -953,6 +988,35 @@ __global__ void k_step(int m, int M, int Mp, double dt, double t_end, double tol
     if (r >= M) return;
     double t = lane_t[r];
     int64_t k = lane_k[r];
+    if (!gram) {
+        // direct mode: one step from c = beta e_1, i.e. c = beta E(:,0) (the
+        // general loop's sum adds only E(i,j) * 0 beyond j = 0: the same bits),
+        // with the loop's bookkeeping; a second step would flag the lane
+        const double b0 = beta0[r];
+        if (t < t_end) {
+            if (k + 1 >= n_out) {
+                for (int i = 0; i < m; ++i) cout[(int64_t)i * Mp + r] = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
+                flags[3] = 1;
+                lane_k[r] = (int32_t)(k + 1);
+                return;
+            }
+            double c2 = 0.0, cm2 = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double2 ci = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
+                cout[(int64_t)i * Mp + r] = ci;
+                c2 += cabs2(ci);
+                if (i == m - 1) cm2 = cabs2(ci);
+            }
+            k = k + 1;
+            t = t + dt;
+            if (cm2 <= tol * c2 && t < t_end) flags[1] = 1;
+        } else {
+            for (int j = 0; j < m; ++j) cout[(int64_t)j * Mp + r] = j == 0 ? cz(b0, 0.0) : cz(0, 0);
+        }
+        lane_t[r] = t;
+        lane_k[r] = (int32_t)k;
+        return;
+    }
     double2 c[kMaxKrylov], cn[kMaxKrylov];
     for (int j = 0; j < m; ++j) c[j] = cz(0, 0);
     c[0] = cz(beta0[r], 0.0);
@@ -1164,7 +1228,7 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
         if ((e = reduce(w, 2, nrb1, w.dots1, s)) != hipSuccess) return e;
         {
             ProfScope ps(w, PF_COEFFS, 0.0, s);
-            hipLaunchKernelGGL(k_coeffs, dim3(nch), dim3(kBlock), 0, s, i, w.m, w.M, w.Mp, w.kS, w.kT, w.dots1,
+            hipLaunchKernelGGL(k_coeffs, dim3(w.Mp / kCoefLanes), dim3(kBlock), 0, s, i, w.m, w.M, w.Mp, w.kS, w.kT, w.dots1,
                                w.dots, w.coef, w.tprev, w.hess, w.beta0, w.flags);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
