@@ -630,6 +630,13 @@ __global__ void k_finalize(int m, int M, int Mp, const double2 *__restrict__ dot
 // wave w then forms, for k = w mod 4, the dots of u_{i+1} with q_k over the 4
 // rows (a) and of the singlet parts, rows ab - ba, (s).  LAST: column m-1 only
 // needs a, and u_m is not stored.
+// From this many basis vectors on, the update's dot phase exchanges rows
+// through LDS instead of re-reading them (see k_update).
+#ifndef SPC_UPDATE_LDS_DOTS
+#define SPC_UPDATE_LDS_DOTS 13
+#endif
+constexpr int kUpdateLdsDots = SPC_UPDATE_LDS_DOTS;
+
 template <int NQ, bool LAST>
 __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q, const double2 *__restrict__ Qre,
                                                     double2 *__restrict__ qout, double2 *__restrict__ u,
@@ -643,6 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
     constexpr int NK = (NA + kWaves - 1) / kWaves; // dot indices per wave
     __shared__ double2 sc[NC * kWave];
     __shared__ double2 sun[kWaves * kWave], sqi[kWaves * kWave];
+    __shared__ double2 sx[NQ >= kUpdateLdsDots ? kWaves : 1][kWaves][kWave];
     int rb, ch;
     block_map(blockIdx.x, nrb, Mp / kWave, rb, ch);
     const int lane = threadIdx.x & (kWave - 1);
@@ -691,28 +699,58 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
 #pragma unroll
         for (int e = 0; e < 4; ++e) w4[e] = sun[e * kWave + lane];
         const double2 dun = csub(w4[1], w4[2]);
-        const int64_t base = (4 * n) * Mp + r;
-        double2 xd[NK][4]; // the dot phase's rows, all loads issued first as above
+        if constexpr (NQ >= kUpdateLdsDots) {
+            // the dot phase's rows come from the waves that loaded them, through
+            // LDS, four basis indices (one per wave) at a time: re-reading them
+            // from L2 misses once 64 blocks per XCD hold NQ * 4 KiB each
 #pragma unroll
-        for (int j = 0; j < NK; ++j) {
-            const int k = wave + kWaves * j;
+            for (int c0 = 0; c0 < NA; c0 += kWaves) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                xd[j][e] = k < NQ ? ntload(Qre + k * NMp + base + e * Mp) : sqi[e * kWave + lane];
+                for (int kk = 0; kk < kWaves; ++kk) {
+                    if (c0 + kk < NQ)
+                        sx[kk][wave][lane] = xq[c0 + kk < NQ ? c0 + kk : 0];
+                    else if (c0 + kk == NQ)
+                        sx[kk][wave][lane] = qi;
+                }
+                __syncthreads();
+                const int k = c0 + wave;
+                if (k <= NQ) {
+                    double2 x[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) x[e] = sx[wave][e][lane];
+                    double2 t = aa[c0 / kWaves];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) t = cfma_conj(x[e], w4[e], t);
+                    aa[c0 / kWaves] = t;
+                    if constexpr (!LAST)
+                        as[c0 / kWaves] = cfma_conj(cscale(0.5, csub(x[1], x[2])), dun, as[c0 / kWaves]);
+                }
+                __syncthreads(); // sx (and, last, sun/sqi) are rewritten next
+            }
+        } else {
+            const int64_t base = (4 * n) * Mp + r;
+            double2 xd[NK][4]; // the dot phase's rows, all loads issued first as above
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const int k = wave + kWaves * j;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    xd[j][e] = k < NQ ? ntload(Qre + k * NMp + base + e * Mp) : sqi[e * kWave + lane];
+            }
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const int k = wave + kWaves * j;
+                if (k > NQ) break;
+                const double2 *x = xd[j];
+                double2 t = aa[j];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) t = cfma_conj(x[e], w4[e], t);
+                aa[j] = t;
+                if constexpr (!LAST) as[j] = cfma_conj(cscale(0.5, csub(x[1], x[2])), dun, as[j]);
+            }
+            __syncthreads(); // sun/sqi are rewritten for the next nuclear index
         }
-        __asm__ volatile("" ::: "memory");
-#pragma unroll
-        for (int j = 0; j < NK; ++j) {
-            const int k = wave + kWaves * j;
-            if (k > NQ) break;
-            const double2 *x = xd[j];
-            double2 t = aa[j];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) t = cfma_conj(x[e], w4[e], t);
-            aa[j] = t;
-            if constexpr (!LAST) as[j] = cfma_conj(cscale(0.5, csub(x[1], x[2])), dun, as[j]);
-        }
-        __syncthreads(); // sun/sqi are rewritten for the next nuclear index
     }
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
