@@ -46,8 +46,10 @@ struct HamDev {
     KronDev *kron = nullptr; // Kronecker form (when H has it exactly; else null)
     double2 *kdiag = nullptr, *kgen = nullptr;
     int kq = 0; // in-wave nuclei of the Kronecker product (kron_inwave)
+    uint64_t *kmask = nullptr; // [Z][3] neighbour masks of the Kronecker product
     void release()
     {
+        hipFree(kmask);
         hipFree(kron);
         hipFree(kdiag);
         hipFree(kgen);
@@ -252,6 +254,7 @@ void set_ham(KrylovWork &w, const HamDev &h)
     w.kgen = kr ? h.kgen : nullptr;
     w.kdiag_stride = w.kgen_stride = 0;
     w.kq = kr ? h.kq : 0;
+    w.kmask = kr ? h.kmask : nullptr;
 }
 
 KronDev kron_of(const DevSystem &S)
@@ -344,6 +347,10 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
         hipFree(h.kgen);
         h.kron = nullptr;
         h.kdiag = h.kgen = nullptr;
+    } else {
+        HIP_TRY(hipMalloc(&h.kmask, sizeof(uint64_t) * 3 * Z));
+        HIP_TRY(kron_masks(h.kron, Z, h.kmask, s));
+        HIP_TRY(hipStreamSynchronize(s));
     }
     return SPC_OK;
 }
@@ -797,6 +804,7 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
             w.kdiag = kd;
             w.kgen = kg;
             w.kq = kron_inwave(kron_of(hs));
+            w.kmask = ctx->fpat.kmask;
             w.kdiag_stride = Z * 16;
             w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
         }

@@ -459,6 +459,31 @@ hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *z
     return hipGetLastError();
 }
 
+// Neighbour masks per nuclear index n over all nuclei k: minus (digit d_k > 0),
+// plus (d_k < mult_k - 1), one (d_k = 1); out[3n + 0, 1, 2].
+__global__ void k_kron_masks(const KronDev *__restrict__ kd, int64_t Z, uint64_t *__restrict__ out)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Z) return;
+    uint64_t mi = 0, pl = 0, on = 0;
+    for (int k = 0; k < kd->nn; ++k) {
+        const int mu = kd->mult[k];
+        const int d = (int)((n / kd->stride[k]) % mu);
+        if (d > 0) mi |= 1ull << k;
+        if (d < mu - 1) pl |= 1ull << k;
+        if (d == 1) on |= 1ull << k;
+    }
+    out[3 * n] = mi;
+    out[3 * n + 1] = pl;
+    out[3 * n + 2] = on;
+}
+
+hipError_t kron_masks(const KronDev *d_kron, int64_t Z, uint64_t *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_kron_masks, dim3((unsigned)((Z + 255) / 256)), dim3(256), 0, s, d_kron, Z, out);
+    return hipGetLastError();
+}
+
 hipError_t kron_tables(const KronDev *d_kron, const KronDev &h_kron, const int32_t *browptr, const int32_t *bcol,
                        const double2 *bval, int64_t nb, int nsets, int64_t Z, double2 *diag, double2 *gen,
                        int32_t *bad, hipStream_t s)

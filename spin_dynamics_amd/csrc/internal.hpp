@@ -34,12 +34,13 @@ struct KronDev {
     int32_t elec[64]; // 0: electron 1 (spin s1 of e = 2 s1 + s2), 1: electron 2
 };
 
-// How many of the last nuclei (at most 3) are spin-1/2 with strides 1, 2, 4:
-// the product can then give a wave 2^q consecutive nuclear indices.
+// How many of the last nuclei (at most 8) are spin-1/2 with strides 1, 2, 4,
+// ...: the product can then give a wave (and a workgroup) 2^q consecutive
+// nuclear indices.
 inline int kron_inwave(const KronDev &k)
 {
     int q = 0;
-    while (q < 3 && q < k.nn && k.mult[k.nn - 1 - q] == 2 && k.stride[k.nn - 1 - q] == (1 << q)) ++q;
+    while (q < 8 && q < k.nn && k.mult[k.nn - 1 - q] == 2 && k.stride[k.nn - 1 - q] == (1 << q)) ++q;
     return q;
 }
 
@@ -68,6 +69,8 @@ hipError_t h_block_fill_pattern(const int32_t *rowptr, const int32_t *col, const
 hipError_t kron_tables(const KronDev *d_kron, const KronDev &h_kron, const int32_t *browptr, const int32_t *bcol,
                        const double2 *bval, int64_t nb, int nsets, int64_t Z, double2 *diag, double2 *gen,
                        int32_t *bad, hipStream_t s);
+// Per nuclear index the neighbour masks (minus, plus, one) over all nuclei: out[3 Z].
+hipError_t kron_masks(const KronDev *d_kron, int64_t Z, uint64_t *out, hipStream_t s);
 // out[o] = v0 + sum_a fields[o][a] z_a for n_fields field vectors (count values each).
 hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *zy, const double2 *zz,
                           const double *fields, int64_t count, int n_fields, double2 *out, hipStream_t s);
@@ -172,7 +175,8 @@ struct KrylovWork {
     const double2 *kdiag;    // [Z][16]
     const double2 *kgen;     // [nn][2][3][4]
     int64_t kdiag_stride, kgen_stride;
-    int kq;                  // trailing spin-1/2 nuclei of strides 1, 2, 4 usable in-wave (0..3)
+    int kq;                  // trailing spin-1/2 nuclei of strides 1, 2, 4, .. (kron_inwave, 0..8)
+    const uint64_t *kmask;   // [Z][3] neighbour masks (kron_masks), or null
     double2 *Q;      // [m] basis vectors [N][Mp], qs elements apart (qstride)
     int64_t qs;      // basis vector stride in elements (0: N*Mp)
     double2 *u;      // [N][Mp]

@@ -214,6 +214,15 @@ __device__ __forceinline__ NbrMasks nbr_masks(const KronDev *__restrict__ kd, in
     return m;
 }
 
+// The masks of n over the nuclei k < kmax, from the precomputed table when there is one.
+__device__ __forceinline__ NbrMasks masks_at(const uint64_t *__restrict__ kmask, const KronDev *__restrict__ kd, int n,
+                                             int kmax)
+{
+    if (!kmask) return nbr_masks(kd, n, kmax);
+    const uint64_t keep = kmax >= 64 ? ~0ull : (1ull << kmax) - 1;
+    return NbrMasks{kmask[3 * (int64_t)n] & keep, kmask[3 * (int64_t)n + 1] & keep, kmask[3 * (int64_t)n + 2] & keep};
+}
+
 // y += the flips of the existing neighbours in `mask`, G at a time: minus
 // neighbours n - stride_k in ascending k, plus neighbours n + stride_k in
 // descending k (ascending columns either way).  The G rows of a batch are
@@ -259,7 +268,9 @@ __device__ __forceinline__ void kron_neighbours(const KronDev *__restrict__ kd, 
 }
 
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict__ kd, const double2 *__restrict__ diags,
+__global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict__ kd,
+                                                       const uint64_t *__restrict__ kmask,
+                                                       const double2 *__restrict__ diags,
                                                        const double2 *__restrict__ gens, int64_t dstride,
                                                        int64_t gstride, int chunks_per_set,
                                                        const double2 *__restrict__ u, double2 *__restrict__ v,
@@ -284,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
 #pragma unroll
         for (int f = 0; f < 4; ++f) x0[f] = u[(4 * n + f) * Mp + r];
         double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
-        const NbrMasks nm = nbr_masks(kd, nu, nn);
+        const NbrMasks nm = masks_at(kmask, kd, nu, nn);
         const int64_t base = 4 * n * Mp + r, M4 = 4 * (int64_t)Mp;
         kron_neighbours<G, true>(kd, gen, u, base, M4, Mp, nm.minus, nm.one, y);
         // the diagonal block
@@ -319,6 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
 // the order of k_spmm_kron (bit-identical products).
 template <int G, int Q>
 __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__restrict__ kd,
+                                                              const uint64_t *__restrict__ kmask,
                                                               const double2 *__restrict__ diags,
                                                               const double2 *__restrict__ gens, int64_t dstride,
                                                               int64_t gstride, int chunks_per_set,
@@ -366,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
         for (int f = 0; f < 4; ++f) x0[f] = u[rowb + toff + f * Mp];
         double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
         // neighbours n - stride_k of the uniform nuclei, k ascending
-        const NbrMasks nm = nbr_masks(kd, nb, nuni);
+        const NbrMasks nm = masks_at(kmask, kd, nb, nuni);
         kron_neighbours<G, true>(kd, gen, u, rowb + toff, M4, Mp, nm.minus, nm.one, y);
         // the in-wave nuclei: every index has exactly one neighbour per nucleus,
         // g ^ 2^j (below it when bit j of g is set, above it otherwise), whose
@@ -446,6 +458,156 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
     }
 }
 
+// The lane-slice product with NW waves per workgroup on NW consecutive quads
+// (tiles of NW 2^Q consecutive nuclear indices): the BD = log2(NW) nuclei above
+// the in-wave ones have wave-uniform digits (the wave's bits), and their flips
+// pair the workgroup's own waves, so those neighbour rows also come from LDS
+// instead of L2.  Needs the last Q + BD nuclei spin-1/2.  Same summation order
+// as k_spmm_kron (bit-identical products).  rpg = tiles per workgroup.
+template <int G, int Q, int NW>
+__global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__restrict__ kd,
+                                                            const uint64_t *__restrict__ kmask,
+                                                            const double2 *__restrict__ diags,
+                                                            const double2 *__restrict__ gens, int64_t dstride,
+                                                            int64_t gstride, int chunks_per_set,
+                                                            const double2 *__restrict__ u, double2 *__restrict__ v,
+                                                            double2 *__restrict__ part, int64_t Z, int Mp, int rpg,
+                                                            int nrb)
+{
+    constexpr int W = kWave >> Q;
+    constexpr int NG = 1 << Q;
+    constexpr int BD = NW == 16 ? 4 : NW == 8 ? 3 : NW == 4 ? 2 : NW == 2 ? 1 : 0;
+    static_assert((1 << BD) == NW, "NW a power of two up to 16");
+    __shared__ double2 red[NW - 1][2][W];
+    __shared__ double2 dl[NW][NG * 16];
+    __shared__ double2 xl[NW][NG][4][W]; // the workgroup's own rows, for the in-tile flips
+    const int nsl = Mp / W;
+    int rb, sl;
+    if ((nsl & 7) == 0) {
+        const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+        sl = (j / nrb) * 8 + x;
+        rb = j % nrb;
+    } else {
+        sl = blockIdx.x / nrb;
+        rb = blockIdx.x % nrb;
+    }
+    const int t = threadIdx.x & (kWave - 1);
+    const int lane = t & (W - 1), g = t >> (6 - Q);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int r = sl * W + lane;
+    const int set = (sl * W / kWave) / chunks_per_set;
+    const double2 *__restrict__ diag = diags + (int64_t)set * dstride;
+    const double2 *__restrict__ gen = gens + (int64_t)set * gstride;
+    const int nn = kd->nn, nuni = nn - Q - BD; // nuclei 0 .. nuni-1: digits uniform over the tile
+    const int64_t M4 = 4 * (int64_t)Mp;
+    const int64_t toff = (int64_t)g * M4 + r;
+    double2 acc[2] = {cz(0, 0), cz(0, 0)};
+    const int64_t Zt = Z >> (Q + BD);
+    const int64_t t0 = (int64_t)rb * rpg;
+    const int64_t t1 = t0 + rpg < Zt ? t0 + rpg : Zt;
+    for (int64_t ti = t0; ti < t1; ++ti) {
+        const int nb = (int)((((ti << BD) + wave)) << Q); // this wave's quad
+        const int64_t rowb = (int64_t)nb * M4;
+#pragma unroll
+        for (int i = 0; i < (NG * 16 + kWave - 1) / kWave; ++i)
+            if (i * kWave + t < NG * 16) dl[wave][i * kWave + t] = diag[(int64_t)nb * 16 + i * kWave + t];
+        double2 x0[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) x0[f] = u[rowb + toff + f * Mp];
+        double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
+        // neighbours n - stride_k of the uniform nuclei, k ascending
+        const NbrMasks nm = masks_at(kmask, kd, nb, nuni);
+        kron_neighbours<G, true>(kd, gen, u, rowb + toff, M4, Mp, nm.minus, nm.one, y);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
+        __syncthreads();
+        // the tile nuclei (bit b of the wave: partner wave ^ 2^b), k ascending
+#pragma unroll
+        for (int b = BD - 1; b >= 0; --b) {
+            const int k = nn - 1 - Q - b;
+            if ((wave >> b) & 1) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave ^ (1 << b)][g][f][lane];
+                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, kd->elec[k], xp, y);
+            }
+        }
+        // the in-wave nuclei (bit j of g: partner lane group g ^ 2^j), k ascending
+#pragma unroll
+        for (int j = Q - 1; j >= 0; --j) {
+            const int k = nn - 1 - j;
+            if ((g >> j) & 1) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
+                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, kd->elec[k], xp, y);
+            }
+        }
+        {
+            const double2 *D = &dl[wave][g * 16];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                double2 de[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) de[f] = D[4 * e + f];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) y[e] = cfma(de[f], x0[f], y[e]);
+                __asm__ volatile("" ::: "memory");
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < Q; ++j) { // in-wave, k descending
+            const int k = nn - 1 - j;
+            if (!((g >> j) & 1)) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
+                kron_apply(gen + ((int64_t)k * 6) * 4, kd->elec[k], xp, y);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < BD; ++b) { // tile nuclei, k descending
+            const int k = nn - 1 - Q - b;
+            if (!((wave >> b) & 1)) {
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = xl[wave ^ (1 << b)][g][f][lane];
+                kron_apply(gen + ((int64_t)k * 6) * 4, kd->elec[k], xp, y);
+            }
+        }
+        // neighbours n + stride_k of the uniform nuclei, k descending
+        kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[rowb + toff + e * Mp] = y[e];
+            acc[0] = cfma_conj(x0[e], y[e], acc[0]);
+            acc[1].x += cabs2(x0[e]);
+        }
+        __syncthreads(); // xl and dl are rewritten for the next tile
+    }
+#pragma unroll
+    for (int s = W; s < kWave; s <<= 1)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double2 o = cz(__shfl_xor(acc[j].x, s), __shfl_xor(acc[j].y, s));
+            acc[j] = cadd(acc[j], o);
+        }
+    if (wave > 0 && g == 0) {
+        red[wave - 1][0][lane] = acc[0];
+        red[wave - 1][1][lane] = acc[1];
+    }
+    __syncthreads();
+    if (wave == 0 && g == 0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double2 s = acc[j];
+#pragma unroll
+            for (int w = 0; w < NW - 1; ++w) s = cadd(s, red[w][j][lane]);
+            part[((int64_t)rb * 2 + j) * Mp + r] = s;
+        }
+    }
+}
+
 // Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].
 __global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ part, double2 *__restrict__ out,
                                                     int nacc, int nrb, int Mp)
@@ -480,16 +642,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ p
 //   from the previous update.  Writes coef = [cq_0..cq_{i-1}, cu_0..cu_{i-1},
 //   cqu, cuu, cuv], tprev = t_i, Hessenberg column i-1, beta0 (i = 0).
 constexpr int kCoefLanes = 8;
-__global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, double kS, double kT,
+// (at most 4 waves per SIMD: the compiler would otherwise spill the staging
+// registers to reach 8)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_coeffs(
+    int i, int m, int M, int Mp, double kS, double kT,
                                                     const double2 *__restrict__ d1, const double2 *__restrict__ d2,
                                                     double2 *__restrict__ coef, double2 *__restrict__ tprev,
                                                     double2 *__restrict__ hess, double *__restrict__ beta0,
                                                     int32_t *__restrict__ flags)
 {
     constexpr int NL = kCoefLanes, TP = kBlock / kCoefLanes;
-    constexpr int kPack = kMaxKrylov * (kMaxKrylov + 3) / 2;
     static_assert(TP >= kMaxKrylov, "one thread per index k");
-    __shared__ double2 sh[NL][kPack];
+    // packed Hessenberg columns of the NL lanes, lane-major: NL * coef_pack(m)
+    // values of dynamic LDS (sized by the Krylov order at launch)
+    extern __shared__ double2 shd[];
+    const int pack = m * (m + 3) / 2;
     __shared__ double2 sa[NL][kMaxKrylov], sst[NL][kMaxKrylov], sb[NL][kMaxKrylov];
     __shared__ double red[TP][NL];
     __shared__ double2 redc[TP][NL];
@@ -499,8 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, 
     const int r = blockIdx.x * NL + l;
     const bool active = r < M;
     const int64_t ld = Mp;
+    const int lbase = l * pack; // this lane's packed columns in shd
     auto off = [](int col) { return col * (col + 3) / 2; };
-    auto H = [&](int row, int col) -> double2 & { return hess[((int64_t)col * m + row) * ld + r]; };
 
     if (q < i) {
         sa[l][q] = d2[(int64_t)q * ld + r];
@@ -518,11 +685,11 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, 
             int col = (int)((sqrt(9.0 + 8.0 * e) - 3.0) * 0.5); // then one rounding correction
             col += off(col + 1) <= e ? 1 : 0;
             col -= off(col) > e ? 1 : 0;
-            hv[u] = H(e - off(col), col);
+            hv[u] = hess[((int64_t)col * m + (e - off(col))) * ld + r];
         }
+        // (slots past the end rewrite the last entry with its own value)
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u)
-            if (q + TP * (u0 + u) < ne) sh[l][q + TP * (u0 + u)] = hv[u];
+        for (int u = 0; u < kBatch; ++u) shd[lbase + min(q + TP * (u0 + u), ne - 1)] = hv[u];
     }
     __syncthreads();
 
@@ -550,12 +717,12 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, 
     if (ok && i > 0) {
         if (q < i) {
             const double2 h = cadd(tprev[(int64_t)q * ld + r], sa[l][q]);
-            H(q, i - 1) = h;
-            sh[l][off(i - 1) + q] = h;
+            hess[((int64_t)(i - 1) * m + q) * ld + r] = h;
+            shd[lbase + off(i - 1) + q] = h;
         }
         if (q == 0) {
-            H(i, i - 1) = cz(beta, 0.0);
-            sh[l][off(i - 1) + i] = cz(beta, 0.0);
+            hess[((int64_t)(i - 1) * m + i) * ld + r] = cz(beta, 0.0);
+            shd[lbase + off(i - 1) + i] = cz(beta, 0.0);
         }
     }
     __syncthreads();
@@ -564,13 +731,13 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int i, int m, int M, int Mp, 
     if (ok && q <= i) {
         const int k = q;
         double2 g = cz(0, 0); // g_k = sum_{j >= k-1} h(k,j) a_j
-        for (int j = k > 0 ? k - 1 : 0; j < i; ++j) g = cfma(sh[l][off(j) + k], sa[l][j], g);
+        for (int j = k > 0 ? k - 1 : 0; j < i; ++j) g = cfma(shd[lbase + off(j) + k], sa[l][j], g);
         if (k == i) {
             sgi[l] = g;
         } else {
             double2 b = cz(0, 0); // (H q_k)^H u_i
             const int lmax = k + 1 < i ? k + 1 : i - 1;
-            for (int ll = 0; ll <= lmax; ++ll) b = cfma_conj(sh[l][off(k) + ll], sa[l][ll], b);
+            for (int ll = 0; ll <= lmax; ++ll) b = cfma_conj(shd[lbase + off(k) + ll], sa[l][ll], b);
             if (k == i - 1) b.x += beta * beta; // h(i,i-1) q_i^H u_i = beta^2
             const double2 a = sa[l][k], sv = sst[l][k];
             // - i (kT a_k + (kS - kT) s_k)
@@ -1178,8 +1345,8 @@ hipError_t reduce(const KrylovWork &w, int nacc, int nrb, double2 *out, hipStrea
     return hipGetLastError();
 }
 
-using KronFn = void (*)(const KronDev *, const double2 *, const double2 *, int64_t, int64_t, int, const double2 *,
-                        double2 *, double2 *, int64_t, int, int, int);
+using KronFn = void (*)(const KronDev *, const uint64_t *, const double2 *, const double2 *, int64_t, int64_t, int,
+                        const double2 *, double2 *, double2 *, int64_t, int, int, int);
 
 // Lane-slice depth of the Kronecker product: 2^Q nuclear indices per wave on
 // 64 >> Q lanes.  Default 2 where the system allows it (N = 36864, M = 256 on
@@ -1193,10 +1360,39 @@ int kron_slices(const KrylovWork &w)
     return std::min(q, w.kq);
 }
 
+// Waves per workgroup of the tiled product (k_spmm_kron_tiles; 0: the plain
+// lane slices): 8 by default (N = 36864, M = 256 on MI355X: 147 us per product
+// and 7 us for its reduce, against 158 and 9 without tiles), halved while the
+// system has fewer than Q + log2(NW) trailing spin-1/2 nuclei; SPC_KRON_TILE =
+// 0, 4, 8 or 16 overrides it (read at every launch).
+int kron_tile(const KrylovWork &w)
+{
+    const int q = kron_slices(w);
+    if (q != 2) return 0;
+    int nw = 8;
+    if (const char *e = std::getenv("SPC_KRON_TILE")) nw = std::atoi(e);
+    if (nw != 4 && nw != 8 && nw != 16) return 0;
+    for (; nw >= 4; nw /= 2)
+        if (w.kq >= q + (nw == 16 ? 4 : nw == 8 ? 3 : 2)) return nw;
+    return 0;
+}
+
+// Tiles per workgroup of the tiled product: about 4096 workgroups in all.
+inline int tile_rpg(const KrylovWork &w, int nw)
+{
+    const int64_t tiles = w.Z >> (2 + (nw == 16 ? 4 : nw == 8 ? 3 : 2));
+    const int64_t nsl = (w.Mp << 2) / kWave;
+    return (int)std::max<int64_t>(1, ceil_div(tiles * nsl, 4096 / std::max(1, nw / 4)));
+}
+
 // Row blocks of the product's partial sums (the reduce after it needs them).
 int spmm_nrb(const KrylovWork &w)
 {
     const int q = kron_slices(w);
+    if (const int nw = kron_tile(w)) {
+        const int64_t tiles = w.Z >> (q + (nw == 16 ? 4 : nw == 8 ? 3 : 2));
+        return (int)ceil_div(tiles, tile_rpg(w, nw));
+    }
     return nrb_for(w.Z >> q, w.rpw1);
 }
 
@@ -1211,19 +1407,32 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
             group = e ? std::min(4, std::max(2, std::atoi(e))) : 2; // 2: fastest at Q = 2 (160 us vs 165 for 3)
         }
         const int q = kron_slices(w);
+        if (const int nw = kron_tile(w)) {
+            const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave, rpg = tile_rpg(w, nw);
+            using TileFn = KronFn;
+            static const TileFn tfn[3][3] = {
+                {k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>, k_spmm_kron_tiles<4, 2, 4>},
+                {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>, k_spmm_kron_tiles<4, 2, 8>},
+                {k_spmm_kron_tiles<2, 2, 16>, k_spmm_kron_tiles<3, 2, 16>, k_spmm_kron_tiles<4, 2, 16>}};
+            const int ti = nw == 4 ? 0 : nw == 8 ? 1 : 2;
+            hipLaunchKernelGGL(tfn[ti][group - 2], dim3(nrb * nsl), dim3(64 * nw), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
+                               w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
+                               w.Mp, rpg, nrb);
+            return hipGetLastError();
+        }
         if (q > 0) {
             static const KronFn fns[3][3] = {
                 {k_spmm_kron_slices<2, 1>, k_spmm_kron_slices<3, 1>, k_spmm_kron_slices<4, 1>},
                 {k_spmm_kron_slices<2, 2>, k_spmm_kron_slices<3, 2>, k_spmm_kron_slices<4, 2>},
                 {k_spmm_kron_slices<2, 3>, k_spmm_kron_slices<3, 3>, k_spmm_kron_slices<4, 3>}};
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave;
-            hipLaunchKernelGGL(fns[q - 1][group - 2], dim3(nrb * nsl), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
+            hipLaunchKernelGGL(fns[q - 1][group - 2], dim3(nrb * nsl), dim3(kBlock), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
                                w.Mp, w.rpw1, nrb);
             return hipGetLastError();
         }
         auto fn = group == 2 ? k_spmm_kron<2> : group == 3 ? k_spmm_kron<3> : k_spmm_kron<4>;
-        hipLaunchKernelGGL(fn, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kdiag, w.kgen,
+        hipLaunchKernelGGL(fn, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
                            w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
                            w.Mp, w.rpw1, nrb1);
         return hipGetLastError();
@@ -1266,7 +1475,8 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
         if ((e = reduce(w, 2, nrb1, w.dots1, s)) != hipSuccess) return e;
         {
             ProfScope ps(w, PF_COEFFS, 0.0, s);
-            hipLaunchKernelGGL(k_coeffs, dim3(w.Mp / kCoefLanes), dim3(kBlock), 0, s, i, w.m, w.M, w.Mp, w.kS, w.kT, w.dots1,
+            const size_t lds = sizeof(double2) * kCoefLanes * (size_t)(w.m * (w.m + 3) / 2);
+            hipLaunchKernelGGL(k_coeffs, dim3(w.Mp / kCoefLanes), dim3(kBlock), lds, s, i, w.m, w.M, w.Mp, w.kS, w.kT, w.dots1,
                                w.dots, w.coef, w.tprev, w.hess, w.beta0, w.flags);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -151,19 +151,21 @@ def test_errors_are_reported(ctx):
     assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "trailing1", "config2_like"])
+@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "trailing1", "config2_like", "trailing6"])
 def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
     # the Kronecker-form product (diagonal blocks + 2x2 flip tables) and the
     # 4x4-block product sum the same terms in the same order: bitwise equal,
     # also when a wave takes 2^q consecutive nuclear indices on 64 / 2^q lanes
-    # (q is capped by the trailing spin-1/2 nuclei: 3 for iso and config2_like,
-    # 0 for aniso_mixed, 1 for trailing1)
+    # (q is capped by the trailing spin-1/2 nuclei: 4 for iso and config2_like,
+    # 0 for aniso_mixed, 1 for trailing1, 6 for trailing6)
     if case == "iso":
         w = workloads.config1(M=8, T_ns=20.0)
     elif case == "aniso_mixed":
         w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3), M=8, T_ns=20.0)
     elif case == "trailing1":
         w = workloads.small_anisotropic(m1=(2, 3), m2=(3, 2), M=8, T_ns=20.0)
+    elif case == "trailing6":
+        w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 2, 2, 2), M=8, T_ns=15.0)
     else:
         w = workloads.small_anisotropic(m1=(3, 3, 2, 2), m2=(2, 2), M=70, m=16, T_ns=15.0)
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
@@ -174,6 +176,7 @@ def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
     b, bw = H.spmm(X), H.spmm(Xw)
     rb = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     monkeypatch.delenv("SPC_SPMM")
+    monkeypatch.setenv("SPC_KRON_TILE", "0")
     for q in ("0", "1", "2", "3"):
         monkeypatch.setenv("SPC_KRON_Q", q)
         a = H.spmm(X)
@@ -184,4 +187,13 @@ def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
         assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, q
         if q == "0":
             assert np.array_equal(ra.P_S, rb.P_S) and np.array_equal(ra.P_Tp, rb.P_Tp)
+    # workgroups of 4, 8 or 16 waves on consecutive quads (where the system
+    # has 2 + log2(waves) trailing spin-1/2 nuclei; otherwise the plain slices)
+    monkeypatch.setenv("SPC_KRON_Q", "2")
+    for nw in ("0", "4", "8", "16"):
+        monkeypatch.setenv("SPC_KRON_TILE", nw)
+        assert np.array_equal(H.spmm(X), b), nw
+        assert np.array_equal(H.spmm(Xw), bw), nw
+        ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+        assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, nw
     assert max_rel(b, X @ H.scipy().toarray().T) < 1e-13
