@@ -859,13 +859,19 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
             return hip_fail(ctx, e, "hipGraphInstantiate");
         }
         for (int64_t k = 0; k < K && e == hipSuccess; ++k) e = hipGraphLaunch(ge, s);
+        const hipError_t er = hipEventRecord(ev1, s);
+        // the graph is destroyed only once its launches have run: destroying it
+        // with launches in flight frees state the runtime still uses
+        const hipError_t es = hipStreamSynchronize(s);
         hipGraphExecDestroy(ge);
         hipGraphDestroy(g);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+        if (er != hipSuccess) return hip_fail(ctx, er, "hipEventRecord");
+        if (es != hipSuccess) return hip_fail(ctx, es, "hipStreamSynchronize");
     } else {
         for (int64_t k = 0; k < K; ++k) HIP_TRY(one_step());
+        HIP_TRY(hipEventRecord(ev1, s));
     }
-    HIP_TRY(hipEventRecord(ev1, s));
     HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, Mp, d_acc + (size_t)o0 * 4 * nout, s));
     int32_t flags[4];
     HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
