@@ -1361,15 +1361,16 @@ int kron_slices(const KrylovWork &w)
 }
 
 // Waves per workgroup of the tiled product (k_spmm_kron_tiles; 0: the plain
-// lane slices): 8 by default (N = 36864, M = 256 on MI355X: 147 us per product
-// and 7 us for its reduce, against 158 and 9 without tiles), halved while the
-// system has fewer than Q + log2(NW) trailing spin-1/2 nuclei; SPC_KRON_TILE =
-// 0, 4, 8 or 16 overrides it (read at every launch).
+// lane slices): 8 by default, 16 from 24576 nuclear indices on (MI355X: 147 us
+// per product at N = 36864, M = 256 against 162 with 16 and 158 without tiles;
+// 423 us at N = 147456, M = 128 against 447 with 8), halved while the system
+// has fewer than Q + log2(NW) trailing spin-1/2 nuclei; SPC_KRON_TILE = 0, 4,
+// 8 or 16 overrides it (read at every launch).
 int kron_tile(const KrylovWork &w)
 {
     const int q = kron_slices(w);
-    if (q != 2) return 0;
-    int nw = 8;
+    if (q != 2 && q != 3) return 0;
+    int nw = w.Z >= 24576 ? 16 : 8;
     if (const char *e = std::getenv("SPC_KRON_TILE")) nw = std::atoi(e);
     if (nw != 4 && nw != 8 && nw != 16) return 0;
     for (; nw >= 4; nw /= 2)
@@ -1380,8 +1381,9 @@ int kron_tile(const KrylovWork &w)
 // Tiles per workgroup of the tiled product: about 4096 workgroups in all.
 inline int tile_rpg(const KrylovWork &w, int nw)
 {
-    const int64_t tiles = w.Z >> (2 + (nw == 16 ? 4 : nw == 8 ? 3 : 2));
-    const int64_t nsl = (w.Mp << 2) / kWave;
+    const int q = kron_slices(w);
+    const int64_t tiles = w.Z >> (q + (nw == 16 ? 4 : nw == 8 ? 3 : 2));
+    const int64_t nsl = (w.Mp << q) / kWave;
     return (int)std::max<int64_t>(1, ceil_div(tiles * nsl, 4096 / std::max(1, nw / 4)));
 }
 
@@ -1410,12 +1412,16 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
         if (const int nw = kron_tile(w)) {
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave, rpg = tile_rpg(w, nw);
             using TileFn = KronFn;
-            static const TileFn tfn[3][3] = {
-                {k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>, k_spmm_kron_tiles<4, 2, 4>},
-                {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>, k_spmm_kron_tiles<4, 2, 8>},
-                {k_spmm_kron_tiles<2, 2, 16>, k_spmm_kron_tiles<3, 2, 16>, k_spmm_kron_tiles<4, 2, 16>}};
+            static const TileFn tfn[2][3][2] = {
+                {{k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>},
+                 {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>},
+                 {k_spmm_kron_tiles<2, 2, 16>, k_spmm_kron_tiles<3, 2, 16>}},
+                {{k_spmm_kron_tiles<2, 3, 4>, k_spmm_kron_tiles<3, 3, 4>},
+                 {k_spmm_kron_tiles<2, 3, 8>, k_spmm_kron_tiles<3, 3, 8>},
+                 {k_spmm_kron_tiles<2, 3, 16>, k_spmm_kron_tiles<3, 3, 16>}}};
             const int ti = nw == 4 ? 0 : nw == 8 ? 1 : 2;
-            hipLaunchKernelGGL(tfn[ti][group - 2], dim3(nrb * nsl), dim3(64 * nw), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
+            hipLaunchKernelGGL(tfn[q - 2][ti][std::min(group, 3) - 2], dim3(nrb * nsl), dim3(64 * nw), 0, s, w.kron,
+                               w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
                                w.Mp, rpg, nrb);
             return hipGetLastError();
