@@ -239,6 +239,15 @@ bool kron_disabled()
     return e && std::string(e) == "blocks";
 }
 
+// Nuclear indices per workgroup of the update pass: about 512 workgroups
+// (two per CU); SPC_UPDATE_BLOCKS sets another count (tuning).
+int update_rows(int64_t Z, int64_t nch)
+{
+    int64_t target = 512;
+    if (const char *e = std::getenv("SPC_UPDATE_BLOCKS")) target = std::max(1, std::atoi(e));
+    return (int)std::max<int64_t>(1, (Z * nch + target - 1) / target);
+}
+
 void set_ham(KrylovWork &w, const HamDev &h)
 {
     w.rowptr = h.rowptr;
@@ -401,7 +410,7 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     // L2), the streaming passes in about 512 blocks per pass
     const int nch = Mp / 64;
     w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
-    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpw2 = update_rows(Z, nch);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     const bool need_gram = sim->tol > 0.0;
 
@@ -752,7 +761,7 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     w.kT = ctx->kT;
     const int nch = Mp / 64;
     w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
-    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpw2 = update_rows(Z, nch);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     double2 *vals = nullptr;
     double *d_fields = nullptr;
@@ -1640,7 +1649,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     w.kT = ctx->kT;
     const int nch = Mp / 64;
     w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
-    w.rpw2 = (int)std::max<int64_t>(1, (Z * nch + 512 - 1) / 512);
+    w.rpw2 = update_rows(Z, nch);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     w.qs = basis_stride(N, Mp);
     if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;

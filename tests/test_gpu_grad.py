@@ -193,3 +193,16 @@ def test_torch_function_chain_rule(ctx):
         fd = (value(h6, **{k: vals[k] + step}) - value(h6, **{k: vals[k] - step})) / (2 * step)
         assert abs(scal[k].grad.item() - fd) <= 1e-6 * max(abs(fd), 1e-3), k
     assert math.isclose(F.item(), value(h6), rel_tol=1e-13)
+
+
+def test_gradient_independent_of_the_product_kernel(ctx, monkeypatch):
+    # five trailing spin-1/2 nuclei: tiled product by default; the 4x4-block
+    # product sums the same terms in the same order
+    w = workloads.small_anisotropic(m1=(3,), m2=(2, 2, 2, 2, 2), M=8, m=10, T_ns=12.0)
+    g = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)
+    monkeypatch.setenv("SPC_SPMM", "blocks")
+    gb = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)
+    s = np.max(np.abs(gb.hfc))
+    assert np.max(np.abs(g.hfc - gb.hfc)) <= 1e-11 * s
+    assert np.max(np.abs(g.field - gb.field)) <= 1e-11 * s
+    assert abs(g.Phi_S - gb.Phi_S) <= 1e-12 * abs(gb.Phi_S)

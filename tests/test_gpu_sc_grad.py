@@ -117,3 +117,15 @@ def test_torch_function_chain_rule(ctx):
         dn[k] -= h
         fd = (prob.value(**up) - prob.value(**dn)) / (2 * h)
         assert abs(t[k].grad.item() - fd) <= 1e-6 * max(abs(fd), 1e-3), k
+
+
+@pytest.mark.parametrize("lanes", ["2", "4", "8"])
+def test_every_lane_split_gives_the_same_derivative(ctx, monkeypatch, lanes):
+    monkeypatch.setenv("SPC_SC_LANES", lanes)
+    a1, a2 = [0.8, -0.3, 0.5, 0.2, -0.9], [0.6, 0.1, -0.4]
+    kw = dict(J=0.03, kS=0.1, kT=0.2, dt=0.05, n_samples=700, n_steps_=60, seed=(5, 6))
+    d = _unit([0.2, 0.5, 0.8])
+    args = (a1, a2, [2, 3, 2, 2, 3], [2, 2, 3])
+    (_, _, _, _), (dps, _, _, _) = sc_dynamics.run_tangent(*args, B=0.3, dfield=d, direction=tuple(d), ctx=ctx, **kw)
+    fps, _, _, _ = _fd(_device_fwd(args, ctx, **kw), 0.3 * d, d)
+    assert np.max(np.abs(dps - fps)) <= 1e-6 * np.max(np.abs(fps))
