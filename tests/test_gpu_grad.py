@@ -206,3 +206,21 @@ def test_gradient_independent_of_the_product_kernel(ctx, monkeypatch):
     assert np.max(np.abs(g.hfc - gb.hfc)) <= 1e-11 * s
     assert np.max(np.abs(g.field - gb.field)) <= 1e-11 * s
     assert abs(g.Phi_S - gb.Phi_S) <= 1e-12 * abs(gb.Phi_S)
+
+
+def test_gradient_on_the_full_size_system_matches_fd(ctx):
+    # the N = 36864 system of the gradient workload (2x14N + 6x1H | 4x1H,
+    # anisotropic, tilted field), 8 trace vectors over 20 steps
+    w = workloads.config2(M=8, m=24, T_ns=20.0)
+    g = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)
+    picks = [("hfc", (0, 0, 0)), ("hfc", (0, 1, 2)), ("hfc", (9, 2, 2)), ("field", 0), ("field", 2), ("J", None),
+             ("kS", None), ("kT", None)]
+    adj, fd = [], []
+    for name, idx in picks:
+        h = 1e-6 * getattr(w.sys, name) if name in ("kS", "kT") else 1e-6
+        sp, Bp = _perturbed(w.sys, w.B, name, idx, h)
+        sm, Bm = _perturbed(w.sys, w.B, name, idx, -h)
+        fd.append((_forward_yield(sp, w.sim, Bp, ctx) - _forward_yield(sm, w.sim, Bm, ctx)) / (2 * h))
+        adj.append(_adj_value(g, name, idx))
+    adj, fd = np.array(adj), np.array(fd)
+    assert np.max(np.abs(adj - fd)) <= 1e-6 * np.max(np.abs(fd)), (adj, fd)
