@@ -159,19 +159,23 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
  * n_orient directions dirs [n][3], with the same M trace vectors (sample_seeds
  * [M][2], as spc_trace_sample_seeds) for every orientation; tol must be 0 and
  * M a multiple of 64.  Orientations share one block pattern of H and go into
- * the lane dimension, orient_per_chunk at a time (0: as many as fit); with
- * use_graph the step is captured once in a hipGraph and replayed.  Outputs:
+ * the lane dimension, orient_per_chunk at a time (0: as many as fit); up to
+ * 4 chunks run at once on their own streams when memory allows
+ * (SPC_LANE_STREAMS caps them); with use_graph each chunk's step is captured
+ * once in a hipGraph and replayed.  Outputs:
  * Phi_S [n_orient] final singlet yields; P_S [n_orient][n_out] or NULL. */
 int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
                           const double *dirs, int32_t orient_per_chunk, int32_t use_graph, double *Phi_S, double *P_S,
                           int64_t n_out);
 
 /* Autotuner for spc_orientation_batch (no counterpart in the reference):
- * the smallest Krylov order (down from sim->n_krylov in steps of 2) whose
+ * the smallest Krylov order (down from sim->n_krylov in steps of 2, then the
+ * odd order between the last that passed and the first that failed) whose
  * pilot yields (64 steps, up to 4 orientations) stay within yield_tol of the
  * requested order's, scaled to the pilot's share of the run, and the
  * orientations per chunk (power of two) with the least device time per
- * orientation-step over a short timed run. */
+ * orientation-step over a short timed run, with as many chunks in flight as
+ * spc_orientation_batch runs. */
 int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
                    const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
                    double *ms_per_orient_step);
@@ -212,7 +216,8 @@ int spc_semiclassical_tangent(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_
                               double *dPhi_T);
 
 /* Last-run timing of the propagation loop: milliseconds of device time and
- * number of Krylov rebuilds, for benchmarks. */
+ * number of Krylov rebuilds summed over the lane chunks (with tol = 0, the
+ * steps times the chunks run), for benchmarks. */
 int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds, int64_t *steps);
 
 /* Kernel-family event profiler (no reference counterpart; benchmarks only).

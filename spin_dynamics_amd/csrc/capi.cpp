@@ -69,6 +69,7 @@ struct HamDev {
 struct spc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t lane_stream[3] = {}; // further lane chunks of propagate (created on first use)
     std::string err;
     size_t lds_limit = 65536;
     // system
@@ -239,11 +240,12 @@ bool kron_disabled()
     return e && std::string(e) == "blocks";
 }
 
-// Nuclear indices per workgroup of the update pass: about 512 workgroups
-// (two per CU); SPC_UPDATE_BLOCKS sets another count (tuning).
-int update_rows(int64_t Z, int64_t nch)
+// Nuclear indices per workgroup of the update pass: about `target`
+// workgroups (512, two per CU, for a pass that has the device to itself; 256
+// per pass when several lane chunks run at once); SPC_UPDATE_BLOCKS sets
+// another count (tuning).
+int update_rows(int64_t Z, int64_t nch, int64_t target = 512)
 {
-    int64_t target = 512;
     if (const char *e = std::getenv("SPC_UPDATE_BLOCKS")) target = std::max(1, std::atoi(e));
     return (int)std::max<int64_t>(1, (Z * nch + target - 1) / target);
 }
@@ -282,12 +284,13 @@ KronDev kron_of(const DevSystem &S)
 // mismatch with the block form the tables are dropped (the block product is
 // used).  diag/gen: nsets * Z * 16 and nsets * nn * 24 values.
 int make_kron(spc_ctx *ctx, const KronDev &hk, KronDev *dk, const HamDev &h, const double2 *bval, int nsets,
-              double2 *diag, double2 *gen, bool &ok)
+              double2 *diag, double2 *gen, bool &ok, hipStream_t s = nullptr,
+              const std::string &bad_name = "kron_bad")
 {
     int32_t *bad = nullptr;
-    int rc = ensure(ctx, "kron_bad", (size_t)1, &bad);
+    int rc = ensure(ctx, bad_name.c_str(), (size_t)1, &bad);
     if (rc) return rc;
-    hipStream_t s = ctx->stream;
+    if (!s) s = ctx->stream;
     HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
     HIP_TRY(kron_tables(dk, hk, h.browptr, h.bcol, bval, h.nblocks, nsets, ctx->Z ? ctx->Z : 1, diag, gen, bad, s));
     int32_t b = 0;
@@ -364,8 +367,180 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     return SPC_OK;
 }
 
-// Propagates M_total trace vectors (chunks of at most max_lanes) and writes
-// the sample-averaged populations; vi.kind selects how vectors are made.
+// Workspace, stream and host staging of one lane chunk of propagate().
+struct LaneSet {
+    KrylovWork w{};
+    hipStream_t s = nullptr;
+    int64_t *d_seeds = nullptr;
+    double2 *d_inj = nullptr;
+    VecInit v{};
+    int64_t start = 0;
+    std::vector<double> obs0, lt;
+};
+
+// Device buffers of lane set L; set 0 uses the plain names (shared with the
+// other entry points), set 1 the same names with a suffix.
+int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, int64_t n_out, bool need_gram,
+                   int vkind)
+{
+    KrylovWork &w = L.w;
+    const int64_t N = ctx->N, Z = ctx->Z;
+    w.N = N;
+    w.Z = Z;
+    w.Mp = Mp;
+    w.m = m;
+    w.n_out = n_out;
+    set_ham(w, ctx->ham);
+    w.prof = &ctx->prof;
+    w.kS = ctx->kS;
+    w.kT = ctx->kT;
+    // launch geometry: the sparse product in small row blocks (gathers stay in
+    // L2), the streaming passes in about 512 blocks per pass
+    const int nch = Mp / 64;
+    w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
+    w.rpw2 = update_rows(Z, nch, nsets > 1 ? 256 : 512);
+    w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
+    w.qs = basis_stride(N, Mp);
+    const std::string sfx = idx ? "#" + std::to_string(idx) : "";
+    auto nm = [&](const char *b) { return std::string(b) + sfx; };
+    int rc;
+    if ((rc = ensure(ctx, nm("Q").c_str(), (size_t)m * w.qs, &w.Q))) return rc;
+    if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
+    w.u = w.psi;
+    if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
+    size_t part_need = partials_needed(w);
+    if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
+    if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
+    w.part_capacity = part_need;
+    if ((rc = ensure(ctx, nm("dots").c_str(), (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots)))
+        return rc;
+    if ((rc = ensure(ctx, nm("dots1").c_str(), (size_t)2 * Mp, &w.dots1))) return rc;
+    if ((rc = ensure(ctx, nm("coef").c_str(), (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, nm("tprev").c_str(), (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, nm("hess").c_str(), (size_t)m * m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, nm("E").c_str(), (size_t)m * m * Mp, &w.E))) return rc;
+    if ((rc = ensure(ctx, nm("c").c_str(), (size_t)m * Mp, &w.c))) return rc;
+    if (need_gram && (rc = ensure(ctx, nm("gram").c_str(), (size_t)4 * m * m * Mp, &w.gram))) return rc;
+    if ((rc = ensure(ctx, nm("beta0").c_str(), (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, nm("lane_t").c_str(), (size_t)Mp, &w.lane_t))) return rc;
+    if ((rc = ensure(ctx, nm("lane_k").c_str(), (size_t)Mp, &w.lane_k))) return rc;
+    if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
+    if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)n_out * 4 * Mp, &w.obs))) return rc;
+    if (vkind == 0 || vkind == 1)
+        if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &L.d_seeds))) return rc;
+    if (vkind == 2)
+        if ((rc = ensure(ctx, nm("inj").c_str(), (size_t)Mp * Z, &L.d_inj))) return rc;
+    L.obs0.assign((size_t)4 * Mp, 0.0);
+    L.lt.assign((size_t)Mp, 0.0);
+    HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, L.s));
+    return SPC_OK;
+}
+
+// Uploads the seeds or injected vectors of chunk [start, start + M) to L.
+int load_chunk(spc_ctx *ctx, LaneSet &L, const VecInit &vi, int64_t start, int M, const std::vector<int64_t> &seeds,
+               const double *injected)
+{
+    L.start = start;
+    L.w.M = M;
+    L.v = vi;
+    if (vi.kind == 0 || vi.kind == 1) {
+        HIP_TRY(hipMemcpyAsync(L.d_seeds, seeds.data() + 2 * start, sizeof(int64_t) * 2 * M, hipMemcpyHostToDevice,
+                               L.s));
+        L.v.seeds = L.d_seeds;
+    } else if (vi.kind == 2) {
+        HIP_TRY(hipMemcpyAsync(L.d_inj, injected + 2 * start * ctx->Z, sizeof(double2) * (size_t)M * ctx->Z,
+                               hipMemcpyHostToDevice, L.s));
+        L.v.inj = L.d_inj;
+    } else {
+        L.v.basis_offset = start;
+    }
+    return SPC_OK;
+}
+
+// Initial vectors, lane clocks and populations of L's chunk.
+int start_pass(spc_ctx *ctx, LaneSet &L, const double pop[4])
+{
+    KrylovWork &w = L.w;
+    const int Mp = w.Mp, M = w.M;
+    HIP_TRY(init_vectors(L.v, M, Mp, w.Z, w.psi, L.s));
+    HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, L.s));
+    HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, L.s));
+    HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, L.s));
+    HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * w.n_out * 4 * Mp, L.s));
+    std::fill(L.obs0.begin(), L.obs0.end(), 0.0);
+    for (int x = 0; x < 4; ++x)
+        for (int r = 0; r < M; ++r) L.obs0[(size_t)x * Mp + r] = pop[x];
+    HIP_TRY(hipMemcpyAsync(w.obs, L.obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, L.s));
+    return SPC_OK;
+}
+
+// One Krylov rebuild of L's lanes: basis, exponential, steps, psi.
+int rebuild(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, bool gram)
+{
+    KrylovWork &w = L.w;
+    HIP_TRY(arnoldi(w, L.s));
+    HIP_TRY(expm_lanes(w, sim->dt, L.s));
+    if (gram) HIP_TRY(gram_projectors(w, L.s));
+    HIP_TRY(step_lanes(w, sim->dt, sim->t_end, sim->tol, gram, L.s));
+    HIP_TRY(psi_update(w, !gram, L.s));
+    return SPC_OK;
+}
+
+// The projector-Gram pass of L's chunk (several steps per basis), run to the
+// end with a host check of the lane clocks after every rebuild.
+int gram_pass(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, const double pop[4], int64_t K, int64_t &rebuilds)
+{
+    int rc = start_pass(ctx, L, pop);
+    if (rc) return rc;
+    for (int64_t round = 0;; ++round) {
+        if ((rc = rebuild(ctx, L, sim, true))) return rc;
+        ++rebuilds;
+        HIP_TRY(hipMemcpyAsync(L.lt.data(), L.w.lane_t, sizeof(double) * L.w.Mp, hipMemcpyDeviceToHost, L.s));
+        HIP_TRY(hipStreamSynchronize(L.s));
+        bool done = true;
+        for (int r = 0; r < L.w.M; ++r) done = done && !(L.lt[r] < sim->t_end);
+        if (done) break;
+        if (round > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
+    }
+    return SPC_OK;
+}
+
+// Host copy of L's flags after its stream has drained.
+int read_flags(spc_ctx *ctx, LaneSet &L, int32_t flags[4])
+{
+    HIP_TRY(hipMemcpyAsync(flags, L.w.flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, L.s));
+    HIP_TRY(hipStreamSynchronize(L.s));
+    if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
+    return SPC_OK;
+}
+
+// Lane chunks that run at once on their own streams (at most one per 64
+// lanes): 4 by default, one per hardware queue of the process (N = 36864,
+// M = 256 on MI355X: 14.7 ms per step with 4 chunks, 15.5 with 2, 16.9 with
+// one); SPC_LANE_STREAMS sets 1 (one chunk at a time) to kMaxLaneSets.
+constexpr int kMaxLaneSets = 4;
+int lane_streams()
+{
+    const char *e = std::getenv("SPC_LANE_STREAMS");
+    return e ? std::clamp(std::atoi(e), 1, kMaxLaneSets) : kMaxLaneSets;
+}
+
+// SPC_LANE_SERIAL=1 keeps the lane chunks but runs them on one stream, one
+// kernel at a time (per-kernel timing under a profiler).
+bool lane_serial()
+{
+    const char *e = std::getenv("SPC_LANE_SERIAL");
+    return e && std::atoi(e) == 1;
+}
+
+// Propagates M_total trace vectors and writes the sample-averaged
+// populations; vi.kind selects how vectors are made.  With tol = 0 the lanes
+// are split into chunks (four by default) that run concurrently on their own
+// streams, so that the short per-rebuild kernels of one chunk (Hessenberg
+// reductions, exponentials, steps) overlap the streaming passes of another.
+// Chunk populations are added to the sums in sample order, as a single chunk
+// would add them.  With the event profiler on, the same chunks run on one
+// stream, so that every kernel's events bracket it alone.
 int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_total,
               const std::vector<int64_t> &seeds, const double *injected, double *P_S, double *P_T0, double *P_Tp,
               double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T)
@@ -382,9 +557,10 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     const int64_t K = steps_taken(sim->t_end, sim->dt);
     if (K + 1 > n_out) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
     const int m = sim->n_krylov;
-    const int64_t N = ctx->N, Z = ctx->Z;
+    const int64_t N = ctx->N;
+    const bool need_gram = sim->tol > 0.0;
 
-    // lanes per chunk: basis + 3 vectors + populations, within half the free memory
+    // lanes in flight: basis + 3 vectors + populations, within half the free memory
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     const size_t per_lane = ((size_t)m + 3) * N * sizeof(double2) + (size_t)n_out * 4 * sizeof(double) +
@@ -392,124 +568,82 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     int64_t max_lanes = (int64_t)std::min<size_t>(1024, (free_b / 2) / std::max<size_t>(per_lane, 1));
     max_lanes = (max_lanes / 64) * 64;
     if (max_lanes < 64) return fail(ctx, SPC_ERR_OOM, "not enough device memory for one 64-vector chunk");
-    const int64_t Mchunk = std::min<int64_t>(M_total, max_lanes);
+    int nsets = 1;
+    if (!need_gram) nsets = (int)std::min<int64_t>({lane_streams(), (M_total + 63) / 64, max_lanes / 64});
+    const int64_t Mchunk =
+        std::min<int64_t>(max_lanes / (64 * nsets) * 64, (M_total + 64 * nsets - 1) / (64 * nsets) * 64);
     const int Mp = (int)((Mchunk + 63) / 64 * 64);
 
-    KrylovWork w{};
-    w.N = N;
-    w.Z = Z;
-    w.Mp = Mp;
-    w.m = m;
-    w.n_out = n_out;
-    set_ham(w, ctx->ham);
-    w.prof = &ctx->prof;
+    hipStream_t s = ctx->stream;
+    const bool serial = ctx->prof.on || lane_serial();
+    LaneSet sets[kMaxLaneSets];
+    sets[0].s = s;
+    for (int j = 1; j < nsets; ++j) {
+        if (!serial && !ctx->lane_stream[j - 1]) HIP_TRY(hipStreamCreate(&ctx->lane_stream[j - 1]));
+        sets[j].s = serial ? s : ctx->lane_stream[j - 1];
+    }
     ctx->prof.clear();
-    w.kS = ctx->kS;
-    w.kT = ctx->kT;
-    // launch geometry: the sparse product in small row blocks (gathers stay in
-    // L2), the streaming passes in about 512 blocks per pass
-    const int nch = Mp / 64;
-    w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
-    w.rpw2 = update_rows(Z, nch);
-    w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
-    const bool need_gram = sim->tol > 0.0;
-
-    w.qs = basis_stride(N, Mp);
-    if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;
-    if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
-    w.u = w.psi;
-    if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
-    size_t part_need = partials_needed(w);
-    if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
-    if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
-    w.part_capacity = part_need;
-    if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
-    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
-    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
-    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
-    if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
-    if ((rc = ensure(ctx, "E", (size_t)m * m * Mp, &w.E))) return rc;
-    if ((rc = ensure(ctx, "c", (size_t)m * Mp, &w.c))) return rc;
-    if (need_gram && (rc = ensure(ctx, "gram", (size_t)4 * m * m * Mp, &w.gram))) return rc;
-    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
-    if ((rc = ensure(ctx, "lane_t", (size_t)Mp, &w.lane_t))) return rc;
-    if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
-    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
-    if ((rc = ensure(ctx, "obs", (size_t)n_out * 4 * Mp, &w.obs))) return rc;
     double *d_acc = nullptr;
     if ((rc = ensure(ctx, "acc", (size_t)4 * n_out, &d_acc))) return rc;
-    int64_t *d_seeds = nullptr;
-    double2 *d_inj = nullptr;
-    if (vi.kind == 0 || vi.kind == 1)
-        if ((rc = ensure(ctx, "seeds", (size_t)2 * Mp, &d_seeds))) return rc;
-    if (vi.kind == 2)
-        if ((rc = ensure(ctx, "inj", (size_t)Mp * Z, &d_inj))) return rc;
-
-    hipStream_t s = ctx->stream;
     HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * n_out, s));
-    HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
+    hipEvent_t ev0, ev1, evr;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ev0, s));
+    for (int j = 1; j < nsets; ++j)
+        if (sets[j].s != s) HIP_TRY(hipStreamWaitEvent(sets[j].s, ev0, 0));
+    for (int j = 0; j < nsets; ++j)
+        if ((rc = alloc_lane_set(ctx, sets[j], j, nsets, Mp, m, n_out, need_gram, vi.kind))) return rc;
 
     double pop[4];
     theta_of(sim->init_state, vi.theta, pop);
-    std::vector<double> obs0((size_t)4 * Mp, 0.0);
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
-    HIP_TRY(hipEventRecord(ev0, s));
     int64_t rebuilds = 0;
     bool breakdown = false;
 
-    for (int64_t start = 0; start < M_total; start += Mchunk) {
-        const int M = (int)std::min<int64_t>(Mchunk, M_total - start);
-        w.M = M;
-        VecInit v = vi;
-        if (vi.kind == 0 || vi.kind == 1) {
-            HIP_TRY(hipMemcpyAsync(d_seeds, seeds.data() + 2 * start, sizeof(int64_t) * 2 * M,
-                                   hipMemcpyHostToDevice, s));
-            v.seeds = d_seeds;
-        } else if (vi.kind == 2) {
-            HIP_TRY(hipMemcpyAsync(d_inj, injected + 2 * start * Z, sizeof(double2) * (size_t)M * Z,
-                                   hipMemcpyHostToDevice, s));
-            v.inj = d_inj;
-        } else {
-            v.basis_offset = start;
+    for (int64_t start = 0; start < M_total; start += nsets * Mchunk) {
+        int used = 0;
+        for (int j = 0; j < nsets && start + j * Mchunk < M_total; ++j, ++used) {
+            const int64_t cs = start + j * Mchunk;
+            if ((rc = load_chunk(ctx, sets[j], vi, cs, (int)std::min<int64_t>(Mchunk, M_total - cs), seeds,
+                                 injected)))
+                return rc;
         }
-        for (bool gram : {need_gram, true}) {
-            HIP_TRY(init_vectors(v, M, Mp, Z, w.psi, s));
-            HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, s));
-            HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
-            HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
-            HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * n_out * 4 * Mp, s));
-            for (int x = 0; x < 4; ++x)
-                for (int r = 0; r < M; ++r) obs0[(size_t)x * Mp + r] = pop[x];
-            HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
-            std::vector<double> lt(Mp);
-            for (int64_t round = 0;; ++round) {
-                if (!gram && round >= K) break;
-                HIP_TRY(arnoldi(w, s));
-                HIP_TRY(expm_lanes(w, sim->dt, s));
-                if (gram) HIP_TRY(gram_projectors(w, s));
-                HIP_TRY(step_lanes(w, sim->dt, sim->t_end, sim->tol, gram, s));
-                HIP_TRY(psi_update(w, !gram, s));
-                ++rebuilds;
-                if (gram) {
-                    HIP_TRY(hipMemcpyAsync(lt.data(), w.lane_t, sizeof(double) * Mp, hipMemcpyDeviceToHost, s));
-                    HIP_TRY(hipStreamSynchronize(s));
-                    bool done = true;
-                    for (int r = 0; r < M; ++r) done = done && !(lt[r] < sim->t_end);
-                    if (done) break;
-                    if (round > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
-                }
+        if (need_gram) {
+            for (int j = 0; j < used; ++j) {
+                if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
+                int32_t flags[4];
+                if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                breakdown = breakdown || flags[0];
             }
-            int32_t flags[4];
-            HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-            if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
-            breakdown = breakdown || flags[0];
-            if (!gram && flags[1]) continue; // a lane needed two steps per rebuild: redo with projector Grams
-            break;
+        } else {
+            // direct mode: one step per rebuild, the chunks' rebuilds interleaved
+            for (int j = 0; j < used; ++j)
+                if ((rc = start_pass(ctx, sets[j], pop))) return rc;
+            for (int64_t round = 0; round < K; ++round)
+                for (int j = 0; j < used; ++j) {
+                    if ((rc = rebuild(ctx, sets[j], sim, false))) return rc;
+                    ++rebuilds;
+                }
+            for (int j = 0; j < used; ++j) {
+                int32_t flags[4];
+                if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                breakdown = breakdown || flags[0];
+                if (!flags[1]) continue;
+                // a lane needed two steps per rebuild: redo the chunk with projector Grams
+                if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
+                if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                breakdown = breakdown || flags[0];
+            }
         }
-        HIP_TRY(obs_reduce_lanes(w.obs, n_out, M, Mp, d_acc, s));
+        // every chunk has drained (read_flags); add them to the sums in order
+        for (int j = 0; j < used; ++j)
+            HIP_TRY(obs_reduce_lanes(sets[j].w.obs, n_out, sets[j].w.M, Mp, d_acc, s));
+        if (nsets > 1) {
+            HIP_TRY(hipEventRecord(evr, s));
+            for (int j = 1; j < nsets; ++j)
+                if (sets[j].s != s) HIP_TRY(hipStreamWaitEvent(sets[j].s, evr, 0));
+        }
     }
     HIP_TRY(hipEventRecord(ev1, s));
     std::vector<double> acc((size_t)4 * n_out);
@@ -519,6 +653,7 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
+    hipEventDestroy(evr);
     ctx->last_ms = ms;
     ctx->last_rebuilds = rebuilds;
     ctx->last_steps = K;
@@ -731,14 +866,32 @@ struct BatchPlan {
     int opc = 0;       // orientations per chunk
     int m = 0;         // Krylov order
     bool graph = true; // replay one captured step
+    int slots = 1;     // chunks in flight at once, each on its own stream
 };
 
-// Runs orientations [o0, o0 + n) of dirs in one lane batch (Mo = M lanes per
-// orientation, M a multiple of 64) for K steps; adds each orientation's summed
-// populations to acc[o][4 * n_out].
-int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const int64_t *sample_seeds, int M,
-                          const double *dirs, int o0, int n, bool graph, int64_t K, int64_t nout, double *d_acc,
-                          double *out_ms)
+// One chunk of orientations in flight: its workspace, stream, step graph and
+// host staging (kept until the chunk has finished).
+struct OrientSlot {
+    KrylovWork w{};
+    hipStream_t s = nullptr;
+    int idx = 0;
+    bool busy = false;
+    int o0 = 0, n = 0;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    std::vector<double> f, obs0;
+    std::vector<int64_t> sd;
+};
+
+// Enqueues orientations [o0, o0 + n) of dirs as one lane batch (M lanes per
+// orientation, M a multiple of 64) for K steps on the slot's stream: field
+// values and Kronecker tables of each orientation's H, the trace vectors,
+// then the steps (one captured step replayed K times when graph is set)
+// between the events a and b, then each orientation's summed populations
+// added to acc[o][4 * n_out].  Returns with the steps in flight.
+int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, int m,
+                             const int64_t *sample_seeds, int M, const double *dirs, int o0, int n, bool graph,
+                             int64_t K, int64_t nout, double *d_acc, hipEvent_t a, hipEvent_t b)
 {
     const int64_t N = ctx->N, Z = ctx->Z;
     const int Mp = n * M;
@@ -746,7 +899,8 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     const double Bmag = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
     const int64_t nb = ctx->fpat.nblocks;
     int rc;
-    KrylovWork w{};
+    KrylovWork &w = sl.w;
+    w = KrylovWork{};
     w.N = N;
     w.Z = Z;
     w.M = Mp;
@@ -767,47 +921,51 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     double *d_fields = nullptr;
     int64_t *d_seeds = nullptr;
     w.qs = basis_stride(N, Mp);
-    if ((rc = ensure(ctx, "Q", (size_t)m * w.qs, &w.Q))) return rc;
-    if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
+    const std::string sfx = sl.idx ? "#" + std::to_string(sl.idx) : "";
+    auto nm = [&](const char *base) { return std::string(base) + sfx; };
+    if ((rc = ensure(ctx, nm("Q").c_str(), (size_t)m * w.qs, &w.Q))) return rc;
+    if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
-    if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
+    if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
     const size_t part_need = partials_needed(w);
-    if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
+    if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
-    if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
-    if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
-    if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
-    if ((rc = ensure(ctx, "tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
-    if ((rc = ensure(ctx, "hess", (size_t)m * m * Mp, &w.hess))) return rc;
-    if ((rc = ensure(ctx, "E", (size_t)m * m * Mp, &w.E))) return rc;
-    if ((rc = ensure(ctx, "c", (size_t)m * Mp, &w.c))) return rc;
-    if ((rc = ensure(ctx, "beta0", (size_t)Mp, &w.beta0))) return rc;
-    if ((rc = ensure(ctx, "lane_t", (size_t)Mp, &w.lane_t))) return rc;
-    if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
-    if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
-    if ((rc = ensure(ctx, "obs", (size_t)nout * 4 * Mp, &w.obs))) return rc;
-    if ((rc = ensure(ctx, "bvals", (size_t)n * nb * 16, &vals))) return rc;
-    if ((rc = ensure(ctx, "bfields", (size_t)3 * n, &d_fields))) return rc;
-    if ((rc = ensure(ctx, "seeds", (size_t)2 * Mp, &d_seeds))) return rc;
+    if ((rc = ensure(ctx, nm("dots").c_str(), (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots)))
+        return rc;
+    if ((rc = ensure(ctx, nm("dots1").c_str(), (size_t)2 * Mp, &w.dots1))) return rc;
+    if ((rc = ensure(ctx, nm("coef").c_str(), (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
+    if ((rc = ensure(ctx, nm("tprev").c_str(), (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &w.tprev))) return rc;
+    if ((rc = ensure(ctx, nm("hess").c_str(), (size_t)m * m * Mp, &w.hess))) return rc;
+    if ((rc = ensure(ctx, nm("E").c_str(), (size_t)m * m * Mp, &w.E))) return rc;
+    if ((rc = ensure(ctx, nm("c").c_str(), (size_t)m * Mp, &w.c))) return rc;
+    if ((rc = ensure(ctx, nm("beta0").c_str(), (size_t)Mp, &w.beta0))) return rc;
+    if ((rc = ensure(ctx, nm("lane_t").c_str(), (size_t)Mp, &w.lane_t))) return rc;
+    if ((rc = ensure(ctx, nm("lane_k").c_str(), (size_t)Mp, &w.lane_k))) return rc;
+    if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
+    if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
+    if ((rc = ensure(ctx, nm("bvals").c_str(), (size_t)n * nb * 16, &vals))) return rc;
+    if ((rc = ensure(ctx, nm("bfields").c_str(), (size_t)3 * n, &d_fields))) return rc;
+    if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &d_seeds))) return rc;
     w.bval = vals;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = sl.s;
     // field vectors B n_o, values of each orientation's H on the pattern
-    std::vector<double> f(3 * n);
+    sl.f.assign((size_t)3 * n, 0.0);
     for (int o = 0; o < n; ++o) {
         const double *d = dirs + 3 * (o0 + o);
         const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         if (!(nd > 0.0)) return fail(ctx, SPC_ERR_INVALID, "orientation direction must be nonzero");
-        for (int a = 0; a < 3; ++a) f[3 * o + a] = Bmag * d[a] / nd;
+        for (int x = 0; x < 3; ++x) sl.f[3 * o + x] = Bmag * d[x] / nd;
     }
-    HIP_TRY(hipMemcpyAsync(d_fields, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_fields, sl.f.data(), sizeof(double) * sl.f.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], d_fields, nb * 16, n, vals, s));
     w.kron = nullptr;
     if (ctx->fpat.kron && !kron_disabled()) { // per-orientation Kronecker tables
         double2 *kd = nullptr, *kg = nullptr;
-        if ((rc = ensure(ctx, "bkdiag", (size_t)n * Z * 16, &kd))) return rc;
-        if ((rc = ensure(ctx, "bkgen", (size_t)n * std::max(hs.nn, 1) * 24, &kg))) return rc;
+        if ((rc = ensure(ctx, nm("bkdiag").c_str(), (size_t)n * Z * 16, &kd))) return rc;
+        if ((rc = ensure(ctx, nm("bkgen").c_str(), (size_t)n * std::max(hs.nn, 1) * 24, &kg))) return rc;
         bool ok = false;
-        if ((rc = make_kron(ctx, kron_of(hs), ctx->fpat.kron, ctx->fpat, vals, n, kd, kg, ok))) return rc;
+        if ((rc = make_kron(ctx, kron_of(hs), ctx->fpat.kron, ctx->fpat, vals, n, kd, kg, ok, s, nm("kron_bad"))))
+            return rc;
         if (ok) {
             w.kron = ctx->fpat.kron;
             w.kdiag = kd;
@@ -819,12 +977,12 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
         }
     }
     // the same trace vectors for every orientation (main.f90:50 reseeds per field)
-    std::vector<int64_t> sd((size_t)2 * Mp);
+    sl.sd.assign((size_t)2 * Mp, 0);
     for (int r = 0; r < Mp; ++r) {
-        sd[2 * r] = sample_seeds[2 * (r % M)];
-        sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
+        sl.sd[2 * r] = sample_seeds[2 * (r % M)];
+        sl.sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
     }
-    HIP_TRY(hipMemcpyAsync(d_seeds, sd.data(), sizeof(int64_t) * sd.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_seeds, sl.sd.data(), sizeof(int64_t) * sl.sd.size(), hipMemcpyHostToDevice, s));
     VecInit vi{};
     vi.kind = sim->vec_kind;
     vi.seeds = d_seeds;
@@ -836,16 +994,10 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
-    {
-        std::vector<double> obs0((size_t)4 * Mp);
-        for (int x = 0; x < 4; ++x)
-            for (int r = 0; r < Mp; ++r) obs0[(size_t)x * Mp + r] = pop[x];
-        HIP_TRY(hipMemcpyAsync(w.obs, obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
-    }
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
+    sl.obs0.assign((size_t)4 * Mp, 0.0);
+    for (int x = 0; x < 4; ++x)
+        for (int r = 0; r < Mp; ++r) sl.obs0[(size_t)x * Mp + r] = pop[x];
+    HIP_TRY(hipMemcpyAsync(w.obs, sl.obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
     auto one_step = [&]() -> hipError_t {
         hipError_t e;
         if ((e = arnoldi(w, s)) != hipSuccess) return e;
@@ -853,43 +1005,41 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
         if ((e = step_lanes(w, sim->dt, sim->t_end, 0.0, false, s)) != hipSuccess) return e;
         return psi_update(w, true, s);
     };
-    HIP_TRY(hipEventRecord(ev0, s));
+    sl.busy = true;
+    sl.o0 = o0;
+    sl.n = n;
+    HIP_TRY(hipEventRecord(a, s));
     if (graph && K > 1) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = one_step();
-        hipError_t e2 = hipStreamEndCapture(s, &g);
+        hipError_t e2 = hipStreamEndCapture(s, &sl.g);
         if (e != hipSuccess) return hip_fail(ctx, e, "step capture");
         if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            hipGraphDestroy(g);
-            return hip_fail(ctx, e, "hipGraphInstantiate");
-        }
-        for (int64_t k = 0; k < K && e == hipSuccess; ++k) e = hipGraphLaunch(ge, s);
-        const hipError_t er = hipEventRecord(ev1, s);
-        // the graph is destroyed only once its launches have run: destroying it
-        // with launches in flight frees state the runtime still uses
-        const hipError_t es = hipStreamSynchronize(s);
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
-        if (er != hipSuccess) return hip_fail(ctx, er, "hipEventRecord");
-        if (es != hipSuccess) return hip_fail(ctx, es, "hipStreamSynchronize");
+        HIP_TRY(hipGraphInstantiate(&sl.ge, sl.g, nullptr, nullptr, 0));
+        for (int64_t k = 0; k < K; ++k) HIP_TRY(hipGraphLaunch(sl.ge, s));
     } else {
         for (int64_t k = 0; k < K; ++k) HIP_TRY(one_step());
-        HIP_TRY(hipEventRecord(ev1, s));
     }
+    HIP_TRY(hipEventRecord(b, s));
     HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, Mp, d_acc + (size_t)o0 * 4 * nout, s));
+    return SPC_OK;
+}
+
+// Waits for the slot's chunk, releases its graph (only once its launches have
+// run: destroying it with launches in flight frees state the runtime still
+// uses) and checks its flags.
+int finish_orientation_chunk(spc_ctx *ctx, OrientSlot &sl)
+{
+    if (!sl.busy) return SPC_OK;
+    sl.busy = false;
+    const hipError_t es = hipStreamSynchronize(sl.s);
+    if (sl.ge) hipGraphExecDestroy(sl.ge);
+    if (sl.g) hipGraphDestroy(sl.g);
+    sl.ge = nullptr;
+    sl.g = nullptr;
+    if (es != hipSuccess) return hip_fail(ctx, es, "hipStreamSynchronize");
     int32_t flags[4];
-    HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
-    if (out_ms) *out_ms = ms;
+    HIP_TRY(hipMemcpy(flags, sl.w.flags, sizeof(flags), hipMemcpyDeviceToHost));
     if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
     if (flags[0])
         return fail(ctx, SPC_ERR_BREAKDOWN,
@@ -897,15 +1047,17 @@ int run_orientation_chunk(spc_ctx *ctx, const spc_sim_params *sim, int m, const 
     return SPC_OK;
 }
 
-// Orientations per chunk that fit `frac` of the free memory (power of two).
+// Orientations per chunk that fit 70% of the free memory (power of two); the
+// workspace of earlier runs counts as free.
 int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout)
 {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
-    for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part"}) {
-        auto it = ctx->bufs.find(nm);
-        if (it != ctx->bufs.end()) free_b += it->second.bytes;
-    }
+    for (const std::string sfx : {"", "#1", "#2", "#3"})
+        for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part"}) {
+            auto it = ctx->bufs.find(nm + sfx);
+            if (it != ctx->bufs.end()) free_b += it->second.bytes;
+        }
     const size_t per_orient = ((size_t)m + 3) * ctx->N * M * sizeof(double2) + (size_t)nout * 4 * M * sizeof(double) +
                               (size_t)ctx->fpat.nblocks * 16 * sizeof(double2);
     int best = 1;
@@ -914,6 +1066,40 @@ int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout)
     return best;
 }
 
+// Chunks of opc orientations in flight at once: one per lane stream (at most
+// kMaxLaneSets, SPC_LANE_STREAMS), as many as the memory for cap
+// orientations holds, no more than there are chunks.
+int orient_slots(int opc, int cap, int n_orient)
+{
+    const int chunks = (n_orient + opc - 1) / opc;
+    return std::max(1, std::min({lane_streams(), cap / std::max(opc, 1), chunks}));
+}
+
+// Length of the union of the intervals [t0_i, t1_i].
+double union_ms(std::vector<std::pair<double, double>> iv)
+{
+    std::sort(iv.begin(), iv.end());
+    double tot = 0.0, lo = 0.0, hi = -1.0;
+    bool open = false;
+    for (const auto &p : iv) {
+        if (open && p.first <= hi) {
+            hi = std::max(hi, p.second);
+            continue;
+        }
+        if (open) tot += hi - lo;
+        lo = p.first;
+        hi = p.second;
+        open = true;
+    }
+    if (open) tot += hi - lo;
+    return tot;
+}
+
+// All orientations in chunks of plan.opc, plan.slots chunks in flight at
+// once (each on its own stream with its own workspace; a slot takes the next
+// chunk once its previous one has finished).  Each chunk writes only its own
+// orientations' sums, so the result does not depend on the slots.
+// total_ms: device time during which some chunk's steps run.
 int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
                        const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms)
 {
@@ -922,19 +1108,67 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
     const int64_t nout = n_steps + 1;
     if (K + 1 > nout) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
     const int M = sim->n_samples;
+    const int nchunks = (n_orient + plan.opc - 1) / plan.opc;
+    const int S = std::max(1, std::min({plan.slots, kMaxLaneSets, nchunks}));
     double *d_acc = nullptr;
     int rc;
     if ((rc = ensure(ctx, "bacc", (size_t)n_orient * 4 * nout, &d_acc))) return rc;
-    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * n_orient * 4 * nout, ctx->stream));
-    double tot = 0.0;
-    for (int o0 = 0; o0 < n_orient; o0 += plan.opc) {
-        const int n = std::min(plan.opc, n_orient - o0);
-        double ms = 0.0;
-        if ((rc = run_orientation_chunk(ctx, sim, plan.m, sample_seeds, M, dirs, o0, n, plan.graph, K, nout, d_acc,
-                                        &ms)))
-            return rc;
-        tot += ms;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * n_orient * 4 * nout, s));
+    OrientSlot slots[kMaxLaneSets];
+    for (int j = 0; j < S; ++j) {
+        slots[j].idx = j;
+        if (j == 0) {
+            slots[j].s = s;
+        } else {
+            if (!ctx->lane_stream[j - 1]) HIP_TRY(hipStreamCreate(&ctx->lane_stream[j - 1]));
+            slots[j].s = ctx->lane_stream[j - 1];
+        }
     }
+    // events: ev0 before everything, a and b around each chunk's steps
+    std::vector<hipEvent_t> evs((size_t)2 * nchunks + 2, nullptr);
+    for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
+    auto destroy_events = [&]() {
+        for (auto e : evs)
+            if (e) hipEventDestroy(e);
+    };
+    auto finish_all = [&]() {
+        int r = SPC_OK;
+        for (int j = 0; j < S; ++j) {
+            const int rj = finish_orientation_chunk(ctx, slots[j]);
+            if (!r) r = rj;
+        }
+        return r;
+    };
+    hipEvent_t ev0 = evs[0];
+    HIP_TRY(hipEventRecord(ev0, s));
+    for (int j = 1; j < S; ++j) HIP_TRY(hipStreamWaitEvent(slots[j].s, ev0, 0));
+    for (int c = 0; c < nchunks; ++c) {
+        OrientSlot &sl = slots[c % S];
+        if ((rc = finish_orientation_chunk(ctx, sl)) ||
+            (rc = launch_orientation_chunk(ctx, sl, sim, plan.m, sample_seeds, M, dirs, c * plan.opc,
+                                           std::min(plan.opc, n_orient - c * plan.opc), plan.graph, K, nout, d_acc,
+                                           evs[2 + 2 * c], evs[3 + 2 * c]))) {
+            finish_all();
+            destroy_events();
+            return rc;
+        }
+    }
+    if ((rc = finish_all())) {
+        destroy_events();
+        return rc;
+    }
+    std::vector<std::pair<double, double>> iv;
+    for (int c = 0; c < nchunks; ++c) {
+        float ta = 0.f, tb = 0.f;
+        if (hipEventElapsedTime(&ta, ev0, evs[2 + 2 * c]) != hipSuccess ||
+            hipEventElapsedTime(&tb, ev0, evs[3 + 2 * c]) != hipSuccess) {
+            destroy_events();
+            return fail(ctx, SPC_ERR_HIP, "hipEventElapsedTime");
+        }
+        iv.emplace_back(ta, tb);
+    }
+    destroy_events();
     ps_out.assign((size_t)n_orient * nout, 0.0);
     std::vector<double> acc((size_t)n_orient * 4 * nout);
     HIP_TRY(hipMemcpy(acc.data(), d_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
@@ -944,7 +1178,7 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
             if (!std::isfinite(v)) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite population");
             ps_out[(size_t)o * nout + k] = v;
         }
-    if (total_ms) *total_ms = tot;
+    if (total_ms) *total_ms = union_ms(iv);
     return SPC_OK;
 }
 
@@ -1006,12 +1240,16 @@ void spc_ctx_destroy(spc_ctx *ctx)
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (hipStream_t ls : ctx->lane_stream)
+        if (ls) hipStreamSynchronize(ls);
     for (auto &kv : ctx->bufs)
         if (kv.second.p) hipFree(kv.second.p);
     ctx->ham.release();
     ctx->adj.release();
     ctx->fpat.release();
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    for (hipStream_t ls : ctx->lane_stream)
+        if (ls) hipStreamDestroy(ls);
     delete ctx;
 }
 
@@ -1857,8 +2095,9 @@ int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     BatchPlan plan;
     plan.m = sim->n_krylov;
     plan.graph = use_graph != 0;
-    plan.opc = orient_per_chunk > 0 ? std::min<int>(orient_per_chunk, n_orient)
-                                     : max_orient_per_chunk(ctx, plan.m, sim->n_samples, n_orient, nout);
+    const int cap = max_orient_per_chunk(ctx, plan.m, sim->n_samples, n_orient, nout);
+    plan.opc = orient_per_chunk > 0 ? std::min<int>(orient_per_chunk, n_orient) : cap;
+    plan.slots = orient_slots(plan.opc, cap, n_orient);
     std::vector<double> ps;
     double ms = 0.0;
     if ((rc = batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, plan, ps, &ms))) return rc;
@@ -1881,8 +2120,9 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
     int rc = check_batch_args(ctx, sim, sample_seeds, n_orient, dirs);
     if (rc) return rc;
     const int64_t K = steps_taken(sim->t_end, sim->dt);
-    // Krylov order: the smallest (in steps of 2) whose pilot yields agree with
-    // the requested order's within yield_tol scaled to the pilot length
+    // Krylov order: the smallest whose pilot yields agree with the requested
+    // order's within yield_tol scaled to the pilot length (in steps of 2, then
+    // the order between the last that passed and the first that failed)
     const int64_t pilot = std::min<int64_t>(K, 64);
     spc_sim_params ps = *sim;
     ps.t_end = sim->dt * (double)pilot - 0.5 * sim->dt;
@@ -1894,22 +2134,26 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
     std::vector<double> ref, cur;
     if ((rc = batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, ref, nullptr))) return rc;
     const double tol = yield_tol * (double)std::max<int64_t>(pilot, 1) / (double)std::max<int64_t>(K, 1);
-    int m_best = sim->n_krylov;
-    for (int m = sim->n_krylov - 2; m >= 6; m -= 2) {
+    auto order_ok = [&](int m) {
         plan.m = m;
         ps.n_krylov = m;
-        if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, nullptr)) break;
+        if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, nullptr)) return false;
         double worst = 0.0;
         for (int o = 0; o < npilot; ++o) {
             const double a = final_yield(ref.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
             const double b = final_yield(cur.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
             worst = std::max(worst, std::fabs(a - b) / std::max(std::fabs(a), 1e-300));
         }
-        if (!(worst <= tol)) break;
-        m_best = m;
-    }
+        return worst <= tol;
+    };
+    int m_best = sim->n_krylov;
+    int m = sim->n_krylov - 2;
+    for (; m >= 6 && order_ok(m); m -= 2) m_best = m;
+    // the order between the last that passed and the first that failed
+    if (m_best - 1 >= 6 && m_best - 1 > m && order_ok(m_best - 1)) m_best -= 1;
     ctx->err.clear();
-    // lane width: fastest orientation-steps per ms over powers of two that fit
+    // lane width: fastest orientation-steps per ms over powers of two that fit,
+    // each timed with as many chunks in flight as spc_orientation_batch runs
     spc_sim_params ts = *sim;
     ts.n_krylov = m_best;
     const int64_t tsteps = std::min<int64_t>(K, 6);
@@ -1923,11 +2167,13 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
         tp.m = m_best;
         tp.opc = c;
         tp.graph = true;
+        tp.slots = orient_slots(c, cap, n_orient);
+        const int no = std::min(n_orient, tp.slots * c);
         std::vector<double> tmp;
         double ms = 0.0;
-        if ((rc = batch_orientations(ctx, &ts, sample_seeds, c, dirs, tp, tmp, &ms))) return rc; // warm
-        if ((rc = batch_orientations(ctx, &ts, sample_seeds, c, dirs, tp, tmp, &ms))) return rc;
-        const double per = ms / ((double)c * (double)std::max<int64_t>(tsteps, 1));
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc; // warm
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc;
+        const double per = ms / ((double)no * (double)std::max<int64_t>(tsteps, 1));
         if (per < best_t * 0.97) {
             best_t = per;
             best_c = c;

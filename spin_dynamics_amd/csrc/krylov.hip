@@ -1350,12 +1350,14 @@ using KronFn = void (*)(const KronDev *, const uint64_t *, const double2 *, cons
 
 // Lane-slice depth of the Kronecker product: 2^Q nuclear indices per wave on
 // 64 >> Q lanes.  Default 2 where the system allows it (N = 36864, M = 256 on
-// MI355X: 160 us per product at Q = 2, 167 at 1, 202 at 3, 212 at 0);
-// SPC_KRON_Q overrides it (read at every launch), capped by the system.
+// MI355X: 160 us per product at Q = 2, 167 at 1, 202 at 3, 212 at 0), 3 for
+// a chunk of 64 lanes (the 4 x 64 lane split of M = 256 steps in 14.7 ms at
+// Q = 3, 15.0 at Q = 2); SPC_KRON_Q overrides it (read at every launch),
+// capped by the system.
 int kron_slices(const KrylovWork &w)
 {
     if (!w.kron) return 0;
-    int q = 2;
+    int q = w.Mp <= kWave ? 3 : 2;
     if (const char *e = std::getenv("SPC_KRON_Q")) q = std::max(0, std::atoi(e));
     return std::min(q, w.kq);
 }

@@ -15,7 +15,12 @@ namespace spc {
 
 namespace {
 
-__global__ void k_init_vectors(VecInit vi, int M, int Mp, int64_t Z, double2 *__restrict__ psi)
+// Sampled vectors (kinds 0, 1): one thread per lane, as its stream is
+// sequential (the polar method rejects pairs).  Every element is written once
+// and nothing is read back: kind 0 draws the stream twice, once for the norm
+// and once for the normalised values (the same numbers in the same order).
+// Padding lanes r >= M are zero.
+__global__ void k_init_sampled(VecInit vi, int M, int Mp, int64_t Z, double2 *__restrict__ psi)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= Mp) return;
@@ -24,49 +29,66 @@ __global__ void k_init_vectors(VecInit vi, int M, int Mp, int64_t Z, double2 *__
         for (int64_t n = 0; n < 4 * Z; ++n) psi[n * ld + r] = cz(0, 0);
         return;
     }
-    if (vi.kind == 0 || vi.kind == 1) {
-        Rng g;
+    Rng g;
+    double nrm = 1.0;
+    if (vi.kind == 0) {
         rng_set_seed(g, vi.seeds[2 * r], vi.seeds[2 * r + 1]);
         double nrm2 = 0.0;
-        const double f = 1.0 / sqrt((double)Z);
         for (int64_t n = 0; n < Z; ++n) {
-            double2 z;
-            if (vi.kind == 0) {
-                z.x = rng_normal(g);
-                z.y = rng_normal(g);
-                nrm2 += z.x * z.x + z.y * z.y;
-            } else {
-                double x, y;
-                rng_circle(g, x, y);
-                z = cz(x * f, y * f);
-            }
-            psi[(4 * n) * ld + r] = z; // staged in the alpha-alpha row
+            const double x = rng_normal(g);
+            const double y = rng_normal(g);
+            nrm2 += x * x + y * y;
         }
-        const double nrm = vi.kind == 0 ? sqrt(nrm2) : 1.0;
-        for (int64_t n = 0; n < Z; ++n) {
-            double2 z = psi[(4 * n) * ld + r];
-            if (vi.kind == 0) z = cz(z.x / nrm, z.y / nrm);
-            for (int e = 0; e < 4; ++e) psi[(4 * n + e) * ld + r] = cmul(vi.theta[e], z);
-        }
-        return;
+        nrm = sqrt(nrm2);
     }
+    rng_set_seed(g, vi.seeds[2 * r], vi.seeds[2 * r + 1]);
+    const double f = 1.0 / sqrt((double)Z);
     for (int64_t n = 0; n < Z; ++n) {
         double2 z;
-        if (vi.kind == 2)
-            z = vi.inj[(int64_t)r * Z + n];
-        else
-            z = (n == vi.basis_offset + r) ? cz(1.0, 0.0) : cz(0.0, 0.0);
+        if (vi.kind == 0) {
+            const double x = rng_normal(g);
+            const double y = rng_normal(g);
+            z = cz(x / nrm, y / nrm);
+        } else {
+            double x, y;
+            rng_circle(g, x, y);
+            z = cz(x * f, y * f);
+        }
         for (int e = 0; e < 4; ++e) psi[(4 * n + e) * ld + r] = cmul(vi.theta[e], z);
     }
+}
+
+// Injected vectors and basis states (kinds 2, 3): one thread per (nuclear
+// index, lane), lanes fastest; padding lanes are zero.
+__global__ void k_init_given(VecInit vi, int M, int Mp, int64_t Z, double2 *__restrict__ psi)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Z * Mp) return;
+    const int64_t n = idx / Mp;
+    const int r = (int)(idx % Mp);
+    double2 z = cz(0.0, 0.0);
+    if (r < M) {
+        if (vi.kind == 2)
+            z = vi.inj[(int64_t)r * Z + n];
+        else if (n == vi.basis_offset + r)
+            z = cz(1.0, 0.0);
+    }
+    for (int e = 0; e < 4; ++e) psi[(4 * n + e) * Mp + r] = cmul(vi.theta[e], z);
 }
 
 } // namespace
 
 hipError_t init_vectors(const VecInit &vi, int M, int Mp, int64_t Z, double2 *psi, hipStream_t s)
 {
-    int threads = 64;
-    int blocks = (Mp + threads - 1) / threads;
-    hipLaunchKernelGGL(k_init_vectors, dim3(blocks), dim3(threads), 0, s, vi, M, Mp, Z, psi);
+    if (vi.kind == 0 || vi.kind == 1) {
+        const int threads = 64;
+        hipLaunchKernelGGL(k_init_sampled, dim3((Mp + threads - 1) / threads), dim3(threads), 0, s, vi, M, Mp, Z,
+                           psi);
+    } else {
+        const int threads = 256;
+        hipLaunchKernelGGL(k_init_given, dim3((unsigned)((Z * Mp + threads - 1) / threads)), dim3(threads), 0, s, vi,
+                           M, Mp, Z, psi);
+    }
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@ Pins: every orientation's P_S against the oracle run at that orientation with
 the same trace vectors (orc_trace_sampling, 1e-10 relative); against the
 plain per-orientation device runs; the captured-graph replay equal to the
 eager launches, bit for bit; results independent of how orientations are
-chunked into lanes; the autotuned Krylov order keeps the yields within the
+chunked into lanes and of how many chunks run at once; the autotuned Krylov order keeps the yields within the
 requested tolerance."""
 import numpy as np
 import pytest
@@ -67,6 +67,22 @@ def test_chunking_does_not_change_results(ctx, opc):
     ref = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=7, ctx=ctx)
     got = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=opc, ctx=ctx)
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_chunks_in_flight_do_not_change_results(ctx, monkeypatch, graph):
+    # 5 chunks of one orientation: with 4 streams a slot takes a second chunk
+    # once its first has finished; every chunk writes only its own sums, so
+    # the result is bit for bit that of one chunk at a time
+    w = _case(T_ns=12.0)
+    dirs = batch.fibonacci_sphere(5)
+    runs = {}
+    for n in (1, 2, 4):
+        monkeypatch.setenv("SPC_LANE_STREAMS", str(n))
+        runs[n] = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=1, graph=graph,
+                                           with_populations=True, ctx=ctx)
+    for n in (2, 4):
+        assert np.array_equal(runs[n][0], runs[1][0]) and np.array_equal(runs[n][1], runs[1][1]), n
 
 
 def test_more_trace_vectors_and_su_z(ctx):
