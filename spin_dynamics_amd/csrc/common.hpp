@@ -141,15 +141,12 @@ __device__ inline double rng_normal(Rng &r)
     return a;
 }
 
-__device__ inline void rng_circle(Rng &r, double &x, double &y)
+// rng%circle (m_random.f90:280-297) draws a pair (a, b) uniform in the unit
+// disc by rejection (2 unif01 per try, accepted when a^2 + b^2 <= 1) and
+// returns the point on the unit circle at twice its angle, computed here.
+__device__ inline void circle_point(double a, double b, double &x, double &y)
 {
-    double a, b, s;
-    for (;;) {
-        a = 2.0 * rng_unif01(r) - 1.0;
-        b = 2.0 * rng_unif01(r) - 1.0;
-        s = a * a + b * b;
-        if (s <= 1.0) break;
-    }
+    const double s = a * a + b * b;
     x = (a * a - b * b) / s;
     y = 2.0 * a * b / s;
 }

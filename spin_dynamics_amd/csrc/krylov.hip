@@ -608,27 +608,37 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
     }
 }
 
-// Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].
-__global__ __launch_bounds__(kBlock) void k_reduce(const double2 *__restrict__ part, double2 *__restrict__ out,
-                                                    int nacc, int nrb, int Mp)
+// Deterministic tree reduction: out[j][r] = sum_rb part[rb][j][r].  The
+// partials were just written (they sit in L2/MALL), so the pass is bound by
+// load latency: 16 waves per (j, 64 lanes), each with 4 loads in flight, take
+// 64 rows per round trip; the order of the sums is fixed.
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(kRedWaves *kWave) void k_reduce(const double2 *__restrict__ part,
+                                                             double2 *__restrict__ out, int nacc, int nrb, int Mp)
 {
-    __shared__ double2 red[kWaves][kWave];
+    __shared__ double2 red[kRedWaves][kWave];
     const int j = blockIdx.x, ch = blockIdx.y;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int r = ch * kWave + lane;
-    double2 s0 = cz(0, 0), s1 = cz(0, 0);
+    const int64_t step = (int64_t)nacc * Mp;
+    const double2 *p = part + (int64_t)j * Mp + r;
+    double2 s0 = cz(0, 0), s1 = cz(0, 0), s2 = cz(0, 0), s3 = cz(0, 0);
     int rb = w;
-    for (; rb + kWaves < nrb; rb += 2 * kWaves) {
-        s0 = cadd(s0, part[((int64_t)rb * nacc + j) * Mp + r]);
-        s1 = cadd(s1, part[((int64_t)(rb + kWaves) * nacc + j) * Mp + r]);
+    for (; rb + 3 * kRedWaves < nrb; rb += 4 * kRedWaves) {
+        const double2 a0 = p[rb * step], a1 = p[(rb + kRedWaves) * step];
+        const double2 a2 = p[(rb + 2 * kRedWaves) * step], a3 = p[(rb + 3 * kRedWaves) * step];
+        s0 = cadd(s0, a0);
+        s1 = cadd(s1, a1);
+        s2 = cadd(s2, a2);
+        s3 = cadd(s3, a3);
     }
-    for (; rb < nrb; rb += kWaves) s0 = cadd(s0, part[((int64_t)rb * nacc + j) * Mp + r]);
-    red[w][lane] = cadd(s0, s1);
+    for (; rb < nrb; rb += kRedWaves) s0 = cadd(s0, p[rb * step]);
+    red[w][lane] = cadd(cadd(s0, s1), cadd(s2, s3));
     __syncthreads();
     if (w == 0) {
-        double2 s = red[0][lane];
-        for (int q = 1; q < kWaves; ++q) s = cadd(s, red[q][lane]);
-        out[(int64_t)j * Mp + r] = s;
+        double2 t = red[0][lane];
+        for (int q = 1; q < kRedWaves; ++q) t = cadd(t, red[q][lane]);
+        out[(int64_t)j * Mp + r] = t;
     }
 }
 
@@ -1341,7 +1351,8 @@ hipError_t reduce(const KrylovWork &w, int nacc, int nrb, double2 *out, hipStrea
 {
     if (nacc == 0) return hipSuccess;
     ProfScope ps(w, PF_REDUCE, 16.0 * (double)nacc * w.Mp * (nrb + 1), s);
-    hipLaunchKernelGGL(k_reduce, dim3(nacc, w.Mp / kWave), dim3(kBlock), 0, s, w.part, out, nacc, nrb, w.Mp);
+    hipLaunchKernelGGL(k_reduce, dim3(nacc, w.Mp / kWave), dim3(kRedWaves * kWave), 0, s, w.part, out, nacc, nrb,
+                       w.Mp);
     return hipGetLastError();
 }
 
