@@ -280,26 +280,6 @@ KronDev kron_of(const DevSystem &S)
     return k;
 }
 
-// Kronecker tables of h (nsets value sets of its block pattern); on a
-// mismatch with the block form the tables are dropped (the block product is
-// used).  diag/gen: nsets * Z * 16 and nsets * nn * 24 values.
-int make_kron(spc_ctx *ctx, const KronDev &hk, KronDev *dk, const HamDev &h, const double2 *bval, int nsets,
-              double2 *diag, double2 *gen, bool &ok, hipStream_t s = nullptr,
-              const std::string &bad_name = "kron_bad")
-{
-    int32_t *bad = nullptr;
-    int rc = ensure(ctx, bad_name.c_str(), (size_t)1, &bad);
-    if (rc) return rc;
-    if (!s) s = ctx->stream;
-    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
-    HIP_TRY(kron_tables(dk, hk, h.browptr, h.bcol, bval, h.nblocks, nsets, ctx->Z ? ctx->Z : 1, diag, gen, bad, s));
-    int32_t b = 0;
-    HIP_TRY(hipMemcpyAsync(&b, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    ok = b == 0;
-    return SPC_OK;
-}
-
 // Assembles H for the parameters S on the device: CSR by count / scan / fill
 // (getHiso_sp + coo_to_csr), then the 4x4 block form.
 int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
@@ -868,36 +848,47 @@ struct BatchPlan {
     int m = 0;         // Krylov order
     bool graph = true; // replay one captured step
     int slots = 1;     // chunks in flight at once, each on its own stream
+    bool kron = true;  // per-orientation Kronecker tables (else the block product)
 };
 
-// One chunk of orientations in flight: its workspace, stream, step graph and
-// host staging (kept until the chunk has finished).
+// One stream of the batched run: its workspace and the step graph of its
+// current chunk width (captured at its first chunk of that width, replayed
+// for the next ones: the chunks of a slot use the same buffers).
 struct OrientSlot {
     KrylovWork w{};
     hipStream_t s = nullptr;
     int idx = 0;
-    bool busy = false;
-    int o0 = 0, n = 0;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    std::vector<double> f, obs0;
-    std::vector<int64_t> sd;
+    int n = 0;            // orientations per chunk the workspace is set up for
+    bool capture = true;  // the step graph of this width is still to be captured
+    std::vector<std::pair<hipGraph_t, hipGraphExec_t>> graphs; // released after the final synchronize
+    void release()
+    {
+        for (auto &g : graphs) {
+            if (g.second) hipGraphExecDestroy(g.second);
+            if (g.first) hipGraphDestroy(g.first);
+        }
+        graphs.clear();
+    }
 };
 
-// Enqueues orientations [o0, o0 + n) of dirs as one lane batch (M lanes per
-// orientation, M a multiple of 64) for K steps on the slot's stream: field
-// values and Kronecker tables of each orientation's H, the trace vectors,
-// then the steps (one captured step replayed K times when graph is set)
-// between the events a and b, then each orientation's summed populations
-// added to acc[o][4 * n_out].  Returns with the steps in flight.
-int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, int m,
-                             const int64_t *sample_seeds, int M, const double *dirs, int o0, int n, bool graph,
-                             int64_t K, int64_t nout, double *d_acc, hipEvent_t a, hipEvent_t b)
+// Per-batch device inputs shared by every chunk: field vectors of all
+// orientations, the lanes' seeds (the same M trace vectors for every
+// orientation), initial populations, and per-chunk flag and table-check
+// slots, checked once the batch has drained.
+struct BatchShared {
+    double *fields = nullptr; // [n_orient][3]
+    int64_t *seeds = nullptr; // [opc * M][2]
+    double *obs0 = nullptr;   // [4][opc * M]
+    int32_t *flags = nullptr; // [chunks][4]
+    int32_t *bad = nullptr;   // [chunks]
+};
+
+// Sets up the slot's workspace for chunks of n orientations (M lanes each).
+int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n, int64_t nout)
 {
     const int64_t N = ctx->N, Z = ctx->Z;
-    const int Mp = n * M;
+    const int Mp = n * M, m = plan.m;
     const DevSystem &hs = ctx->hsys;
-    const double Bmag = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
     const int64_t nb = ctx->fpat.nblocks;
     int rc;
     KrylovWork &w = sl.w;
@@ -918,12 +909,10 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
     w.rpw1 = (int)std::max<int64_t>(1, (Z * nch + 4 * 4096 - 1) / (4 * 4096));
     w.rpw2 = update_rows(Z, nch);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
-    double2 *vals = nullptr;
-    double *d_fields = nullptr;
-    int64_t *d_seeds = nullptr;
     w.qs = basis_stride(N, Mp);
     const std::string sfx = sl.idx ? "#" + std::to_string(sl.idx) : "";
     auto nm = [&](const char *base) { return std::string(base) + sfx; };
+    double2 *vals = nullptr;
     if ((rc = ensure(ctx, nm("Q").c_str(), (size_t)m * w.qs, &w.Q))) return rc;
     if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
@@ -945,48 +934,51 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
     if ((rc = ensure(ctx, nm("bvals").c_str(), (size_t)n * nb * 16, &vals))) return rc;
-    if ((rc = ensure(ctx, nm("bfields").c_str(), (size_t)3 * n, &d_fields))) return rc;
-    if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &d_seeds))) return rc;
     w.bval = vals;
-    hipStream_t s = sl.s;
-    // field vectors B n_o, values of each orientation's H on the pattern
-    sl.f.assign((size_t)3 * n, 0.0);
-    for (int o = 0; o < n; ++o) {
-        const double *d = dirs + 3 * (o0 + o);
-        const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        if (!(nd > 0.0)) return fail(ctx, SPC_ERR_INVALID, "orientation direction must be nonzero");
-        for (int x = 0; x < 3; ++x) sl.f[3 * o + x] = Bmag * d[x] / nd;
-    }
-    HIP_TRY(hipMemcpyAsync(d_fields, sl.f.data(), sizeof(double) * sl.f.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], d_fields, nb * 16, n, vals, s));
     w.kron = nullptr;
-    if (ctx->fpat.kron && !kron_disabled()) { // per-orientation Kronecker tables
+    if (plan.kron && ctx->fpat.kron && !kron_disabled()) { // per-orientation Kronecker tables
         double2 *kd = nullptr, *kg = nullptr;
         if ((rc = ensure(ctx, nm("bkdiag").c_str(), (size_t)n * Z * 16, &kd))) return rc;
         if ((rc = ensure(ctx, nm("bkgen").c_str(), (size_t)n * std::max(hs.nn, 1) * 24, &kg))) return rc;
-        bool ok = false;
-        if ((rc = make_kron(ctx, kron_of(hs), ctx->fpat.kron, ctx->fpat, vals, n, kd, kg, ok, s, nm("kron_bad"))))
-            return rc;
-        if (ok) {
-            w.kron = ctx->fpat.kron;
-            w.kdiag = kd;
-            w.kgen = kg;
-            w.kq = kron_inwave(kron_of(hs));
-            w.kmask = ctx->fpat.kmask;
-            w.kdiag_stride = Z * 16;
-            w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
-        }
+        w.kron = ctx->fpat.kron;
+        w.kdiag = kd;
+        w.kgen = kg;
+        w.kq = kron_inwave(kron_of(hs));
+        w.kmask = ctx->fpat.kmask;
+        w.kdiag_stride = Z * 16;
+        w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
     }
-    // the same trace vectors for every orientation (main.f90:50 reseeds per field)
-    sl.sd.assign((size_t)2 * Mp, 0);
-    for (int r = 0; r < Mp; ++r) {
-        sl.sd[2 * r] = sample_seeds[2 * (r % M)];
-        sl.sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
+    sl.n = n;
+    sl.capture = true;
+    return SPC_OK;
+}
+
+// Enqueues chunk c (orientations [o0, o0 + n)) on the slot's stream for K
+// steps: the values of each orientation's H on the block pattern and its
+// Kronecker tables, the trace vectors, the steps between the events a and b,
+// then each orientation's summed populations added to acc[o][4 * n_out]
+// and the chunk's flags saved.  Nothing here waits for the device.
+int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, const BatchPlan &plan,
+                             const BatchShared &sh, int M, int c, int o0, int n, int64_t K, int64_t nout,
+                             double *d_acc, hipEvent_t a, hipEvent_t b)
+{
+    int rc;
+    if (sl.n != n) { // a new width: the slot's earlier chunks drain before its buffers and graph change
+        if (sl.n) HIP_TRY(hipStreamSynchronize(sl.s));
+        if ((rc = slot_setup(ctx, sl, plan, M, n, nout))) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(d_seeds, sl.sd.data(), sizeof(int64_t) * sl.sd.size(), hipMemcpyHostToDevice, s));
+    KrylovWork &w = sl.w;
+    const int Mp = w.Mp, m = w.m;
+    const int64_t Z = ctx->Z, nb = ctx->fpat.nblocks;
+    hipStream_t s = sl.s;
+    HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], sh.fields + 3 * (size_t)o0, nb * 16, n,
+                           const_cast<double2 *>(w.bval), s));
+    if (w.kron)
+        HIP_TRY(kron_tables(ctx->fpat.kron, kron_of(ctx->hsys), ctx->fpat.browptr, ctx->fpat.bcol, w.bval, nb, n, Z,
+                            const_cast<double2 *>(w.kdiag), const_cast<double2 *>(w.kgen), sh.bad + c, s));
     VecInit vi{};
     vi.kind = sim->vec_kind;
-    vi.seeds = d_seeds;
+    vi.seeds = sh.seeds;
     double pop[4];
     theta_of(sim->init_state, vi.theta, pop);
     HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
@@ -995,10 +987,8 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
-    sl.obs0.assign((size_t)4 * Mp, 0.0);
-    for (int x = 0; x < 4; ++x)
-        for (int r = 0; r < Mp; ++r) sl.obs0[(size_t)x * Mp + r] = pop[x];
-    HIP_TRY(hipMemcpyAsync(w.obs, sl.obs0.data(), sizeof(double) * 4 * Mp, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(w.obs, sizeof(double) * Mp, sh.obs0, sizeof(double) * plan.opc * M,
+                             sizeof(double) * Mp, 4, hipMemcpyDeviceToDevice, s));
     auto one_step = [&]() -> hipError_t {
         hipError_t e;
         if ((e = arnoldi(w, s)) != hipSuccess) return e;
@@ -1006,45 +996,27 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
         if ((e = step_lanes(w, sim->dt, sim->t_end, 0.0, false, s)) != hipSuccess) return e;
         return psi_update(w, true, s);
     };
-    sl.busy = true;
-    sl.o0 = o0;
-    sl.n = n;
     HIP_TRY(hipEventRecord(a, s));
-    if (graph && K > 1) {
-        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = one_step();
-        hipError_t e2 = hipStreamEndCapture(s, &sl.g);
-        if (e != hipSuccess) return hip_fail(ctx, e, "step capture");
-        if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
-        HIP_TRY(hipGraphInstantiate(&sl.ge, sl.g, nullptr, nullptr, 0));
-        for (int64_t k = 0; k < K; ++k) HIP_TRY(hipGraphLaunch(sl.ge, s));
+    if (plan.graph && K > 1) {
+        if (sl.capture) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = one_step();
+            hipError_t e2 = hipStreamEndCapture(s, &g);
+            sl.graphs.emplace_back(g, nullptr);
+            if (e != hipSuccess) return hip_fail(ctx, e, "step capture");
+            if (e2 != hipSuccess) return hip_fail(ctx, e2, "hipStreamEndCapture");
+            HIP_TRY(hipGraphInstantiate(&sl.graphs.back().second, g, nullptr, nullptr, 0));
+            sl.capture = false;
+        }
+        hipGraphExec_t ge = sl.graphs.back().second;
+        for (int64_t k = 0; k < K; ++k) HIP_TRY(hipGraphLaunch(ge, s));
     } else {
         for (int64_t k = 0; k < K; ++k) HIP_TRY(one_step());
     }
     HIP_TRY(hipEventRecord(b, s));
     HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, Mp, d_acc + (size_t)o0 * 4 * nout, s));
-    return SPC_OK;
-}
-
-// Waits for the slot's chunk, releases its graph (only once its launches have
-// run: destroying it with launches in flight frees state the runtime still
-// uses) and checks its flags.
-int finish_orientation_chunk(spc_ctx *ctx, OrientSlot &sl)
-{
-    if (!sl.busy) return SPC_OK;
-    sl.busy = false;
-    const hipError_t es = hipStreamSynchronize(sl.s);
-    if (sl.ge) hipGraphExecDestroy(sl.ge);
-    if (sl.g) hipGraphDestroy(sl.g);
-    sl.ge = nullptr;
-    sl.g = nullptr;
-    if (es != hipSuccess) return hip_fail(ctx, es, "hipStreamSynchronize");
-    int32_t flags[4];
-    HIP_TRY(hipMemcpy(flags, sl.w.flags, sizeof(flags), hipMemcpyDeviceToHost));
-    if (flags[3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
-    if (flags[0])
-        return fail(ctx, SPC_ERR_BREAKDOWN,
-                    "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+    HIP_TRY(hipMemcpyAsync(sh.flags + 4 * (size_t)c, w.flags, sizeof(int32_t) * 4, hipMemcpyDeviceToDevice, s));
     return SPC_OK;
 }
 
@@ -1096,11 +1068,13 @@ double union_ms(std::vector<std::pair<double, double>> iv)
     return tot;
 }
 
-// All orientations in chunks of plan.opc, plan.slots chunks in flight at
-// once (each on its own stream with its own workspace; a slot takes the next
-// chunk once its previous one has finished).  Each chunk writes only its own
-// orientations' sums, so the result does not depend on the slots.
-// total_ms: device time during which some chunk's steps run.
+// All orientations in chunks of plan.opc; chunk c runs on slot c % slots,
+// each slot a stream with its own workspace, so plan.slots chunks are in
+// flight at once.  The host enqueues every chunk without waiting (inputs of
+// all chunks are uploaded first; flags and table checks are read once the
+// batch has drained).  Each chunk writes only its own orientations' sums,
+// so the result does not depend on the slots.  total_ms: device time during
+// which some chunk's steps run.
 int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
                        const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms)
 {
@@ -1109,13 +1083,50 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
     const int64_t nout = n_steps + 1;
     if (K + 1 > nout) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
     const int M = sim->n_samples;
-    const int nchunks = (n_orient + plan.opc - 1) / plan.opc;
+    const int opc = plan.opc;
+    const int nchunks = (n_orient + opc - 1) / opc;
     const int S = std::max(1, std::min({plan.slots, kMaxLaneSets, nchunks}));
-    double *d_acc = nullptr;
+    const int Mw = opc * M; // lanes of a full chunk
     int rc;
+    // inputs of every chunk
+    const DevSystem &hs = ctx->hsys;
+    const double Bmag = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
+    std::vector<double> f((size_t)3 * n_orient);
+    for (int o = 0; o < n_orient; ++o) {
+        const double *d = dirs + 3 * o;
+        const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (!(nd > 0.0)) return fail(ctx, SPC_ERR_INVALID, "orientation direction must be nonzero");
+        for (int x = 0; x < 3; ++x) f[3 * o + x] = Bmag * d[x] / nd;
+    }
+    // the same trace vectors for every orientation (main.f90:50 reseeds per field)
+    std::vector<int64_t> sd((size_t)2 * Mw);
+    for (int r = 0; r < Mw; ++r) {
+        sd[2 * r] = sample_seeds[2 * (r % M)];
+        sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
+    }
+    double2 th[4];
+    double pop[4];
+    theta_of(sim->init_state, th, pop);
+    std::vector<double> obs0((size_t)4 * Mw);
+    for (int x = 0; x < 4; ++x)
+        for (int r = 0; r < Mw; ++r) obs0[(size_t)x * Mw + r] = pop[x];
+    BatchShared sh;
+    double *d_acc = nullptr;
     if ((rc = ensure(ctx, "bacc", (size_t)n_orient * 4 * nout, &d_acc))) return rc;
+    if ((rc = ensure(ctx, "bfields", f.size(), &sh.fields))) return rc;
+    if ((rc = ensure(ctx, "bseeds", sd.size(), &sh.seeds))) return rc;
+    if ((rc = ensure(ctx, "bobs0", obs0.size(), &sh.obs0))) return rc;
+    if ((rc = ensure(ctx, "bflags", (size_t)4 * nchunks, &sh.flags))) return rc;
+    if ((rc = ensure(ctx, "bbad", (size_t)nchunks, &sh.bad))) return rc;
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * n_orient * 4 * nout, s));
+    HIP_TRY(hipMemsetAsync(sh.flags, 0, sizeof(int32_t) * 4 * nchunks, s));
+    HIP_TRY(hipMemsetAsync(sh.bad, 0, sizeof(int32_t) * nchunks, s));
+    HIP_TRY(hipMemcpyAsync(sh.fields, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sh.seeds, sd.data(), sizeof(int64_t) * sd.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sh.obs0, obs0.data(), sizeof(double) * obs0.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+
     OrientSlot slots[kMaxLaneSets];
     for (int j = 0; j < S; ++j) {
         slots[j].idx = j;
@@ -1127,49 +1138,66 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         }
     }
     // events: ev0 before everything, a and b around each chunk's steps
-    std::vector<hipEvent_t> evs((size_t)2 * nchunks + 2, nullptr);
-    for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
-    auto destroy_events = [&]() {
+    std::vector<hipEvent_t> evs((size_t)2 * nchunks + 1, nullptr);
+    auto cleanup = [&]() {
+        for (int j = 0; j < S; ++j) {
+            (void)hipStreamSynchronize(slots[j].s);
+            slots[j].release();
+        }
         for (auto e : evs)
             if (e) hipEventDestroy(e);
     };
-    auto finish_all = [&]() {
-        int r = SPC_OK;
-        for (int j = 0; j < S; ++j) {
-            const int rj = finish_orientation_chunk(ctx, slots[j]);
-            if (!r) r = rj;
+    for (auto &e : evs)
+        if (hipEventCreate(&e) != hipSuccess) {
+            cleanup();
+            return fail(ctx, SPC_ERR_HIP, "hipEventCreate");
         }
-        return r;
-    };
     hipEvent_t ev0 = evs[0];
     HIP_TRY(hipEventRecord(ev0, s));
     for (int j = 1; j < S; ++j) HIP_TRY(hipStreamWaitEvent(slots[j].s, ev0, 0));
     for (int c = 0; c < nchunks; ++c) {
-        OrientSlot &sl = slots[c % S];
-        if ((rc = finish_orientation_chunk(ctx, sl)) ||
-            (rc = launch_orientation_chunk(ctx, sl, sim, plan.m, sample_seeds, M, dirs, c * plan.opc,
-                                           std::min(plan.opc, n_orient - c * plan.opc), plan.graph, K, nout, d_acc,
-                                           evs[2 + 2 * c], evs[3 + 2 * c]))) {
-            finish_all();
-            destroy_events();
+        const int o0 = c * opc;
+        if ((rc = launch_orientation_chunk(ctx, slots[c % S], sim, plan, sh, M, c, o0, std::min(opc, n_orient - o0),
+                                           K, nout, d_acc, evs[1 + 2 * c], evs[2 + 2 * c]))) {
+            cleanup();
             return rc;
         }
     }
-    if ((rc = finish_all())) {
-        destroy_events();
-        return rc;
+    hipError_t es = hipSuccess;
+    for (int j = 0; j < S; ++j) {
+        const hipError_t e = hipStreamSynchronize(slots[j].s);
+        if (es == hipSuccess) es = e;
+    }
+    if (es != hipSuccess) {
+        cleanup();
+        return hip_fail(ctx, es, "hipStreamSynchronize");
     }
     std::vector<std::pair<double, double>> iv;
     for (int c = 0; c < nchunks; ++c) {
         float ta = 0.f, tb = 0.f;
-        if (hipEventElapsedTime(&ta, ev0, evs[2 + 2 * c]) != hipSuccess ||
-            hipEventElapsedTime(&tb, ev0, evs[3 + 2 * c]) != hipSuccess) {
-            destroy_events();
+        if (hipEventElapsedTime(&ta, ev0, evs[1 + 2 * c]) != hipSuccess ||
+            hipEventElapsedTime(&tb, ev0, evs[2 + 2 * c]) != hipSuccess) {
+            cleanup();
             return fail(ctx, SPC_ERR_HIP, "hipEventElapsedTime");
         }
         iv.emplace_back(ta, tb);
     }
-    destroy_events();
+    cleanup();
+    std::vector<int32_t> flags((size_t)4 * nchunks), bad((size_t)nchunks);
+    HIP_TRY(hipMemcpy(flags.data(), sh.flags, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bad.data(), sh.bad, sizeof(int32_t) * bad.size(), hipMemcpyDeviceToHost));
+    if (std::any_of(bad.begin(), bad.end(), [](int32_t b) { return b != 0; })) {
+        // an orientation's H did not match its Kronecker tables: the block product
+        BatchPlan p2 = plan;
+        p2.kron = false;
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms);
+    }
+    for (int c = 0; c < nchunks; ++c) {
+        if (flags[4 * c + 3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
+        if (flags[4 * c])
+            return fail(ctx, SPC_ERR_BREAKDOWN,
+                        "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+    }
     ps_out.assign((size_t)n_orient * nout, 0.0);
     std::vector<double> acc((size_t)n_orient * 4 * nout);
     HIP_TRY(hipMemcpy(acc.data(), d_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));

@@ -69,20 +69,23 @@ def test_chunking_does_not_change_results(ctx, opc):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_chunks_in_flight_do_not_change_results(ctx, monkeypatch, graph):
-    # 5 chunks of one orientation: with 4 streams a slot takes a second chunk
-    # once its first has finished; every chunk writes only its own sums, so
-    # the result is bit for bit that of one chunk at a time
+def test_chunks_in_flight_do_not_change_results(ctx, monkeypatch):
+    # 5 chunks of one orientation, then 7 orientations in chunks of 2 (the last
+    # one narrower): with 2 or 4 streams a slot takes further chunks, replaying
+    # its step graph between their eager set-up work; every chunk writes only
+    # its own sums, so each result is bit for bit that of eager launches, one
+    # chunk at a time
     w = _case(T_ns=12.0)
-    dirs = batch.fibonacci_sphere(5)
-    runs = {}
-    for n in (1, 2, 4):
-        monkeypatch.setenv("SPC_LANE_STREAMS", str(n))
-        runs[n] = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=1, graph=graph,
+    for n_dirs, opc in ((5, 1), (7, 2)):
+        dirs = batch.fibonacci_sphere(n_dirs)
+        monkeypatch.setenv("SPC_LANE_STREAMS", "1")
+        ref = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=opc, graph=False,
+                                       with_populations=True, ctx=ctx)
+        for n in (1, 2, 4):
+            monkeypatch.setenv("SPC_LANE_STREAMS", str(n))
+            got = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, orient_per_chunk=opc, graph=True,
                                            with_populations=True, ctx=ctx)
-    for n in (2, 4):
-        assert np.array_equal(runs[n][0], runs[1][0]) and np.array_equal(runs[n][1], runs[1][1]), n
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (n_dirs, opc, n)
 
 
 def test_more_trace_vectors_and_su_z(ctx):
