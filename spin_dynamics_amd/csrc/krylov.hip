@@ -1374,16 +1374,18 @@ int kron_slices(const KrylovWork &w)
 }
 
 // Waves per workgroup of the tiled product (k_spmm_kron_tiles; 0: the plain
-// lane slices): 8 by default, 16 from 24576 nuclear indices on (MI355X: 147 us
+// lane slices).  At Q = 2: 8, 16 from 24576 nuclear indices on (MI355X: 147 us
 // per product at N = 36864, M = 256 against 162 with 16 and 158 without tiles;
-// 423 us at N = 147456, M = 128 against 447 with 8), halved while the system
-// has fewer than Q + log2(NW) trailing spin-1/2 nuclei; SPC_KRON_TILE = 0, 4,
-// 8 or 16 overrides it (read at every launch).
+// 423 us at N = 147456, M = 128 against 447 with 8).  At Q = 3 (64-lane
+// chunks): 4 (17.0 / 51.6 / 189 us at N = 12288 / 49152 / 147456 against
+// 18.6 / 55.3 / 188 with 8).  Halved while the system has fewer than
+// Q + log2(NW) trailing spin-1/2 nuclei; SPC_KRON_TILE = 0, 4, 8 or 16
+// overrides it (read at every launch).
 int kron_tile(const KrylovWork &w)
 {
     const int q = kron_slices(w);
     if (q != 2 && q != 3) return 0;
-    int nw = w.Z >= 24576 ? 16 : 8;
+    int nw = q == 3 ? 4 : w.Z >= 24576 ? 16 : 8;
     if (const char *e = std::getenv("SPC_KRON_TILE")) nw = std::atoi(e);
     if (nw != 4 && nw != 8 && nw != 16) return 0;
     for (; nw >= 4; nw /= 2)
