@@ -566,7 +566,21 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     double *d_acc = nullptr;
     if ((rc = ensure(ctx, "acc", (size_t)4 * n_out, &d_acc))) return rc;
     HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * n_out, s));
-    hipEvent_t ev0, ev1, evr;
+    // on every return: the chunks' streams drained (their buffers are reused by
+    // the next call) and the events released
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evr = nullptr;
+    struct Drain {
+        LaneSet *sets;
+        int n;
+        hipEvent_t *evs[3];
+        ~Drain()
+        {
+            for (int j = 0; j < n; ++j)
+                if (sets[j].s) (void)hipStreamSynchronize(sets[j].s);
+            for (hipEvent_t *e : evs)
+                if (*e) hipEventDestroy(*e);
+        }
+    } drain{sets, nsets, {&ev0, &ev1, &evr}};
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
@@ -632,9 +646,6 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
-    hipEventDestroy(evr);
     ctx->last_ms = ms;
     ctx->last_rebuilds = rebuilds;
     ctx->last_steps = K;
