@@ -1687,8 +1687,11 @@ static int sc_common(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_samples, 
     p.nn = nn;
     p.n1 = sys->n_nuc1;
     p.lanes = lanes;
-    p.nl1 = (sys->n_nuc1 + lanes - 1) / lanes;
     p.slots = slots;
+    // electron-2 slots from slots/2 on when both electrons' nuclei fit a half
+    // (the kernel's compile-time layout); else right after electron 1's
+    const int need1 = (sys->n_nuc1 + lanes - 1) / lanes, need2 = (sys->n_nuc2 + lanes - 1) / lanes;
+    p.nl1 = (need1 <= slots / 2 && need2 <= slots / 2) ? slots / 2 : need1;
     p.dt = dt;
     const double shalf = std::sqrt(0.75);
     std::vector<double> nuc(128, 0.0); // [a | sqrt(I(I+1))] x 64, draw order

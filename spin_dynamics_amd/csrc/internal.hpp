@@ -90,7 +90,7 @@ constexpr int kScThreads = 256; // threads per block of the semiclassical kernel
 struct ScParams {
     int nn, n1;          // nuclei, of which the first n1 couple to electron 1
     int lanes;           // lanes per sample: 2, 4 or 8
-    int nl1;             // slots per lane for electron-1 nuclei: ceil(n1 / lanes)
+    int nl1;             // first electron-2 slot: slots/2 when both electrons fit a half, else ceil(n1 / lanes)
     int slots;           // slots per lane (kernel instantiation), from sc_slots
     const double *nuc;   // device [2][64]: a_k (mT), sqrt(I(I+1)) per nucleus in draw order
     const double *cs;    // device [slots][lanes][cos, sin] of a_k sqrt(3)/2 dt (1, 0 for empty slots)

@@ -275,8 +275,10 @@ __device__ __forceinline__ void axis_tan(const double *S, const double *dS, cons
 // samples per group).
 // With TAN the kernel also carries the tangent (dx, dI) of the state for the
 // field direction p.dw and adds (P_S(0) dP_S(t), P_S(0) dx_16(t)) to
-// partd[st][wave].
-template <int NL, int L, bool TAN>
+// partd[st][wave].  With BAL the electron-2 slots start at NL/2 (p.nl1 =
+// NL/2), known at compile time: the unrolled slot loops then pick each
+// slot's electron without selects.
+template <int NL, int L, bool TAN, bool BAL>
 __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__restrict__ seeds,
                                                    double *__restrict__ part, double *__restrict__ partd,
                                                    int64_t n_samples, int64_t n_steps, int64_t n_groups)
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
     const double wz[3] = {p.w[0], p.w[1], p.w[2]};
     const double dwz[3] = {p.dw[0], p.dw[1], p.dw[2]};
     const double th = 0.5 * p.dt;
-    const int nl1 = p.nl1;
+    const int nl1 = BAL ? NL / 2 : p.nl1;
 
     for (int64_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
         const int64_t sample = grp * (kScThreads / L) + threadIdx.x / L;
@@ -459,8 +461,12 @@ hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, double 
                   int64_t n_samples, int64_t n_steps, hipStream_t s)
 {
     const int64_t n_groups = (n_samples + kScThreads / L - 1) / (kScThreads / L);
-    hipLaunchKernelGGL((k_sc<NL, L, TAN>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd, n_samples,
-                       n_steps, n_groups);
+    if (2 * p.nl1 == NL)
+        hipLaunchKernelGGL((k_sc<NL, L, TAN, true>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd,
+                           n_samples, n_steps, n_groups);
+    else
+        hipLaunchKernelGGL((k_sc<NL, L, TAN, false>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd,
+                           n_samples, n_steps, n_groups);
     return hipGetLastError();
 }
 
@@ -469,7 +475,7 @@ hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, double 
 int sc_slots(int n1, int n2, int lanes, bool tangent)
 {
     const int need = (n1 + lanes - 1) / lanes + (n2 + lanes - 1) / lanes;
-    for (int nl : {2, 4, 8, 12, 16, 24, 32})
+    for (int nl : {2, 4, 6, 8, 12, 16, 24, 32})
         if (need <= nl && (tangent ? nl <= 8 : (lanes == 2 || nl <= 12))) return nl;
     return -1;
 }
@@ -497,13 +503,13 @@ hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nbl
     if (tan && p.slots == NL && p.lanes == L)                                                                      \
         e = launch<NL, L, true>(p, seeds, part, partd, nblocks, n_samples, n_steps, s);                           \
     else
-    SPC_SC_CASE(2, 2) SPC_SC_CASE(4, 2) SPC_SC_CASE(8, 2) SPC_SC_CASE(12, 2) SPC_SC_CASE(16, 2)
+    SPC_SC_CASE(2, 2) SPC_SC_CASE(4, 2) SPC_SC_CASE(6, 2) SPC_SC_CASE(8, 2) SPC_SC_CASE(12, 2) SPC_SC_CASE(16, 2)
     SPC_SC_CASE(24, 2) SPC_SC_CASE(32, 2)
-    SPC_SC_CASE(2, 4) SPC_SC_CASE(4, 4) SPC_SC_CASE(8, 4) SPC_SC_CASE(12, 4)
-    SPC_SC_CASE(2, 8) SPC_SC_CASE(4, 8) SPC_SC_CASE(8, 8) SPC_SC_CASE(12, 8)
-    SPC_SC_TCASE(2, 2) SPC_SC_TCASE(4, 2) SPC_SC_TCASE(8, 2)
-    SPC_SC_TCASE(2, 4) SPC_SC_TCASE(4, 4) SPC_SC_TCASE(8, 4)
-    SPC_SC_TCASE(2, 8) SPC_SC_TCASE(4, 8) SPC_SC_TCASE(8, 8)
+    SPC_SC_CASE(2, 4) SPC_SC_CASE(4, 4) SPC_SC_CASE(6, 4) SPC_SC_CASE(8, 4) SPC_SC_CASE(12, 4)
+    SPC_SC_CASE(2, 8) SPC_SC_CASE(4, 8) SPC_SC_CASE(6, 8) SPC_SC_CASE(8, 8) SPC_SC_CASE(12, 8)
+    SPC_SC_TCASE(2, 2) SPC_SC_TCASE(4, 2) SPC_SC_TCASE(6, 2) SPC_SC_TCASE(8, 2)
+    SPC_SC_TCASE(2, 4) SPC_SC_TCASE(4, 4) SPC_SC_TCASE(6, 4) SPC_SC_TCASE(8, 4)
+    SPC_SC_TCASE(2, 8) SPC_SC_TCASE(4, 8) SPC_SC_TCASE(6, 8) SPC_SC_TCASE(8, 8)
     return hipErrorInvalidValue;
 #undef SPC_SC_CASE
 #undef SPC_SC_TCASE
