@@ -351,6 +351,7 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
 struct LaneSet {
     KrylovWork w{};
     hipStream_t s = nullptr;
+    int idx = 0; // buffer-name suffix ("" for 0, "#idx" otherwise)
     int64_t *d_seeds = nullptr;
     double2 *d_inj = nullptr;
     VecInit v{};
@@ -381,6 +382,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     w.rpw2 = update_rows(Z, nch, nsets > 1 ? 256 : 512);
     w.rpwp = (int)std::max<int64_t>(1, (Z * nch + 4 * 1024 - 1) / (4 * 1024));
     w.qs = basis_stride(N, Mp);
+    L.idx = idx;
     const std::string sfx = idx ? "#" + std::to_string(idx) : "";
     auto nm = [&](const char *b) { return std::string(b) + sfx; };
     int rc;
@@ -467,11 +469,21 @@ int rebuild(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, bool gram)
 }
 
 // The projector-Gram pass of L's chunk (several steps per basis), run to the
-// end with a host check of the lane clocks after every rebuild.
+// end with a host check of the lane clocks after every rebuild.  Its Gram
+// matrices and their partial sums are allocated here when the chunk was set
+// up for direct mode (a tol = 0 lane that needed a second step).
 int gram_pass(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, const double pop[4], int64_t K, int64_t &rebuilds)
 {
-    int rc = start_pass(ctx, L, pop);
-    if (rc) return rc;
+    KrylovWork &w = L.w;
+    const std::string sfx = L.idx ? "#" + std::to_string(L.idx) : "";
+    const size_t part_need = std::max(partials_needed(w), (size_t)4 * w.m * w.m * w.Mp * 8);
+    int rc;
+    if (w.part_capacity < part_need) {
+        if ((rc = ensure(ctx, ("part" + sfx).c_str(), part_need, &w.part))) return rc;
+        w.part_capacity = part_need;
+    }
+    if (!w.gram && (rc = ensure(ctx, ("gram" + sfx).c_str(), (size_t)4 * w.m * w.m * w.Mp, &w.gram))) return rc;
+    if ((rc = start_pass(ctx, L, pop))) return rc;
     for (int64_t round = 0;; ++round) {
         if ((rc = rebuild(ctx, L, sim, true))) return rc;
         ++rebuilds;
@@ -1208,6 +1220,10 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         if (flags[4 * c])
             return fail(ctx, SPC_ERR_BREAKDOWN,
                         "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+        if (flags[4 * c + 1])
+            return fail(ctx, SPC_ERR_UNSUPPORTED,
+                        "a trace vector needed a second step from one Krylov basis (|c_m|^2 = 0 at tol = 0); "
+                        "the batched path takes one step per basis");
     }
     ps_out.assign((size_t)n_orient * nout, 0.0);
     std::vector<double> acc((size_t)n_orient * 4 * nout);
@@ -2041,6 +2057,10 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if (flags[0])
         return fail(ctx, SPC_ERR_BREAKDOWN,
                     "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
+    if (flags[1])
+        return fail(ctx, SPC_ERR_UNSUPPORTED,
+                    "a trace vector needed a second step from one Krylov basis (|c_m|^2 = 0 at tol = 0); "
+                    "the gradient takes one step per basis");
     std::vector<double> ps(nout);
     for (int64_t k = 0; k < nout; ++k) ps[k] = acc[k] / (double)M;
     double Funit = 0.0; // the yield per unit kS
