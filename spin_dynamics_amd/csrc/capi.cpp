@@ -1891,22 +1891,31 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         if (cw[n] != 0.0) ntop = n;
     const double wscale = ctx->kS / (double)M;
 
-    // memory: basis, 3 work vectors, J b-vectors, checkpoints every L steps, a
-    // segment of recomputed step vectors (psi_n and the a_n(s_j)), and the step
-    // vectors of the last `keep` steps kept from the first pass (as many as
-    // 80% of the free memory allows; those steps are not recomputed)
+    // The backward sweep recomputes the forward steps of each segment from its
+    // checkpoint.  With two streams, segment j - 1 is recomputed (forward
+    // workspace, stream B) while the adjoint steps of segment j run (their own
+    // workspace, the context stream); two segments of recomputed vectors are
+    // held at once.  Under the event profiler (or SPC_LANE_SERIAL=1) both run
+    // on the context stream, in the same order, with one segment buffer.
+    const bool overlap = !ctx->prof.on && !lane_serial() && lane_streams() > 1;
+    const int64_t nbuf = overlap ? 2 : 1;
+    // memory: basis, 3 work vectors, J b-vectors (and the adjoint's own basis and
+    // product vector when overlapped), checkpoints every L steps, nbuf segments
+    // of recomputed step vectors (psi_n and the a_n(s_j)), and the step vectors
+    // of the last `keep` steps kept from the first pass (as many as 80% of the
+    // free memory allows; those steps are not recomputed)
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    for (const char *nm : {"adj_store", "adj_ckpt"}) { // our own buffers are reusable
+    for (const char *nm : {"adj_store", "adj_ckpt", "adjb_Q"}) { // our own buffers are reusable
         auto it = ctx->bufs.find(nm);
         if (it != ctx->bufs.end()) free_b += it->second.bytes;
     }
     const size_t budget = free_b / 10 * 8;
     const size_t V = (size_t)N * Mp * sizeof(double2);
-    const size_t base = V * ((size_t)m + 3 + kGL);
+    const size_t base = V * ((size_t)m + 3 + kGL + (overlap ? (size_t)m + 1 : 0));
     int64_t L = checkpoint_every > 0 ? checkpoint_every : 64;
     auto need = [&](int64_t l, int64_t kp) {
-        return base + V * ((size_t)((ntop - kp + l - 1) / l) + (size_t)(l + kp) * (kGL + 1));
+        return base + V * ((size_t)((ntop - kp + l - 1) / l) + (size_t)(nbuf * l + kp) * (kGL + 1));
     };
     while (L > 1 && need(L, 0) > budget) L /= 2;
     if (need(L, 0) > budget) return fail(ctx, SPC_ERR_OOM, "not enough device memory for the adjoint");
@@ -1968,21 +1977,37 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
     if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, "obs", (size_t)nout * 4 * Mp, &w.obs))) return rc;
-    double2 *lam = nullptr, *bvec = nullptr, *ckpt = nullptr, *store = nullptr, *Cb = nullptr, *Sg = nullptr;
+    double2 *lam = nullptr, *bvec = nullptr, *ckpt = nullptr, *store = nullptr, *Cb = nullptr, *Cf = nullptr,
+            *Sg = nullptr;
     double *d_acc = nullptr;
     if ((rc = ensure(ctx, "adj_lam", (size_t)N * Mp, &lam))) return rc;
     if ((rc = ensure(ctx, "adj_b", (size_t)kGL * N * Mp, &bvec))) return rc;
     if ((rc = ensure(ctx, "adj_ckpt", (size_t)std::max<int64_t>(nseg, 1) * N * Mp, &ckpt))) return rc;
-    if ((rc = ensure(ctx, "adj_store", (size_t)(L + keep) * (kGL + 1) * N * Mp, &store))) return rc;
+    if ((rc = ensure(ctx, "adj_store", (size_t)(nbuf * L + keep) * (kGL + 1) * N * Mp, &store))) return rc;
     if ((rc = ensure(ctx, "adj_C", (size_t)kMaxTaus * m * Mp, &Cb))) return rc;
+    if ((rc = ensure(ctx, "adj_CF", (size_t)kMaxTaus * m * Mp, &Cf))) return rc;
     if ((rc = ensure(ctx, "adj_S", (size_t)Z * nslot * 16, &Sg))) return rc;
     if ((rc = ensure(ctx, "acc", (size_t)4 * nout, &d_acc))) return rc;
     KrylovWork wb = w; // the adjoint Krylov space: H', input lam
     set_ham(wb, ctx->adj);
     wb.psi = wb.u = lam;
+    if (overlap) { // its own basis and scratch
+        if ((rc = ensure(ctx, "adjb_Q", (size_t)m * wb.qs, &wb.Q))) return rc;
+        if ((rc = ensure(ctx, "adjb_v", (size_t)N * Mp, &wb.v))) return rc;
+        if ((rc = ensure(ctx, "adjb_part", part_need, &wb.part))) return rc;
+        if ((rc = ensure(ctx, "adjb_dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &wb.dots))) return rc;
+        if ((rc = ensure(ctx, "adjb_dots1", (size_t)2 * Mp, &wb.dots1))) return rc;
+        if ((rc = ensure(ctx, "adjb_coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &wb.coef))) return rc;
+        if ((rc = ensure(ctx, "adjb_tprev", (size_t)(SPC_MAX_KRYLOV + 1) * Mp, &wb.tprev))) return rc;
+        if ((rc = ensure(ctx, "adjb_hess", (size_t)m * m * Mp, &wb.hess))) return rc;
+        if ((rc = ensure(ctx, "adjb_E", (size_t)m * m * Mp, &wb.E))) return rc;
+        if ((rc = ensure(ctx, "adjb_c", (size_t)m * Mp, &wb.c))) return rc;
+        if ((rc = ensure(ctx, "adjb_beta0", (size_t)Mp, &wb.beta0))) return rc;
+        if ((rc = ensure(ctx, "adjb_flags", (size_t)4, &wb.flags))) return rc;
+    }
     const int64_t NMp = (int64_t)N * Mp;
     auto stored = [&](int64_t n, int o) {
-        const int64_t slot = n >= kstart ? L + (n - kstart) : n % L;
+        const int64_t slot = n >= kstart ? nbuf * L + (n - kstart) : ((n / L) % nbuf) * L + n % L;
         return store + (slot * (kGL + 1) + o) * NMp;
     };
     TauSet tf{}, tb{};
@@ -1997,6 +2022,28 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     for (int j = 0; j < kGL; ++j) tb.tau[1 + j] = (1.0 - kGLs[j]) * dt, tb.scale[1 + j] = kGLw[j];
 
     hipStream_t s = ctx->stream;
+    hipStream_t sr = s; // the recomputation's stream
+    if (overlap) {
+        if (!ctx->lane_stream[0]) HIP_TRY(hipStreamCreate(&ctx->lane_stream[0]));
+        sr = ctx->lane_stream[0];
+    }
+    // events: start and end of the run, then per segment "recomputed" and
+    // "adjoint steps done"; every exit drains the recomputation's stream
+    std::vector<hipEvent_t> evs((size_t)2 + 2 * (size_t)std::max<int64_t>(nseg, 1), nullptr);
+    struct Guard {
+        std::vector<hipEvent_t> &evs;
+        hipStream_t sr;
+        ~Guard()
+        {
+            if (sr) (void)hipStreamSynchronize(sr);
+            for (auto e : evs)
+                if (e) hipEventDestroy(e);
+        }
+    } guard{evs, sr};
+    for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t ev0 = evs[0], ev1 = evs[1];
+    auto ev_rec = [&](int64_t seg) { return evs[2 + 2 * seg]; };
+    auto ev_adj = [&](int64_t seg) { return evs[3 + 2 * seg]; };
     int64_t *d_seeds = nullptr;
     double2 *d_inj = nullptr;
     if (vi.kind == 2) {
@@ -2010,14 +2057,12 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     }
     double pop[4];
     theta_of(sim->init_state, vi.theta, pop);
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventRecord(ev0, s));
 
     // forward, with checkpoints every L steps and lam = omega P_S psi at ntop
     HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * nout, s));
     HIP_TRY(hipMemsetAsync(w.hess, 0, sizeof(double2) * m * m * Mp, s));
+    if (overlap) HIP_TRY(hipMemsetAsync(wb.hess, 0, sizeof(double2) * m * m * Mp, s));
     HIP_TRY(init_vectors(vi, (int)M, Mp, Z, w.psi, s));
     HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, s));
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
@@ -2069,6 +2114,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     // backward sweep, segment by segment
     HIP_TRY(hipMemsetAsync(Sg, 0, sizeof(double2) * (size_t)Z * nslot * 16, s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
+    if (overlap) HIP_TRY(hipMemsetAsync(wb.flags, 0, sizeof(int32_t) * 4, s));
     auto backward_step = [&](int64_t n) -> int { // lam_{n+1} -> lam_n, and the density of step n
         HIP_TRY(arnoldi(wb, s));
         HIP_TRY(expm_coeffs(wb, tb, Cb, s));
@@ -2080,38 +2126,56 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
         return SPC_OK;
     };
+    // forward steps of segment seg from its checkpoint, on the recomputation's
+    // stream: psi_n and a_n(s_j) into the segment's buffer
+    auto recompute = [&](int64_t seg) -> int {
+        const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, kstart);
+        HIP_TRY(hipMemcpyAsync(w.psi, ckpt + seg * NMp, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, sr));
+        for (int64_t n = n0; n < n1; ++n) {
+            HIP_TRY(arnoldi(w, sr));
+            HIP_TRY(expm_coeffs(w, tf, Cf, sr));
+            CombineOut out{};
+            out.n = kGL + 2;
+            for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
+            out.y[kGL + 1] = w.psi;
+            HIP_TRY(combine(w.Q, qstride(w), m, Cf, out, nullptr, 0.0, Z, (int)M, Mp, sr));
+        }
+        HIP_TRY(hipEventRecord(ev_rec(seg), sr));
+        return SPC_OK;
+    };
     if (backward) {
+        if (overlap) HIP_TRY(hipStreamWaitEvent(sr, ev0, 0)); // (the forward pass has drained)
+        if (nseg > 0 && (rc = recompute(nseg - 1))) return rc;
         for (int64_t n = ntop - 1; n >= kstart; --n)
             if ((rc = backward_step(n))) return rc;
         for (int64_t seg = nseg - 1; seg >= 0; --seg) {
             const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, kstart);
-            HIP_TRY(hipMemcpyAsync(w.psi, ckpt + seg * NMp, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
-            for (int64_t n = n0; n < n1; ++n) { // recompute: psi_n and a_n(s_j) kept
-                HIP_TRY(arnoldi(w, s));
-                HIP_TRY(expm_coeffs(w, tf, Cb, s));
-                CombineOut out{};
-                out.n = kGL + 2;
-                for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
-                out.y[kGL + 1] = w.psi;
-                HIP_TRY(combine(w.Q, qstride(w), m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+            if (sr != s) HIP_TRY(hipStreamWaitEvent(s, ev_rec(seg), 0));
+            if (seg > 0) {
+                // the buffer of segment seg - 1 was last read by the adjoint steps of
+                // segment seg + 1 (nbuf = 2) or seg (nbuf = 1: same stream, in order)
+                if (nbuf == 2 && seg + 1 < nseg) HIP_TRY(hipStreamWaitEvent(sr, ev_adj(seg + 1), 0));
+                if (overlap && (rc = recompute(seg - 1))) return rc;
             }
             for (int64_t n = n1 - 1; n >= n0; --n)
                 if ((rc = backward_step(n))) return rc;
+            HIP_TRY(hipEventRecord(ev_adj(seg), s));
+            if (!overlap && seg > 0 && (rc = recompute(seg - 1))) return rc;
         }
     }
     HIP_TRY(hipEventRecord(ev1, s));
     std::vector<double2> Sh((size_t)Z * nslot * 16);
     HIP_TRY(hipMemcpyAsync(Sh.data(), Sg, sizeof(double2) * Sh.size(), hipMemcpyDeviceToHost, s));
+    int32_t flags_b[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    if (overlap) HIP_TRY(hipMemcpyAsync(flags_b, wb.flags, sizeof(flags_b), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
     ctx->last_ms = ms;
     ctx->last_rebuilds = K + ntop + kstart;
     ctx->last_steps = K;
-    if (flags[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
+    if (flags[0] || flags_b[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
 
     double gf[3], gT[9], gkS, gkT;
     std::vector<double> gA((size_t)9 * std::max(nn, 1));

@@ -5,7 +5,8 @@ vectors, tol = 0) for every parameter of H; central finite differences of
 the oracle's forward (orc_trace_sampling with injected vectors) for a set of
 parameters, so the gradient is that of the reference algorithm's yield;
 independence of the checkpoint interval and of how many final steps keep
-their vectors instead of being recomputed; the yield itself equal to the
+their vectors instead of being recomputed; the overlapped recomputation
+equal to the one-stream sweep, bit for bit; the yield itself equal to the
 forward run's; and the torch.autograd.Function's chain rule to the
 (symmetric tensor, |B|, theta, phi, J, kS, kT) parameterisation, again
 against finite differences.  Tolerance: 1e-6 relative to the largest
@@ -149,6 +150,20 @@ def test_checkpoint_interval_does_not_change_the_gradient(ctx, monkeypatch, ever
     assert np.max(np.abs(g.hfc - ref.hfc)) <= 1e-12 * s
     assert np.max(np.abs(g.field - ref.field)) <= 1e-12 * s
     assert abs(g.kS - ref.kS) <= 1e-12 * max(abs(ref.kS), s)
+
+
+@pytest.mark.parametrize("every,keep", [(1, "0"), (7, "0"), (7, "13")])
+def test_overlapped_recomputation_equals_one_stream(ctx, monkeypatch, every, keep):
+    # segments recomputed on a second stream while the adjoint steps of the
+    # next segment run: bit for bit the gradient of one stream, one kernel at
+    # a time (SPC_LANE_SERIAL=1), with many segments and both buffer parities
+    w = _small(T_ns=50.0)
+    monkeypatch.setenv("SPC_ADJOINT_KEEP", keep)
+    a = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=every, ctx=ctx)
+    monkeypatch.setenv("SPC_LANE_SERIAL", "1")
+    b = gradients.yield_gradient(w.sys, w.sim, B=w.B, checkpoint_every=every, ctx=ctx)
+    for k in ("hfc", "field", "J", "D", "kS", "kT", "Phi_S"):
+        assert np.array_equal(np.asarray(getattr(a, k)), np.asarray(getattr(b, k))), k
 
 
 def test_odd_and_even_step_counts(ctx):
