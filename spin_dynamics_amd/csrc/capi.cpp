@@ -360,7 +360,7 @@ struct LaneSet {
 };
 
 // Device buffers of lane set L; set 0 uses the plain names (shared with the
-// other entry points), set 1 the same names with a suffix.
+// other entry points), sets 1-3 the same names with the suffix "#idx".
 int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, int64_t n_out, bool need_gram,
                    int vkind)
 {
