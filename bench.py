@@ -88,7 +88,7 @@ def side_workloads(ctx, a):
     from spin_dynamics_amd import batch, dynamics, gradients, sc_dynamics, workloads
     out = {}
     # adjoint gradient of the yield vs the forward run (step-uniform cost at tol = 0)
-    w4 = workloads.config2(M=64, m=24, T_ns=float(a.extra_steps) - 0.5)
+    w4 = workloads.anisotropic_36864(M=64, m=24, T_ns=float(a.extra_steps) - 0.5)
     dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)  # warm
     dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)
     fms, _, steps = dynamics.last_run_stats(ctx)
@@ -111,8 +111,8 @@ def side_workloads(ctx, a):
     for p in batch.radical_pairs():
         counts[p.N] = counts.get(p.N, 0) + 1
     for p in sorted(pairs, key=lambda q: q.N):
-        sim = batch.config3_sim(T_ns=float(steps_b) - 0.5)
-        sim2 = batch.config3_sim(T_ns=float(2 * steps_b) - 0.5)
+        sim = batch.orientation_sim(T_ns=float(steps_b) - 0.5)
+        sim2 = batch.orientation_sim(T_ns=float(2 * steps_b) - 0.5)
         plain = 0.0
         for d in dirs[:2]:
             # per-step cost: the difference of runs of 8 and 16 steps (the once-per-run
@@ -126,7 +126,7 @@ def side_workloads(ctx, a):
             plain += (ms2 - ms1) / (2 * (st2 - st1))
         # the order is tuned for the full run length (10 us: the pilot's tolerance is
         # scaled to it), the timing uses a few steps
-        opc, m, _ = batch.tune(p.sys, batch.config3_sim(), dirs, B=0.05, ctx=ctx)
+        opc, m, _ = batch.tune(p.sys, batch.orientation_sim(), dirs, B=0.05, ctx=ctx)
         s2 = copy.deepcopy(sim)
         s2.N_krylov = m
         batch.orientation_yields(p.sys, s2, dirs, B=0.05, orient_per_chunk=opc, ctx=ctx)
@@ -145,7 +145,7 @@ def side_workloads(ctx, a):
                   f"10 us run (1e-10 yield tolerance), device time of the chunks' steps; weighted by the 48 "
                   f"pairs' classes"}
     # semiclassical
-    w5 = workloads.config5()
+    w5 = workloads.semiclassical_48()
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **{**w5.kwargs(), "n_steps_": 10})  # warm
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **w5.kwargs())
     ms, _, st = dynamics.last_run_stats(ctx)
@@ -182,7 +182,7 @@ def main(argv=None):
 
     from spin_dynamics_amd import _lib, dynamics, workloads
     ctx = _lib.Context(0)
-    w = workloads.config2(M=a.M, m=a.krylov)
+    w = workloads.anisotropic_36864(M=a.M, m=a.krylov)
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
     lib = ctx.lib
     N, M, m, dt = H.N, a.M, a.krylov, w.sim.dt
@@ -259,7 +259,7 @@ def main(argv=None):
     line = {"metric": METRIC, "value": value, "unit": "state·steps/s", "n_gpus": 1, "steps": int(taken),
             "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
-            "config": {"workload": f"config2 trace sampling: N={N} (2x14N+6x1H | 4x1H), anisotropic HFC, "
+            "config": {"workload": f"trace sampling: N={N} (2x14N+6x1H | 4x1H), anisotropic HFC, "
                                    f"B=0.05 mT @37deg, J=0.01 mT, kS=2/kT=0.5 us^-1, M={M} random-phase vectors, "
                                    f"m={m}, dt=1 ns, tol=0", "N": N, "M": M, "krylov": m, "nnz": H.nnz,
                        "lane_chunks": chunks,

@@ -112,7 +112,7 @@ def radical_pairs(n_pairs: int = 48, seed=SYSTEM_SEED, J: float = 0.01, kS_us: f
     return pairs
 
 
-def config3_sim(M: int = 64, m: int = 24, T_ns: float = 10000.0) -> SimParam:
+def orientation_sim(M: int = 64, m: int = 24, T_ns: float = 10000.0) -> SimParam:
     """M = 64 phase vectors, Krylov order 24 (before tuning), dt = 1 ns and the
     plain 36 864-dimensional case's T = 10 us (the batched case states no T)."""
     return _sim(M, m, T_ns, 1.0, vec_kind="phase")

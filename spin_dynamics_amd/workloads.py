@@ -110,7 +110,7 @@ def _sim(M, m, T_ns, dt_ns, tol=0.0, vec_kind="phase", seed=(42, 99)) -> SimPara
                     vec_kind=vec_kind)
 
 
-def config1(M: int = 32, m: int = 20, T_ns: float = 5000.0, vec_kind: str = "phase") -> Workload:
+def isotropic_192(M: int = 32, m: int = 20, T_ns: float = 5000.0, vec_kind: str = "phase") -> Workload:
     """Plain trace sampling, CPU-runnable: e1 = 14N + 2x1H, e2 = 2x1H (N = 192);
     isotropic a_k = 2 unif - 1 mT drawn in nucleus order; B = 0.05 mT along z;
     J = D = 0; kS = kT = 1 us^-1; dt = 1 ns; T = 5 us."""
@@ -121,7 +121,7 @@ def config1(M: int = 32, m: int = 20, T_ns: float = 5000.0, vec_kind: str = "pha
                    J=0.0, kS=1.0 / GAMMA_E, kT=1.0 / GAMMA_E)
     B = 0.05
     sys.e1.w = sys.e2.w = -B
-    return Workload("config1", sys, _sim(M, m, T_ns, 1.0, vec_kind=vec_kind), B, 1.0, 1.0, T_ns, 1.0)
+    return Workload("isotropic_192", sys, _sim(M, m, T_ns, 1.0, vec_kind=vec_kind), B, 1.0, 1.0, T_ns, 1.0)
 
 
 def anisotropic_system(m1: Sequence[int], m2: Sequence[int], *, half_width=1.5, theta_deg=37.0, J=0.01,
@@ -136,13 +136,13 @@ def anisotropic_system(m1: Sequence[int], m2: Sequence[int], *, half_width=1.5, 
     return sys
 
 
-def config2(M: int = 256, m: int = 24, T_ns: float = 10000.0, vec_kind: str = "phase") -> Workload:
+def anisotropic_36864(M: int = 256, m: int = 24, T_ns: float = 10000.0, vec_kind: str = "phase") -> Workload:
     """Plain method at scale: e1 = 2x14N + 6x1H, e2 = 4x1H (N = 36 864); full
     anisotropic tensors (eigenvalues U[-1.5, 1.5] mT, random rotation); B = 0.05
     mT at 37 deg from z in the xz plane; J = 0.01 mT; kS = 2, kT = 0.5 us^-1;
     M = 256, m = 24, dt = 1 ns, T = 10 us, tol = 0."""
     sys = anisotropic_system([3, 3, 2, 2, 2, 2, 2, 2], [2, 2, 2, 2])
-    return Workload("config2", sys, _sim(M, m, T_ns, 1.0, vec_kind=vec_kind), 0.05, 2.0, 0.5, T_ns, 1.0)
+    return Workload("anisotropic_36864", sys, _sim(M, m, T_ns, 1.0, vec_kind=vec_kind), 0.05, 2.0, 0.5, T_ns, 1.0)
 
 
 def small_anisotropic(m1=(3, 2), m2=(2,), M=8, m=12, T_ns=200.0, tol=0.0, vec_kind="phase") -> Workload:
@@ -182,7 +182,7 @@ def gamma_like_couplings(rng: Xoroshiro, n: int, lo: float = 0.05, hi: float = 1
     return [lo + (hi - lo) * (x - gmin) / (gmax - gmin) for x in g]
 
 
-def config5(n_samples: int = 1 << 20, T_ns: float = 2000.0, dt_ns: float = 0.5, n_1h: int = 20,
+def semiclassical_48(n_samples: int = 1 << 20, T_ns: float = 2000.0, dt_ns: float = 0.5, n_1h: int = 20,
             n_14n: int = 4) -> ScWorkload:
     """Semiclassical Monte Carlo at large nuclear counts: per radical 20 1H
     then 4 14N, couplings from gamma_like_couplings (radical 1 then radical 2);
@@ -198,8 +198,8 @@ def config5(n_samples: int = 1 << 20, T_ns: float = 2000.0, dt_ns: float = 0.5, 
         m.append([2] * n_1h + [3] * n_14n)
     dt = dt_ns * GAMMA_E / 1000.0
     t_end = T_ns * GAMMA_E / 1000.0
-    return ScWorkload("config5", a[0], a[1], m[0], m[1], 0.05, (0.0, 0.0, 1.0), 0.0, 2.0 / GAMMA_E, 0.5 / GAMMA_E,
-                      n_samples, dt, math.ceil((t_end + dt) / dt))
+    return ScWorkload("semiclassical_48", a[0], a[1], m[0], m[1], 0.05, (0.0, 0.0, 1.0), 0.0, 2.0 / GAMMA_E,
+                      0.5 / GAMMA_E, n_samples, dt, math.ceil((t_end + dt) / dt))
 
 
 def write_ini(w: Workload, path: str, output_folder: str, *, simulation_type: str = "trace_sampling",

@@ -1,4 +1,4 @@
-"""Writes oracle outputs for the CPU-runnable workload (config 1: N = 192,
+"""Writes oracle outputs for the CPU-runnable workload (isotropic, N = 192,
 M = 32, m = 20, dt = 1 ns, T = 5 us, tol = 0) with both trace-vector kinds, so
 GPU parity tests compare against them without re-running the oracle.  The
 oracle itself is pinned by the reference fixtures and dense checks in tests/.
@@ -22,7 +22,7 @@ from helpers import oracle_system  # noqa: E402
 
 def main():
     for kind in ("phase", "SUZ"):
-        w = workloads.config1(vec_kind=kind)
+        w = workloads.isotropic_192(vec_kind=kind)
         s = oracle_system(w.sys, w.B)
         h = pyoracle.build_iso_csr(s)
         t0 = time.time()
@@ -30,7 +30,7 @@ def main():
                                       tol=0.0, vec_kind=1 if kind == "phase" else 0, seed=w.sim.seed)
         iden, phs, pht = pyoracle.get_kinetics(res["P_S"], res["P_T0"], res["P_Tp"], res["P_Tm"], w.sim.dt,
                                                s.kS, s.kT)
-        np.savez_compressed(os.path.join(HERE, f"oracle_config1_{kind}.npz"), **res, Phi_S=phs, Phi_T=pht)
+        np.savez_compressed(os.path.join(HERE, f"oracle_iso192_{kind}.npz"), **res, Phi_S=phs, Phi_T=pht)
         print(kind, f"{time.time() - t0:.1f}s", "Phi_S", phs[-1])
 
 

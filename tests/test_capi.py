@@ -91,7 +91,7 @@ def test_ini_comments_quotes_and_fortran_exponents(tmp_path):
 
 
 def test_workload_ini_round_trip(tmp_path):
-    w = workloads.config2(M=8)
+    w = workloads.anisotropic_36864(M=8)
     p = tmp_path / "w.ini"
     workloads.write_ini(w, str(p), str(tmp_path))
     sys_, sim = spinsys.read_inp(str(p))
@@ -125,7 +125,7 @@ def test_driver_reads_input_and_reports_missing_device(tmp_path):
     except ImportError:
         pass
     import subprocess
-    w = workloads.config1(M=4, T_ns=10.0)
+    w = workloads.isotropic_192(M=4, T_ns=10.0)
     out = tmp_path / "out"
     out.mkdir()
     ini = tmp_path / "in.ini"

@@ -31,11 +31,11 @@ def load(folder):
             ("S_prob", "T0_prob", "Tp_prob", "Tm_prob", "S_yield", "T_yield", "total_prob")}
 
 
-def test_driver_config1_matches_oracle(tmp_path):
-    w = workloads.config1(vec_kind="SUZ")
+def test_driver_isotropic_192_matches_oracle(tmp_path):
+    w = workloads.isotropic_192(vec_kind="SUZ")
     out = run_driver(w, tmp_path, fields=[0.05])
     got = load(out / ".050")  # F0.3 of 0.05, as the reference names the folder
-    ref = np.load(os.path.join(ROOT, "tests", "golden", "oracle_config1_SUZ.npz"))
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "oracle_iso192_SUZ.npz"))
     assert len(got["S_prob"]) == pyoracle.n_steps(w.sim.t_end, w.sim.dt) + 1
     for a, b in (("S_prob", "P_S"), ("T0_prob", "P_T0"), ("Tp_prob", "P_Tp"), ("Tm_prob", "P_Tm")):
         assert np.max(np.abs(got[a] - ref[b])) < 1e-10, a
@@ -76,7 +76,7 @@ def test_driver_exact_dynamics(tmp_path):
 def test_driver_semi_classical_matches_oracle(tmp_path):
     # simulation_type = semi_classical (unreachable in the reference driver):
     # isotropic parts of the tensors, N_samples classical samples from the seed
-    w = workloads.config1(M=3000, T_ns=20.0)
+    w = workloads.isotropic_192(M=3000, T_ns=20.0)
     out = run_driver(w, tmp_path, sim_type="semi_classical", fields=[0.05])
     folder = out / ".050"
     got = {k: np.loadtxt(os.path.join(folder, f"{k}.data")) for k in ("S_prob", "T_prob", "S_yield", "T_yield")}

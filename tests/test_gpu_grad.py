@@ -226,7 +226,7 @@ def test_gradient_independent_of_the_product_kernel(ctx, monkeypatch):
 def test_gradient_on_the_full_size_system_matches_fd(ctx):
     # the N = 36864 system of the gradient workload (2x14N + 6x1H | 4x1H,
     # anisotropic, tilted field), 8 trace vectors over 20 steps
-    w = workloads.config2(M=8, m=24, T_ns=20.0)
+    w = workloads.anisotropic_36864(M=8, m=24, T_ns=20.0)
     g = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)
     picks = [("hfc", (0, 0, 0)), ("hfc", (0, 1, 2)), ("hfc", (9, 2, 2)), ("field", 0), ("field", 2), ("J", None),
              ("kS", None), ("kT", None)]

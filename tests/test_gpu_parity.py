@@ -1,7 +1,7 @@
 """GPU parity: the HIP path against the CPU oracle and the dense model.
 
 Tolerances: Hamiltonian entries and single products to 1e-14 relative;
-populations and yields to 1e-10 relative (the bar BASELINE sets for yields)."""
+populations and yields to 1e-10 relative (the yield tolerance of the path)."""
 import numpy as np
 import pytest
 
@@ -18,7 +18,7 @@ def _dense(h):
 
 
 def test_hamiltonian_isotropic_matches_reference_builder(ctx):
-    w = workloads.config1()
+    w = workloads.isotropic_192()
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
     ref = pyoracle.build_iso_csr(oracle_system(w.sys, w.B))
     assert H.N == 192 and H.nnz <= ref.nnz
@@ -109,11 +109,11 @@ def test_trace_sampling_tol_gram_mode(ctx):
 
 
 @pytest.mark.parametrize("kind", ["phase", "SUZ"])
-def test_config1_vs_oracle_fixture(ctx, kind):
-    """Config 1 (N=192, M=32, m=20, T=5 us) with on-device trace vectors."""
+def test_isotropic_192_vs_oracle_fixture(ctx, kind):
+    """The isotropic N=192 workload (M=32, m=20, T=5 us) with on-device trace vectors."""
     import os
-    w = workloads.config1(vec_kind=kind)
-    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", f"oracle_config1_{kind}.npz"))
+    w = workloads.isotropic_192(vec_kind=kind)
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", f"oracle_iso192_{kind}.npz"))
     res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     for k in ("P_S", "P_T0", "P_Tp", "P_Tm"):
         assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
@@ -151,15 +151,15 @@ def test_errors_are_reported(ctx):
     assert ei.value.code == _lib.SPC_ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "trailing1", "config2_like", "trailing6"])
+@pytest.mark.parametrize("case", ["iso", "aniso_mixed", "trailing1", "aniso_36864_like", "trailing6"])
 def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
     # the Kronecker-form product (diagonal blocks + 2x2 flip tables) and the
     # 4x4-block product sum the same terms in the same order: bitwise equal,
     # also when a wave takes 2^q consecutive nuclear indices on 64 / 2^q lanes
-    # (q is capped by the trailing spin-1/2 nuclei: 4 for iso and config2_like,
+    # (q is capped by the trailing spin-1/2 nuclei: 4 for iso and aniso_36864_like,
     # 0 for aniso_mixed, 1 for trailing1, 6 for trailing6)
     if case == "iso":
-        w = workloads.config1(M=8, T_ns=20.0)
+        w = workloads.isotropic_192(M=8, T_ns=20.0)
     elif case == "aniso_mixed":
         w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3), M=8, T_ns=20.0)
     elif case == "trailing1":

@@ -64,7 +64,7 @@ def test_every_lane_split_matches_oracle(ctx, monkeypatch, lanes):
 
 
 def test_large_system_short_run_matches_oracle(ctx):
-    w = workloads.config5(n_samples=2048)
+    w = workloads.semiclassical_48(n_samples=2048)
     kw = {**w.kwargs(), "n_steps_": 20}
     (ps, pt, _, _), (ops, opt) = _both(w.a1, w.a2, w.m1, w.m2, ctx, **kw)
     assert max_rel(ps, ops) < 1e-10

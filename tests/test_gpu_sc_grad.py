@@ -79,8 +79,8 @@ def test_tangent_matches_oracle_finite_differences(ctx):
 
 
 def test_large_nuclear_set_uses_the_wide_lane_split(ctx):
-    # 24 + 24 nuclei (the config-5 layout): 8 lanes per sample, 6 slots each
-    w = workloads.config5(n_samples=1024)
+    # 24 + 24 nuclei (the 2^20-sample workload's layout): 8 lanes per sample, 6 slots each
+    w = workloads.semiclassical_48(n_samples=1024)
     d = _unit([0.2, 0.4, 0.9])
     kw = dict(n_samples=1024, dt=w.dt, n_steps_=120, J=0.0, kS=w.kS, kT=w.kT, seed=(3, 4))
     args = (w.a1, w.a2, w.m1, w.m2)

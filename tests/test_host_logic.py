@@ -51,7 +51,7 @@ def test_symmetric_tensor_parameterisation_round_trip():
         assert np.array_equal(A[:, a, b], h6[:, q])
     n = gradients.field_direction(0.7, 1.9)
     assert math.isclose(np.linalg.norm(n), 1.0, rel_tol=1e-15)
-    prob = gradients.YieldProblem(mult1=[3, 2], mult2=[2, 2], sim=workloads.config1().sim)
+    prob = gradients.YieldProblem(mult1=[3, 2], mult2=[2, 2], sim=workloads.isotropic_192().sim)
     sys = prob.system(h6, 0.3, 0.7, 1.9, 0.01, 2.0 / GAMMA_E, 0.5 / GAMMA_E)
     assert sys.e1.g_I == [3, 2] and sys.e2.g_I == [2, 2]
     assert np.array_equal(sys.e2.A[1], A[3]) and sys.e1.w == -0.3
@@ -59,7 +59,7 @@ def test_symmetric_tensor_parameterisation_round_trip():
 
 
 def test_semiclassical_workload():
-    w = workloads.config5(n_samples=1024)
+    w = workloads.semiclassical_48(n_samples=1024)
     assert len(w.a1) == len(w.a2) == 24
     assert w.m1 == [2] * 20 + [3] * 4 and w.m2 == w.m1
     for a in (w.a1, w.a2):
@@ -69,7 +69,7 @@ def test_semiclassical_workload():
     assert math.isclose(w.dt, 0.5 * GAMMA_E / 1000)
 
 
-def test_config3_simulation_parameters():
-    sim = batch.config3_sim()
+def test_orientation_simulation_parameters():
+    sim = batch.orientation_sim()
     assert sim.N_samples == 64 and sim.N_krylov == 24 and sim.vec_kind == "phase"
     assert math.isclose(sim.dt, GAMMA_E / 1000) and math.isclose(sim.t_end, 10 * GAMMA_E)
