@@ -99,9 +99,10 @@ struct ScParams {
     double dw[3];        // its tangent (the tangent kernel only)
     double dt;
 };
-// Slots per lane (2 .. 32 for 2 lanes, 2 .. 12 for 4 or 8, and 2 .. 8 for the
-// tangent kernel, which carries twice the state) for n1 + n2 nuclei split over
-// `lanes` lanes; -1 if more are needed.
+// Slots per lane (2 .. 32 for 2 lanes, 2 .. 12 for 4 or 8; for the tangent
+// kernel, which carries twice the state, 2 .. 8, or 12 when each electron's
+// nuclei fit half of them) for n1 + n2 nuclei split over `lanes` lanes; -1 if
+// more are needed.
 int sc_slots(int n1, int n2, int lanes, bool tangent = false);
 // part: 2 (n_steps + 1) nblocks kScThreads/64 doubles (zeroed here); ps, pt:
 // n_steps + 1 values of P_S(t), P_T(t).  The summation order depends on nblocks.

@@ -461,12 +461,19 @@ hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, double 
                   int64_t n_samples, int64_t n_steps, hipStream_t s)
 {
     const int64_t n_groups = (n_samples + kScThreads / L - 1) / (kScThreads / L);
-    if (2 * p.nl1 == NL)
+    const bool bal = 2 * p.nl1 == NL;
+    if constexpr (TAN && NL > 8) { // only the balanced layout fits the registers (sc_slots)
+        if (!bal) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_sc<NL, L, TAN, true>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd,
                            n_samples, n_steps, n_groups);
-    else
-        hipLaunchKernelGGL((k_sc<NL, L, TAN, false>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part, partd,
-                           n_samples, n_steps, n_groups);
+    } else {
+        if (bal)
+            hipLaunchKernelGGL((k_sc<NL, L, TAN, true>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part,
+                               partd, n_samples, n_steps, n_groups);
+        else
+            hipLaunchKernelGGL((k_sc<NL, L, TAN, false>), dim3(nblocks), dim3(kScThreads), 0, s, p, seeds, part,
+                               partd, n_samples, n_steps, n_groups);
+    }
     return hipGetLastError();
 }
 
@@ -474,9 +481,14 @@ hipError_t launch(const ScParams &p, const int64_t *seeds, double *part, double 
 
 int sc_slots(int n1, int n2, int lanes, bool tangent)
 {
-    const int need = (n1 + lanes - 1) / lanes + (n2 + lanes - 1) / lanes;
-    for (int nl : {2, 4, 6, 8, 12, 16, 24, 32})
-        if (need <= nl && (tangent ? nl <= 8 : (lanes == 2 || nl <= 12))) return nl;
+    const int need1 = (n1 + lanes - 1) / lanes, need2 = (n2 + lanes - 1) / lanes, need = need1 + need2;
+    for (int nl : {2, 4, 6, 8, 12, 16, 24, 32}) {
+        if (need > nl) continue;
+        // the tangent kernel carries twice the state: up to 8 slots, or 12 in
+        // the balanced layout (each electron's nuclei in one half)
+        const bool fits = tangent ? (nl <= 8 || (nl == 12 && need1 <= 6 && need2 <= 6)) : (lanes == 2 || nl <= 12);
+        if (fits) return nl;
+    }
     return -1;
 }
 
@@ -507,9 +519,9 @@ hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nbl
     SPC_SC_CASE(24, 2) SPC_SC_CASE(32, 2)
     SPC_SC_CASE(2, 4) SPC_SC_CASE(4, 4) SPC_SC_CASE(6, 4) SPC_SC_CASE(8, 4) SPC_SC_CASE(12, 4)
     SPC_SC_CASE(2, 8) SPC_SC_CASE(4, 8) SPC_SC_CASE(6, 8) SPC_SC_CASE(8, 8) SPC_SC_CASE(12, 8)
-    SPC_SC_TCASE(2, 2) SPC_SC_TCASE(4, 2) SPC_SC_TCASE(6, 2) SPC_SC_TCASE(8, 2)
-    SPC_SC_TCASE(2, 4) SPC_SC_TCASE(4, 4) SPC_SC_TCASE(6, 4) SPC_SC_TCASE(8, 4)
-    SPC_SC_TCASE(2, 8) SPC_SC_TCASE(4, 8) SPC_SC_TCASE(6, 8) SPC_SC_TCASE(8, 8)
+    SPC_SC_TCASE(2, 2) SPC_SC_TCASE(4, 2) SPC_SC_TCASE(6, 2) SPC_SC_TCASE(8, 2) SPC_SC_TCASE(12, 2)
+    SPC_SC_TCASE(2, 4) SPC_SC_TCASE(4, 4) SPC_SC_TCASE(6, 4) SPC_SC_TCASE(8, 4) SPC_SC_TCASE(12, 4)
+    SPC_SC_TCASE(2, 8) SPC_SC_TCASE(4, 8) SPC_SC_TCASE(6, 8) SPC_SC_TCASE(8, 8) SPC_SC_TCASE(12, 8)
     return hipErrorInvalidValue;
 #undef SPC_SC_CASE
 #undef SPC_SC_TCASE

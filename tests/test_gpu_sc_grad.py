@@ -79,7 +79,8 @@ def test_tangent_matches_oracle_finite_differences(ctx):
 
 
 def test_large_nuclear_set_uses_the_wide_lane_split(ctx):
-    # 24 + 24 nuclei (the 2^20-sample workload's layout): 8 lanes per sample, 6 slots each
+    # 24 + 24 nuclei (the 2^20-sample workload's layout): 4 lanes per sample, 12 slots
+    # each, electron 2 from slot 6 on (the balanced layout the 12-slot tangent needs)
     w = workloads.semiclassical_48(n_samples=1024)
     d = _unit([0.2, 0.4, 0.9])
     kw = dict(n_samples=1024, dt=w.dt, n_steps_=120, J=0.0, kS=w.kS, kT=w.kT, seed=(3, 4))
