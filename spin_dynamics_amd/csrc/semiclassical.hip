@@ -72,10 +72,10 @@ __device__ __forceinline__ Rot make_rot(const double *f, double t)
     Rot r;
     const double w = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
     const bool z = !(w > 0.0);
-    const double ws = z ? 1.0 : w;
-    r.h[0] = z ? 0.0 : f[0] / ws;
-    r.h[1] = z ? 0.0 : f[1] / ws;
-    r.h[2] = z ? 1.0 : f[2] / ws;
+    const double iw = 1.0 / (z ? 1.0 : w); // one division for the three components
+    r.h[0] = z ? 0.0 : f[0] * iw;
+    r.h[1] = z ? 0.0 : f[1] * iw;
+    r.h[2] = z ? 1.0 : f[2] * iw;
     double sn, cs;
     sincos(z ? 0.0 : w * t, &sn, &cs);
     r.c = cs;
@@ -362,9 +362,9 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                 const double n1 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
                 const double n2 = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
                 const bool z1 = !(n1 > 0.0), z2 = !(n2 > 0.0);
-                const double d1 = z1 ? 1.0 : n1, d2 = z2 ? 1.0 : n2;
-                const double h1[3] = {x[0] / d1, x[1] / d1, x[2] / d1};
-                const double h2[3] = {x[3] / d2, x[4] / d2, x[5] / d2};
+                const double i1 = 1.0 / (z1 ? 1.0 : n1), i2 = 1.0 / (z2 ? 1.0 : n2);
+                const double h1[3] = {x[0] * i1, x[1] * i1, x[2] * i1};
+                const double h2[3] = {x[3] * i2, x[4] * i2, x[5] * i2};
                 if constexpr (TAN) {
                     double u1[3], v1[3], u2[3], v2[3];
                     axis_tan(x + 0, dx + 0, h1, n1, z1, u1, v1);
