@@ -1,0 +1,55 @@
+"""GPU: the trace-sampling workload at its full size (N = 36 864, full
+anisotropic tensors, tilted field, M = 256 random-phase vectors run as four
+concurrent 64-lane chunks, m = 24).
+
+Pins at this size: the first two steps against the oracle on all 256 vectors
+(1e-10 relative, the yield tolerance); the exact invariant of equal rates
+(kS = kT = k makes the Haberkorn term a multiple of the identity, so every
+vector's total population decays as exp(-k t) whatever the rest of H is; to
+1e-12 over 40 steps); and the four-chunk result equal to the one-chunk result
+to rounding (1e-12)."""
+import numpy as np
+import pytest
+
+from helpers import max_rel, oracle_system
+from oracle import pyoracle
+from spin_dynamics_amd import dynamics, workloads
+from spin_dynamics_amd.spinsys import GAMMA_E
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("P_S", "P_T0", "P_Tp", "P_Tm")
+
+
+def test_first_steps_match_the_oracle(ctx):
+    w = workloads.anisotropic_36864(M=256, m=24, T_ns=1.5)  # two steps
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    _, rebuilds, steps = dynamics.last_run_stats(ctx)
+    assert steps == 2 and rebuilds == 4 * steps  # four lane chunks, one rebuild each per step
+    s = oracle_system(w.sys, w.B)
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=256, m=24, t_end=w.sim.t_end, dt=w.sim.dt,
+                                  vec_kind=1, seed=w.sim.seed)
+    for k in KEYS:
+        assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
+
+
+def test_equal_rates_decay_exactly(ctx):
+    w = workloads.anisotropic_36864(M=256, m=24, T_ns=39.5)  # 40 steps
+    k = 1.5 / GAMMA_E
+    w.sys.kS = w.sys.kT = k
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    _, _, K = dynamics.last_run_stats(ctx)
+    tot = sum(np.asarray(getattr(res, x)) for x in KEYS)[:K + 1]
+    t = np.zeros(K + 1)
+    for i in range(1, K + 1):
+        t[i] = t[i - 1] + w.sim.dt
+    assert np.max(np.abs(tot - np.exp(-k * t))) < 1e-12
+
+
+def test_four_chunks_equal_one_chunk(ctx, monkeypatch):
+    w = workloads.anisotropic_36864(M=256, m=24, T_ns=4.5)
+    four = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    monkeypatch.setenv("SPC_LANE_STREAMS", "1")
+    one = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    for k in KEYS:
+        assert max_rel(getattr(four, k), getattr(one, k)) < 1e-12, k
