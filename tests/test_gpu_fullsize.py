@@ -6,14 +6,15 @@ Pins at this size: the first two steps against the oracle on all 256 vectors
 (1e-10 relative, the yield tolerance); the exact invariant of equal rates
 (kS = kT = k makes the Haberkorn term a multiple of the identity, so every
 vector's total population decays as exp(-k t) whatever the rest of H is; to
-1e-12 over 40 steps); and the four-chunk result equal to the one-chunk result
-to rounding (1e-12)."""
+1e-12 over 40 steps); the four-chunk result equal to the one-chunk result to
+rounding (1e-12); and, for the largest batched pair (N = 147 456), every
+orientation's yield from the batched path equal to its plain run (1e-12)."""
 import numpy as np
 import pytest
 
 from helpers import max_rel, oracle_system
 from oracle import pyoracle
-from spin_dynamics_amd import dynamics, workloads
+from spin_dynamics_amd import batch, dynamics, workloads
 from spin_dynamics_amd.spinsys import GAMMA_E
 
 pytestmark = pytest.mark.gpu
@@ -53,3 +54,17 @@ def test_four_chunks_equal_one_chunk(ctx, monkeypatch):
     one = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     for k in KEYS:
         assert max_rel(getattr(four, k), getattr(one, k)) < 1e-12, k
+
+
+def test_largest_batched_pair_matches_plain_runs(ctx):
+    import copy
+    p = [q for q in batch.radical_pairs() if q.N == 147456][0]
+    sim = batch.orientation_sim(T_ns=3.5)  # four steps, m = 24
+    dirs = batch.fibonacci_sphere(3)
+    phi = batch.orientation_yields(p.sys, sim, dirs, B=0.05, ctx=ctx)
+    for o, d in enumerate(dirs):
+        s = copy.deepcopy(p.sys)
+        s.field_dir = tuple(d)
+        res = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
+        res.get_kinetics(sim.dt, s.kS, s.kT)
+        assert abs(phi[o] - res.Phi_S[-1]) <= 1e-12 * abs(res.Phi_S[-1]), o
