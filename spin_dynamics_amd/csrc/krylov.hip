@@ -1369,7 +1369,7 @@ int kron_slices(const KrylovWork &w)
 {
     if (!w.kron) return 0;
     int q = w.Mp <= kWave ? 3 : 2;
-    if (const char *e = std::getenv("SPC_KRON_Q")) q = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("SPC_KRON_Q")) q = std::min(3, std::max(0, std::atoi(e))); // kernels for 0..3
     return std::min(q, w.kq);
 }
 
