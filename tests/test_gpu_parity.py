@@ -187,13 +187,15 @@ def test_kronecker_product_equals_block_product(ctx, monkeypatch, case):
         assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, q
         if q == "0":
             assert np.array_equal(ra.P_S, rb.P_S) and np.array_equal(ra.P_Tp, rb.P_Tp)
-    # workgroups of 4, 8 or 16 waves on consecutive quads (where the system
-    # has 2 + log2(waves) trailing spin-1/2 nuclei; otherwise the plain slices)
-    monkeypatch.setenv("SPC_KRON_Q", "2")
-    for nw in ("0", "4", "8", "16"):
-        monkeypatch.setenv("SPC_KRON_TILE", nw)
-        assert np.array_equal(H.spmm(X), b), nw
-        assert np.array_equal(H.spmm(Xw), bw), nw
-        ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
-        assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, nw
+    # workgroups of 4, 8 or 16 waves on consecutive wave blocks (where the
+    # system has q + log2(waves) trailing spin-1/2 nuclei; otherwise fewer
+    # waves, then the plain slices)
+    for q in ("2", "3"):
+        monkeypatch.setenv("SPC_KRON_Q", q)
+        for nw in ("0", "4", "8", "16"):
+            monkeypatch.setenv("SPC_KRON_TILE", nw)
+            assert np.array_equal(H.spmm(X), b), (q, nw)
+            assert np.array_equal(H.spmm(Xw), bw), (q, nw)
+            ra = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+            assert max_rel(ra.P_S, rb.P_S) < 1e-12 and max_rel(ra.P_Tp, rb.P_Tp) < 1e-12, (q, nw)
     assert max_rel(b, X @ H.scipy().toarray().T) < 1e-13
