@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
         kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            v[rowb + toff + e * Mp] = y[e];
+            ntstore(&v[rowb + toff + e * Mp], y[e]); // streamed past L2, which keeps u's rows
             acc[0] = cfma_conj(x0[e], y[e], acc[0]);
             acc[1].x += cabs2(x0[e]);
         }
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            v[rowb + toff + e * Mp] = y[e];
+            ntstore(&v[rowb + toff + e * Mp], y[e]); // streamed past L2, which keeps u's rows
             acc[0] = cfma_conj(x0[e], y[e], acc[0]);
             acc[1].x += cabs2(x0[e]);
         }
