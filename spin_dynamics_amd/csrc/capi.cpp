@@ -46,6 +46,8 @@ struct HamDev {
     KronDev *kron = nullptr; // Kronecker form (when H has it exactly; else null)
     double2 *kdiag = nullptr, *kgen = nullptr;
     int kq = 0; // in-wave nuclei of the Kronecker product (kron_inwave)
+    int slow_mult = 1;        // multiplicity of the slowest nucleus (digit of largest stride)
+    int64_t slow_stride = 0;  // its stride in nuclear indices
     uint64_t *kmask = nullptr; // [Z][3] neighbour masks of the Kronecker product
     void release()
     {
@@ -265,6 +267,8 @@ void set_ham(KrylovWork &w, const HamDev &h)
     w.kgen = kr ? h.kgen : nullptr;
     w.kdiag_stride = w.kgen_stride = 0;
     w.kq = kr ? h.kq : 0;
+    w.slow_mult = h.slow_mult;
+    w.slow_stride = h.slow_stride;
     w.kmask = kr ? h.kmask : nullptr;
 }
 
@@ -322,6 +326,8 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     // the Kronecker form used by the product
     const KronDev hk = kron_of(S);
     h.kq = kron_inwave(hk);
+    h.slow_mult = hk.nn > 0 ? hk.mult[0] : 1;
+    h.slow_stride = hk.nn > 0 ? hk.stride[0] : 0;
     HIP_TRY(hipMalloc(&h.kron, sizeof(KronDev)));
     HIP_TRY(hipMemcpyAsync(h.kron, &hk, sizeof(KronDev), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMalloc(&h.kdiag, sizeof(double2) * 16 * Z));

@@ -177,6 +177,8 @@ struct KrylovWork {
     const double2 *kgen;     // [nn][2][3][4]
     int64_t kdiag_stride, kgen_stride;
     int kq;                  // trailing spin-1/2 nuclei of strides 1, 2, 4, .. (kron_inwave, 0..8)
+    int slow_mult;           // multiplicity and stride (nuclear indices) of the slowest nucleus
+    int64_t slow_stride;
     const uint64_t *kmask;   // [Z][3] neighbour masks (kron_masks), or null
     double2 *Q;      // [m] basis vectors [N][Mp], qs elements apart (qstride)
     int64_t qs;      // basis vector stride in elements (0: N*Mp)

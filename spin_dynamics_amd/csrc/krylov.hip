@@ -472,7 +472,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
                                                             int64_t gstride, int chunks_per_set,
                                                             const double2 *__restrict__ u, double2 *__restrict__ v,
                                                             double2 *__restrict__ part, int64_t Z, int Mp, int rpg,
-                                                            int nrb)
+                                                            int nrb, int lslow)
 {
     constexpr int W = kWave >> Q;
     constexpr int NG = 1 << Q;
@@ -505,7 +505,10 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
     const int64_t Zt = Z >> (Q + BD);
     const int64_t t0 = (int64_t)rb * rpg;
     const int64_t t1 = t0 + rpg < Zt ? t0 + rpg : Zt;
-    for (int64_t ti = t0; ti < t1; ++ti) {
+    const int64_t zs = Zt / lslow; // tiles per value of the slowest digits
+    for (int64_t tj = t0; tj < t1; ++tj) {
+        // work order: the slowest digits (lslow combinations) vary fastest
+        const int64_t ti = (tj % lslow) * zs + tj / lslow;
         const int nb = (int)((((ti << BD) + wave)) << Q); // this wave's quad
         const int64_t rowb = (int64_t)nb * M4;
 #pragma unroll
@@ -1393,6 +1396,21 @@ int kron_tile(const KrylovWork &w)
     return 0;
 }
 
+// Work order of the tiles: the slowest nucleus's digits vary fastest when its
+// flip partners lie further apart than an XCD's L2 holds of the XCD's lane
+// slice (stride x 4 rows x W lanes x 16 B > 4 MiB), so that both partners of
+// a flip are read while in L2; returns the number of its digits (1: natural
+// order).  N = 147456 (stride 12288): 155 against 184 us per 64-lane product;
+// smaller strides gain nothing.  SPC_KRON_ORDER=0 keeps the natural order.
+int tile_order(const KrylovWork &w, int fast, int lanes)
+{
+    if (const char *e = std::getenv("SPC_KRON_ORDER"))
+        if (std::atoi(e) == 0) return 1;
+    const int l = w.slow_mult;
+    if (l < 2 || w.slow_stride * 4 * lanes * (int64_t)sizeof(double2) <= (int64_t)4 << 20) return 1;
+    return ((w.Z >> fast) % l) == 0 ? l : 1;
+}
+
 // Tiles per workgroup of the tiled product: about 4096 workgroups in all.
 inline int tile_rpg(const KrylovWork &w, int nw)
 {
@@ -1426,7 +1444,8 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
         const int q = kron_slices(w);
         if (const int nw = kron_tile(w)) {
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave, rpg = tile_rpg(w, nw);
-            using TileFn = KronFn;
+            using TileFn = void (*)(const KronDev *, const uint64_t *, const double2 *, const double2 *, int64_t,
+                                    int64_t, int, const double2 *, double2 *, double2 *, int64_t, int, int, int, int);
             static const TileFn tfn[2][3][2] = {
                 {{k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>},
                  {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>},
@@ -1438,7 +1457,7 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
             hipLaunchKernelGGL(tfn[q - 2][ti][std::min(group, 3) - 2], dim3(nrb * nsl), dim3(64 * nw), 0, s, w.kron,
                                w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
-                               w.Mp, rpg, nrb);
+                               w.Mp, rpg, nrb, tile_order(w, q + (nw == 16 ? 4 : nw == 8 ? 3 : 2), kWave >> q));
             return hipGetLastError();
         }
         if (q > 0) {

@@ -8,7 +8,8 @@ Pins at this size: the first two steps against the oracle on all 256 vectors
 vector's total population decays as exp(-k t) whatever the rest of H is; to
 1e-12 over 40 steps); the four-chunk result equal to the one-chunk result to
 rounding (1e-12); and, for the largest batched pair (N = 147 456), every
-orientation's yield from the batched path equal to its plain run (1e-12)."""
+orientation's yield from the batched path equal to its plain run (1e-12) and
+its product the same in either tile work order (bit for bit)."""
 import numpy as np
 import pytest
 
@@ -68,3 +69,17 @@ def test_largest_batched_pair_matches_plain_runs(ctx):
         res = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
         res.get_kinetics(sim.dt, s.kS, s.kT)
         assert abs(phi[o] - res.Phi_S[-1]) <= 1e-12 * abs(res.Phi_S[-1]), o
+
+
+def test_slow_digit_tile_order_gives_the_same_product(ctx, monkeypatch):
+    # at N = 147456 the slowest nucleus (spin 1, stride 12288) has its digits
+    # varied fastest in the product's work order; each tile's rows are summed
+    # in the same order either way, so the products are bit-identical
+    p = [q for q in batch.radical_pairs() if q.N == 147456][0]
+    H = dynamics.Hamiltonian(p.sys, 0.05, ctx)
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(64, H.N)) + 1j * rng.normal(size=(64, H.N))
+    a = H.spmm(X)
+    monkeypatch.setenv("SPC_KRON_ORDER", "0")
+    b = H.spmm(X)
+    assert np.array_equal(a, b)
