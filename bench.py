@@ -113,6 +113,9 @@ def side_workloads(ctx, a):
     for p in sorted(pairs, key=lambda q: q.N):
         sim = batch.orientation_sim(T_ns=float(steps_b) - 0.5)
         sim2 = batch.orientation_sim(T_ns=float(2 * steps_b) - 0.5)
+        warm = copy.deepcopy(p.sys)
+        warm.field_dir = tuple(dirs[0])
+        dynamics.trace_sampling_para(warm, batch.orientation_sim(T_ns=1.5), B=0.05, ctx=ctx)  # first launches
         plain = 0.0
         for d in dirs[:2]:
             # per-step cost: the difference of runs of 8 and 16 steps (the once-per-run
