@@ -167,7 +167,7 @@ def side_workloads(ctx, a):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--M", type=int, default=256)
     ap.add_argument("--krylov", type=int, default=24)
