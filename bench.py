@@ -112,13 +112,13 @@ def side_workloads(ctx, a):
         counts[p.N] = counts.get(p.N, 0) + 1
     for p in sorted(pairs, key=lambda q: q.N):
         sim = batch.orientation_sim(T_ns=float(steps_b) - 0.5)
-        sim2 = batch.orientation_sim(T_ns=float(2 * steps_b) - 0.5)
+        sim2 = batch.orientation_sim(T_ns=float(3 * steps_b) - 0.5)
         warm = copy.deepcopy(p.sys)
         warm.field_dir = tuple(dirs[0])
         dynamics.trace_sampling_para(warm, batch.orientation_sim(T_ns=1.5), B=0.05, ctx=ctx)  # first launches
         plain = 0.0
         for d in dirs[:2]:
-            # per-step cost: the difference of runs of 8 and 16 steps (the once-per-run
+            # per-step cost: the difference of runs of 8 and 24 steps (the once-per-run
             # vector set-up cancels, as the batched time counts only the chunks' steps)
             s = copy.deepcopy(p.sys)
             s.field_dir = tuple(d)
@@ -143,7 +143,7 @@ def side_workloads(ctx, a):
     out["batched_orientations"] = {
         "speedup_weighted": round(t_plain / t_batch, 2), "classes": classes,
         "sample": f"first pair of each dimension class, 96 orientations, M=64, {steps_b} steps; plain = one "
-                  f"spc_trace_sample per orientation (m=24), per step from runs of {steps_b} and {2 * steps_b} "
+                  f"spc_trace_sample per orientation (m=24), per step from runs of {steps_b} and {3 * steps_b} "
                   f"steps; batched = spc_orientation_batch with the order and width spc_batch_tune picks for the "
                   f"10 us run (1e-10 yield tolerance), device time of the chunks' steps; weighted by the 48 "
                   f"pairs' classes"}
