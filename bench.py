@@ -227,7 +227,8 @@ def main(argv=None):
         dom = max(streaming, key=lambda k: streaming[k]["ms"])
         f = streaming[dom]
         achieved = f["alg_bytes"] / (f["ms"] * 1e-3) / 1e9
-        loop = [families[k] for k in ("pass1_spmm_dots", "pass2_update", "final_dots") if k in families]
+        loop = [families[k] for k in ("pass1_spmm_dots", "pass2_update", "final_dots", "sstep_dots", "sstep_form")
+                if k in families]
         loop_ms = sum(x["ms"] for x in loop)
         loop_bytes = sum(x["alg_bytes"] for x in loop)
         traffic = None

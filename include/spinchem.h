@@ -234,6 +234,8 @@ int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds,
 #define SPC_PROF_EXPM 6   /* m x m Pade exponential per vector         */
 #define SPC_PROF_STEP 7   /* c <- E c step loop                        */
 #define SPC_PROF_GRAM 8   /* projector Gram matrices (tol > 0)         */
+#define SPC_PROF_SDOTS 9  /* s-step block dots [Q_K, Y]^H Y            */
+#define SPC_PROF_SFORM 10 /* s-step block basis vectors                */
 int spc_profile_enable(spc_ctx *ctx, int on);
 int spc_profile_read(spc_ctx *ctx, int family, double *total_ms, int64_t *launches, double *bytes);
 

@@ -25,7 +25,7 @@ SPC_MAX_KRYLOV = 32
 STATE = {"singlet": 0, "T0": 1, "Tp": 2, "Tm": 3}
 VEC_SUZ, VEC_PHASE, VEC_INJECTED = 0, 1, 2
 PROF_FAMILIES = ["pass1_spmm_dots", "pass2_update", "final_dots", "psi_update", "reduce", "coeffs", "expm", "step",
-                 "gram"]
+                 "gram", "sstep_dots", "sstep_form"]
 
 # Exported symbols, in the order of include/spinchem.h.
 EXPORTS = [

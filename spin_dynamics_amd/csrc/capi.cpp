@@ -367,6 +367,16 @@ struct LaneSet {
 
 // Device buffers of lane set L; set 0 uses the plain names (shared with the
 // other entry points), sets 1-3 the same names with the suffix "#idx".
+// Block size of the s-step Arnoldi on the trace-sampling path: SPC_SSTEP
+// (0 or 1: one column per pass; at most 7), read at every run.  Default 6:
+// N = 36 864, M = 256, m = 24 on MI355X steps in 9.4 ms (14.2 ms column by
+// column; 9.9 at 5, 9.6 at 7).
+int sstep_block()
+{
+    const char *e = std::getenv("SPC_SSTEP");
+    return e ? std::clamp(std::atoi(e), 0, 7) : 6;
+}
+
 int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, int64_t n_out, bool need_gram,
                    int vkind)
 {
@@ -398,6 +408,8 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
     size_t part_need = partials_needed(w);
     if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
+    w.sstep = sstep_block();
+    if (w.sstep >= 2) part_need = std::max(part_need, sstep_partials(N, m, w.sstep, Mp));
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
     if ((rc = ensure(ctx, nm("dots").c_str(), (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots)))
@@ -414,6 +426,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("lane_k").c_str(), (size_t)Mp, &w.lane_k))) return rc;
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)n_out * 4 * Mp, &w.obs))) return rc;
+    if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
     if (vkind == 0 || vkind == 1)
         if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &L.d_seeds))) return rc;
     if (vkind == 2)
@@ -946,7 +959,8 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
-    const size_t part_need = partials_needed(w);
+    w.sstep = sstep_block();
+    const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
     if ((rc = ensure(ctx, nm("dots").c_str(), (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots)))
@@ -962,6 +976,7 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("lane_k").c_str(), (size_t)Mp, &w.lane_k))) return rc;
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
+    if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
     if ((rc = ensure(ctx, nm("bvals").c_str(), (size_t)n * nb * 16, &vals))) return rc;
     w.bval = vals;
     w.kron = nullptr;
@@ -1968,7 +1983,8 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
-    const size_t part_need = partials_needed(w);
+    w.sstep = sstep_block();
+    const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
     w.part_capacity = part_need;
     if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
@@ -1983,6 +1999,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if ((rc = ensure(ctx, "lane_k", (size_t)Mp, &w.lane_k))) return rc;
     if ((rc = ensure(ctx, "flags", (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, "obs", (size_t)nout * 4 * Mp, &w.obs))) return rc;
+    if (w.sstep >= 2 && (rc = ensure(ctx, "sbuf", (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
     double2 *lam = nullptr, *bvec = nullptr, *ckpt = nullptr, *store = nullptr, *Cb = nullptr, *Cf = nullptr,
             *Sg = nullptr;
     double *d_acc = nullptr;
@@ -2010,6 +2027,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         if ((rc = ensure(ctx, "adjb_c", (size_t)m * Mp, &wb.c))) return rc;
         if ((rc = ensure(ctx, "adjb_beta0", (size_t)Mp, &wb.beta0))) return rc;
         if ((rc = ensure(ctx, "adjb_flags", (size_t)4, &wb.flags))) return rc;
+        if (wb.sstep >= 2 && (rc = ensure(ctx, "adjb_sbuf", (size_t)kSStepBuf * Mp, &wb.sbuf))) return rc;
     }
     const int64_t NMp = (int64_t)N * Mp;
     auto stored = [&](int64_t n, int o) {

@@ -114,7 +114,10 @@ hipError_t sc_run(const ScParams &p, const int64_t *seeds, double *part, int nbl
 
 // --------------------------------------------------------------- krylov.hip
 // Kernel families timed by the optional event profiler (spc_profile_*).
-enum ProfFamily { PF_PASS1 = 0, PF_PASS2, PF_DOTS, PF_PSI, PF_REDUCE, PF_COEFFS, PF_EXPM, PF_STEP, PF_GRAM, PF_COUNT };
+enum ProfFamily {
+    PF_PASS1 = 0, PF_PASS2, PF_DOTS, PF_PSI, PF_REDUCE, PF_COEFFS, PF_EXPM, PF_STEP, PF_GRAM, PF_SDOTS, PF_SFORM,
+    PF_COUNT
+};
 
 // Event pairs around launches on the work stream; read after a synchronize.
 struct Prof {
@@ -207,7 +210,15 @@ struct KrylovWork {
     int64_t nnz;           // Hamiltonian entries (algorithmic bytes of the product)
     double kS, kT;         // recombination rates (mT units), for Q^H H u via H^H = H + iK
     Prof *prof;            // optional kernel timing
+    int sstep = 0;            // block size of the s-step Arnoldi (0 or 1: one column per pass)
+    double2 *sbuf = nullptr;  // its dots and forming coefficients, kSStepBuf x Mp
 };
+// Elements per lane of KrylovWork::sbuf.
+constexpr int kSStepBuf = 640;
+// Row blocks of the s-step dots pass for dimension N (8 .. 256, N/128 rounded
+// up to a multiple of 8), and the partials it needs for order m, block size S.
+int sstep_row_blocks(int64_t N);
+size_t sstep_partials(int64_t N, int m, int S, int Mp);
 
 inline int64_t qstride(const KrylovWork &w) { return w.qs > 0 ? w.qs : w.N * (int64_t)w.Mp; }
 // Stride between the basis vectors of an N x Mp basis: N*Mp plus a pad that

@@ -1482,7 +1482,387 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
     return hipGetLastError();
 }
 
+
+// ------------------------------------------------------- s-step Arnoldi
+// The same Krylov basis and Hessenberg matrix built a block of S columns at a
+// time (Hoemmen 2010, ch. 3; block classical Gram-Schmidt with the Pythagorean
+// Gram matrix and Cholesky QR: BCGS-PIP, Carson, Lund, Rozloznik and Thomas,
+// 2021).  With K orthonormal vectors q_0..q_{K-1}, the chain
+// y_t = H^{t+1} q_{K-1} (t < n, n <= S) takes n sparse products; one pass over
+// [Q_K, Y] gives the dots C = Q_K^H Y and G = Y^H Y; per lane
+// R^H R = G - C^H C (Cholesky, columns scaled to unit diagonal first); one
+// more pass writes the new vectors q_{K+t} = (Y R^-1 - Q_K C R^-1)_t in place
+// of Y.  The first block takes Y = [psi, H psi, .., H^S psi] (K = 0, beta_0 =
+// R_00).  The Hessenberg columns follow from H y_{t-1} = y_t written in the
+// new basis (y_t = Q_K C_t + Q_new R_t).  Per block the basis is read twice
+// instead of once per column; a lane whose Gram matrix is not positive
+// definite raises the breakdown flag.
+constexpr int kSMaxCols = 40;  // K + n columns of one pass (at most m + 1)
+constexpr int kSMaxNY = 8;     // new vectors per block (first block: S + 1)
+constexpr int kSGroup = 4;     // dot columns per workgroup of the dots pass
+constexpr int kSLanes = 16;    // lanes per workgroup of the forming pass
+constexpr int kSCoefLanes = 8; // lanes per workgroup of the coefficient kernel
+struct ColPtrs {
+    const double2 *p[kSMaxCols];
+};
+struct OutPtrs {
+    double2 *p[kSMaxNY];
+};
+
+// part[rb][x*NY + j][r] = sum over the block's rows of conj(x_x) y_j, for the
+// kSGroup columns x of group g; y_j = x_{K+j}.  Workgroup -> (g, row block,
+// lane chunk) with the groups of one row block consecutive on one XCD (they
+// read the same rows of Y).
+template <int NY>
+__global__ __launch_bounds__(kBlock) void k_sdots(ColPtrs x, int nx, int K, double2 *__restrict__ part, int64_t N,
+                                                  int Mp, int64_t rpb, int nrb, int ngroups)
+{
+    constexpr int KG = kSGroup;
+    __shared__ double2 red[kWaves - 1][KG][kWave];
+    const int nb = nrb * (Mp / kWave) * ngroups;
+    int bid = blockIdx.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int g = bid % ngroups, t = bid / ngroups;
+    const int rb = t % nrb, ch = t / nrb;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int r = ch * kWave + lane;
+    const int x0 = g * KG;
+    const int nxg = min(KG, nx - x0);
+    const double2 *xp[KG], *yp[NY];
+#pragma unroll
+    for (int i = 0; i < KG; ++i) xp[i] = x.p[min(x0 + i, nx - 1)];
+#pragma unroll
+    for (int j = 0; j < NY; ++j) yp[j] = x.p[K + j];
+    double2 acc[KG][NY];
+#pragma unroll
+    for (int i = 0; i < KG; ++i)
+#pragma unroll
+        for (int j = 0; j < NY; ++j) acc[i][j] = cz(0, 0);
+    const int64_t row0 = (int64_t)rb * rpb;
+    const int64_t row1 = row0 + rpb < N ? row0 + rpb : N;
+    for (int64_t row = row0 + wave; row < row1; row += kWaves) {
+        const int64_t off = row * Mp + r;
+        double2 yv[NY], xv[KG];
+#pragma unroll
+        for (int j = 0; j < NY; ++j) yv[j] = yp[j][off];
+#pragma unroll
+        for (int i = 0; i < KG; ++i) xv[i] = xp[i][off];
+#pragma unroll
+        for (int i = 0; i < KG; ++i)
+#pragma unroll
+            for (int j = 0; j < NY; ++j) acc[i][j] = cfma_conj(xv[i], yv[j], acc[i][j]);
+    }
+    const int ndots = nx * NY;
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+        if (wave > 0)
+#pragma unroll
+            for (int i = 0; i < KG; ++i) red[wave - 1][i][lane] = acc[i][j];
+        __syncthreads();
+        if (wave == 0)
+#pragma unroll
+            for (int i = 0; i < KG; ++i) {
+                if (i < nxg) {
+                    double2 s = acc[i][j];
+#pragma unroll
+                    for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][i][lane]);
+                    part[((int64_t)rb * ndots + (x0 + i) * NY + j) * Mp + r] = s;
+                }
+            }
+        __syncthreads();
+    }
+}
+
+// o_j = sum_x coef[x*NY + j] x_x over the rows of the block, for kSLanes lanes
+// (their coefficients in LDS); the outputs may be inputs (each thread reads
+// all of its row before writing it).
+template <int NY>
+__global__ __launch_bounds__(kBlock) void k_sform(ColPtrs x, int nx, OutPtrs o, const double2 *__restrict__ coef,
+                                                  int64_t N, int Mp, int64_t rpb)
+{
+    extern __shared__ double2 sc[]; // [nx * NY][kSLanes]
+    constexpr int RS = kBlock / kSLanes;
+    const int ngl = Mp / kSLanes;
+    const int lg = blockIdx.x % ngl, rb = blockIdx.x / ngl;
+    const int ln = threadIdx.x % kSLanes, rs = threadIdx.x / kSLanes;
+    const int r = lg * kSLanes + ln;
+    for (int idx = threadIdx.x; idx < nx * NY * kSLanes; idx += kBlock)
+        sc[idx] = coef[(int64_t)(idx / kSLanes) * Mp + lg * kSLanes + idx % kSLanes];
+    __syncthreads();
+    const int64_t row0 = (int64_t)rb * rpb;
+    const int64_t row1 = row0 + rpb < N ? row0 + rpb : N;
+    for (int64_t row = row0 + rs; row < row1; row += RS) {
+        const int64_t off = row * Mp + r;
+        double2 out[NY];
+#pragma unroll
+        for (int j = 0; j < NY; ++j) out[j] = cz(0, 0);
+        for (int c0 = 0; c0 < nx; c0 += 8) {
+            double2 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = x.p[min(c0 + i, nx - 1)][off];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (c0 + i < nx) {
+#pragma unroll
+                    for (int j = 0; j < NY; ++j) out[j] = cfma(sc[((c0 + i) * NY + j) * kSLanes + ln], v[i], out[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NY; ++j) ntstore(o.p[j] + off, out[j]);
+    }
+}
+
+// Per lane: R from G - C^H C, the forming coefficients and the Hessenberg
+// columns K-1 .. K+n-2 (first block, K = 0: columns 0 .. n-2 and beta_0).
+// K == m: the last column, h(k, m-1) = q_k^H H q_{m-1}.  dots[x*n + t]: C for
+// x < K, G for x >= K; coef[x*n + j] as k_sform reads it.
+__global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, int Mp,
+                                                  const double2 *__restrict__ dots, double2 *__restrict__ coef,
+                                                  double2 *__restrict__ hess, double *__restrict__ beta0,
+                                                  int32_t *__restrict__ flags)
+{
+    constexpr int NL = kSCoefLanes, TP = kBlock / NL;
+    static_assert(TP >= kMaxKrylov, "one thread per Hessenberg row");
+    __shared__ double2 sD[NL][kSMaxCols * kSMaxNY];
+    __shared__ double2 sR[NL][kSMaxNY * kSMaxNY], sRi[NL][kSMaxNY * kSMaxNY];
+    __shared__ int sok[NL];
+    const int l = threadIdx.x % NL, q = threadIdx.x / NL;
+    const int r = blockIdx.x * NL + l;
+    const bool active = r < M;
+    const int nd = (K + n) * n;
+    for (int idx = q; idx < nd; idx += TP) sD[l][idx] = active ? dots[(int64_t)idx * Mp + r] : cz(0, 0);
+    __syncthreads();
+    const double2 *D = sD[l];
+    double2 *R = sR[l], *Ri = sRi[l];
+    if (K == m) {
+        if (q < m) hess[((int64_t)(m - 1) * m + q) * Mp + r] = D[q];
+        return;
+    }
+    // G - C^H C
+    for (int idx = q; idx < n * n; idx += TP) {
+        const int a = idx / n, b = idx % n;
+        double2 g = D[(K + a) * n + b];
+        for (int c = 0; c < K; ++c) g = csub(g, cmul(cconj(D[c * n + a]), D[c * n + b]));
+        R[idx] = g;
+    }
+    __syncthreads();
+    if (q == 0) {
+        int ok = active ? 1 : 0;
+        if (ok) {
+            double d[kSMaxNY];
+            for (int a = 0; a < n; ++a) {
+                const double x = R[a * n + a].x;
+                if (!(x > 0.0) || !isfinite(x)) ok = 0;
+                d[a] = ok ? sqrt(x) : 1.0;
+            }
+            for (int a = 0; ok && a < n; ++a)
+                for (int b = 0; b < n; ++b) R[a * n + b] = cscale(1.0 / (d[a] * d[b]), R[a * n + b]);
+            // upper Cholesky factor of the scaled matrix, row by row in place
+            for (int i = 0; ok && i < n; ++i) {
+                double x = R[i * n + i].x;
+                for (int k = 0; k < i; ++k) x -= cabs2(R[k * n + i]);
+                if (!(x > 0.0) || !isfinite(x)) {
+                    ok = 0;
+                    break;
+                }
+                const double rii = sqrt(x), inv = 1.0 / rii;
+                R[i * n + i] = cz(rii, 0.0);
+                for (int j = i + 1; j < n; ++j) {
+                    double2 s = R[i * n + j];
+                    for (int k = 0; k < i; ++k) s = csub(s, cmul(cconj(R[k * n + i]), R[k * n + j]));
+                    R[i * n + j] = cscale(inv, s);
+                }
+            }
+            if (ok) {
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < n; ++j) R[i * n + j] = j < i ? cz(0, 0) : cscale(d[j], R[i * n + j]);
+                for (int j = 0; j < n; ++j) {
+                    Ri[j * n + j] = cz(1.0 / R[j * n + j].x, 0.0);
+                    for (int i = j - 1; i >= 0; --i) {
+                        double2 s = cz(0, 0);
+                        for (int k = i + 1; k <= j; ++k) s = cfma(R[i * n + k], Ri[k * n + j], s);
+                        Ri[i * n + j] = cscale(-1.0 / R[i * n + i].x, s);
+                    }
+                    for (int i = j + 1; i < n; ++i) Ri[i * n + j] = cz(0, 0);
+                }
+            } else {
+                flags[0] = 1; // Krylov breakdown (the block's vectors are dependent)
+            }
+        }
+        if (!ok)
+            for (int i = 0; i < n * n; ++i) R[i] = Ri[i] = cz(0, 0);
+        sok[l] = ok;
+    }
+    __syncthreads();
+    const bool ok = sok[l] != 0;
+    // forming coefficients: x < K: -(C R^-1)[x][j]; x = K + t: (R^-1)[t][j]
+    for (int idx = q; idx < nd; idx += TP) {
+        const int xx = idx / n, j = idx % n;
+        double2 cf = cz(0, 0);
+        if (xx < K) {
+            for (int t = 0; t <= j; ++t) cf = csub(cf, cmul(D[xx * n + t], Ri[t * n + j]));
+        } else {
+            cf = Ri[(xx - K) * n + j];
+        }
+        coef[(int64_t)idx * Mp + r] = cf;
+    }
+    if (q >= m) return;
+    double2 acc[kSMaxNY];
+    if (K == 0) {
+        // H y_j = y_{j+1}, y_j = Q R_j: column j = (R_{j+1} - sum_{l<j} col_l R[l][j]) / R[j][j]
+#pragma unroll
+        for (int j = 0; j < kSMaxNY - 1; ++j) {
+            acc[j] = cz(0, 0);
+            if (ok && j < n - 1) {
+                double2 a = q < n ? R[q * n + j + 1] : cz(0, 0);
+#pragma unroll
+                for (int k = 0; k < j; ++k) a = csub(a, cmul(acc[k], R[k * n + j]));
+                acc[j] = cscale(1.0 / R[j * n + j].x, a);
+            }
+            if (j < n - 1) hess[((int64_t)j * m + q) * Mp + r] = acc[j];
+        }
+        if (q == 0) beta0[r] = ok ? R[0].x : 0.0;
+        return;
+    }
+    // y_t = Q_K C_t + Q_new R_t; H q_{K-1} = y_0 and H y_{t-1} = y_t
+#pragma unroll
+    for (int t = 0; t < kSMaxNY; ++t)
+        acc[t] = ok && t < n ? (q < K ? D[q * n + t] : (q - K < n ? R[(q - K) * n + t] : cz(0, 0))) : cz(0, 0);
+    if (ok) {
+        // the old columns' entries of row q (zero below the subdiagonal), eight
+        // loads in flight at a time
+        const int c_lo = q > 0 ? q - 1 : 0, c_hi = K - 1;
+        for (int c0 = c_lo; c0 < c_hi; c0 += 8) {
+            double2 h[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) h[u] = hess[((int64_t)min(c0 + u, c_hi - 1) * m + q) * Mp + r];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (c0 + u < c_hi) {
+#pragma unroll
+                    for (int t = 1; t < kSMaxNY; ++t)
+                        if (t < n) acc[t] = csub(acc[t], cmul(h[u], D[(c0 + u) * n + t - 1]));
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 1; t < kSMaxNY; ++t) {
+            if (t < n) {
+                double2 a = csub(acc[t], cmul(acc[0], D[(K - 1) * n + t - 1]));
+#pragma unroll
+                for (int k = 0; k < t - 1; ++k) a = csub(a, cmul(acc[k + 1], R[k * n + t - 1]));
+                acc[t] = cscale(1.0 / R[(t - 1) * n + t - 1].x, a);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kSMaxNY; ++t)
+        if (t < n) hess[((int64_t)(K - 1 + t) * m + q) * Mp + r] = acc[t];
+}
+
+using SDotsFn = void (*)(ColPtrs, int, int, double2 *, int64_t, int, int64_t, int, int);
+using SFormFn = void (*)(ColPtrs, int, OutPtrs, const double2 *, int64_t, int, int64_t);
+const SDotsFn kSDots[kSMaxNY] = {k_sdots<1>, k_sdots<2>, k_sdots<3>, k_sdots<4>,
+                                 k_sdots<5>, k_sdots<6>, k_sdots<7>, k_sdots<8>};
+const SFormFn kSForm[kSMaxNY] = {k_sform<1>, k_sform<2>, k_sform<3>, k_sform<4>,
+                                 k_sform<5>, k_sform<6>, k_sform<7>, k_sform<8>};
+
+// Row blocks of the dots pass: a multiple of 8 (XCD mapping), as many as the
+// partials buffer holds for the largest pass, at most sstep_row_blocks(N).
+int sstep_nrb(const KrylovWork &w, int S)
+{
+    const size_t worst = (size_t)(w.m + 1) * (size_t)std::min(S + 1, kSMaxNY);
+    const size_t fit = w.part_capacity / (worst * (size_t)w.Mp);
+    return (int)std::min<size_t>((size_t)sstep_row_blocks(w.N), fit & ~(size_t)7);
+}
+
+hipError_t sdots_pass(KrylovWork &w, const ColPtrs &x, int nx, int K, int n, int nrb, hipStream_t s)
+{
+    const int ngroups = (nx + kSGroup - 1) / kSGroup;
+    const int64_t rpb = ceil_div(w.N, nrb);
+    {
+        ProfScope ps(w, PF_SDOTS, nx * vec_bytes(w), s);
+        hipLaunchKernelGGL(kSDots[n - 1], dim3(nrb * (w.Mp / kWave) * ngroups), dim3(kBlock), 0, s, x, nx, K, w.part,
+                           w.N, w.Mp, rpb, nrb, ngroups);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return reduce(w, nx * n, nrb, w.sbuf, s);
+}
+
+hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
+{
+    const int m = w.m;
+    const int64_t qs = qstride(w);
+    double2 *const u0 = w.u, *const v0 = w.v;
+    double2 *const sdots = w.sbuf, *const scoef = w.sbuf + (size_t)kSMaxCols * kSMaxNY * w.Mp;
+    auto slot = [&](int k) { return w.Q + (int64_t)k * qs; };
+    auto product = [&](double2 *in, double2 *out) {
+        w.u = in;
+        w.v = out;
+        const hipError_t e = spmm_launch(w, s);
+        w.u = u0;
+        w.v = v0;
+        return e;
+    };
+    hipError_t e = hipSuccess;
+    const int64_t rpf = 128; // rows per workgroup of the forming pass
+    const int nbf = (int)ceil_div(w.N, rpf) * (w.Mp / kSLanes);
+    for (int K = 0;;) {
+        ColPtrs x{};
+        OutPtrs o{};
+        int n;
+        if (K == 0) {
+            n = std::min(S, m - 1) + 1;
+            x.p[0] = u0;
+            o.p[0] = slot(0);
+            for (int j = 1; j < n; ++j) {
+                if ((e = product(j == 1 ? u0 : slot(j - 1), slot(j))) != hipSuccess) return e;
+                x.p[j] = o.p[j] = slot(j);
+            }
+        } else if (K < m) {
+            n = std::min(S, m - K);
+            for (int c = 0; c < K; ++c) x.p[c] = slot(c);
+            for (int t = 0; t < n; ++t) {
+                if ((e = product(slot(K - 1 + t), slot(K + t))) != hipSuccess) return e;
+                x.p[K + t] = o.p[t] = slot(K + t);
+            }
+        } else {
+            n = 1;
+            if ((e = product(slot(m - 1), v0)) != hipSuccess) return e;
+            for (int c = 0; c < m; ++c) x.p[c] = slot(c);
+            x.p[m] = v0;
+        }
+        const int nx = K + n;
+        if ((e = sdots_pass(w, x, nx, K, n, nrb, s)) != hipSuccess) return e;
+        {
+            ProfScope ps(w, PF_COEFFS, 0.0, s);
+            hipLaunchKernelGGL(k_scoef, dim3((unsigned)ceil_div(w.Mp, kSCoefLanes)), dim3(kBlock), 0, s, K, n, m, w.M,
+                               w.Mp, sdots, scoef, w.hess, w.beta0, w.flags);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (K == m) break;
+        {
+            ProfScope ps(w, PF_SFORM, (nx + n) * vec_bytes(w), s);
+            const size_t lds = sizeof(double2) * (size_t)nx * n * kSLanes;
+            hipLaunchKernelGGL(kSForm[n - 1], dim3(nbf), dim3(kBlock), lds, s, x, nx, o, scoef, w.N, w.Mp, rpf);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        K += n;
+    }
+    return hipSuccess;
+}
+
 } // namespace
+
+int sstep_row_blocks(int64_t N) { return (int)std::clamp<int64_t>((N / 128 + 7) & ~(int64_t)7, 8, 256); }
+
+size_t sstep_partials(int64_t N, int m, int S, int Mp)
+{
+    return (size_t)sstep_row_blocks(N) * (size_t)(m + 1) * (size_t)std::min(S + 1, kSMaxNY) * (size_t)Mp;
+}
 
 int64_t basis_stride(int64_t N, int Mp)
 {
@@ -1505,6 +1885,12 @@ size_t partials_needed(const KrylovWork &w)
 
 hipError_t arnoldi(KrylovWork &w, hipStream_t s)
 {
+    static_assert(2 * kSMaxCols * kSMaxNY == kSStepBuf, "s-step buffer size");
+    if (w.sstep >= 2 && w.sbuf && w.m >= 8) {
+        const int S = std::min(w.sstep, kSMaxNY - 1);
+        const int nrb = sstep_nrb(w, S);
+        if (nrb >= 8) return arnoldi_sstep(w, S, nrb, s);
+    }
     // u_0 = psi: w.u aliases w.psi (the update overwrites it in place; psi is
     // rebuilt from Q c after the step)
     const int nch = w.Mp / kWave;
