@@ -1632,7 +1632,14 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, in
     const int r = blockIdx.x * NL + l;
     const bool active = r < M;
     const int nd = (K + n) * n;
-    for (int idx = q; idx < nd; idx += TP) sD[l][idx] = active ? dots[(int64_t)idx * Mp + r] : cz(0, 0);
+    for (int base = 0; base < nd; base += 8 * TP) { // eight loads in flight per thread
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = active ? dots[(int64_t)min(base + q + u * TP, nd - 1) * Mp + r] : cz(0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (base + q + u * TP < nd) sD[l][base + q + u * TP] = v[u];
+    }
     __syncthreads();
     const double2 *D = sD[l];
     double2 *R = sR[l], *Ri = sRi[l];
