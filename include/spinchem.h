@@ -127,7 +127,8 @@ int spc_expm_small(spc_ctx *ctx, int32_t m, const double *A, double *E);
 /* Replaces get_krylov (qm_dynamics.f90:432-467) for M vectors at once: psi
  * [M][N] in, Q [M][m][N] (vector-major), A [M][m][m] (column-major) and
  * c0 = ||psi|| [M] out.  The product path keeps these on the device; this
- * entry point exists to check the Arnoldi kernels. */
+ * entry point exists to check the Arnoldi kernels: the one-column-per-pass
+ * form, or the block s-step form when SPC_SSTEP is set (2..7). */
 int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, double *Q, double *A,
                      double *beta);
 
@@ -225,12 +226,12 @@ int spc_last_run_stats(const spc_ctx *ctx, double *device_ms, int64_t *rebuilds,
  * on the context stream; spc_profile_read sums one family's device time,
  * launch count and algorithmic bytes (complex128 vector passes over the M
  * trace vectors, plus the CSR for the sparse product). */
-#define SPC_PROF_PASS1 0  /* v = H u with Gram-Schmidt dots            */
+#define SPC_PROF_PASS1 0  /* v = H u (with Gram-Schmidt dots: columns)  */
 #define SPC_PROF_PASS2 1  /* basis update q_i, u_{i+1}                 */
 #define SPC_PROF_DOTS 2   /* last-column correction dots               */
 #define SPC_PROF_PSI 3    /* psi = Q c with fused populations          */
 #define SPC_PROF_REDUCE 4 /* workgroup-partial reductions              */
-#define SPC_PROF_COEFFS 5 /* per-vector Arnoldi coefficient algebra    */
+#define SPC_PROF_COEFFS 5 /* per-vector Arnoldi (and block) algebra    */
 #define SPC_PROF_EXPM 6   /* m x m Pade exponential per vector         */
 #define SPC_PROF_STEP 7   /* c <- E c step loop                        */
 #define SPC_PROF_GRAM 8   /* projector Gram matrices (tol > 0)         */

@@ -1550,14 +1550,18 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     w.kS = ctx->kS;
     w.kT = ctx->kT;
     w.rpw1 = w.rpw2 = w.rpwp = 4;
+    // the hooks check the column form unless SPC_SSTEP is set explicitly
+    w.sstep = std::getenv("SPC_SSTEP") ? sstep_block() : 0;
     int rc;
     if ((rc = ensure(ctx, "Q", (size_t)w.m * w.N * Mp, &w.Q))) return rc;
     if ((rc = ensure(ctx, "psi", (size_t)w.N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)w.N * Mp, &w.v))) return rc;
     size_t pn = partials_needed(w);
+    if (w.sstep >= 2) pn = std::max(pn, sstep_partials(w.N, w.m, w.sstep, Mp));
     if ((rc = ensure(ctx, "part", pn, &w.part))) return rc;
     w.part_capacity = pn;
+    if (w.sstep >= 2 && (rc = ensure(ctx, "sbuf", (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
     if ((rc = ensure(ctx, "dots", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
     if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
     if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;

@@ -29,6 +29,33 @@ def _pair(N):
     return [q for q in batch.radical_pairs() if q.N == N][0]
 
 
+@pytest.mark.parametrize("S", ["2", "6", "7"])
+def test_block_basis_arnoldi_relation(ctx, monkeypatch, S):
+    # the block basis and Hessenberg matrix through spc_krylov_basis: the
+    # Arnoldi relation H Q[:, :m-1] = Q A[:, :m-1] holds to rounding (an index
+    # or sign slip in the change of basis breaks it at O(1)); orthogonality
+    # and the MGS Hessenberg within the block form's loss of orthogonality
+    # (numpy model of this system: 3e-11 and 1e-11 at S = 6)
+    monkeypatch.setenv("SPC_SSTEP", S)
+    w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 2))
+    s = oracle_system(w.sys, w.B)
+    H = dynamics.Hamiltonian(w.sys, w.B, ctx)
+    ref_h = pyoracle.build_general_csr(s)
+    rng = np.random.default_rng(3)
+    M, m = 5, 16
+    psi = rng.normal(size=(M, H.N)) + 1j * rng.normal(size=(M, H.N))
+    Q, A, beta = H.krylov(psi, m)
+    Hs = H.scipy()
+    for r in range(M):
+        Qr, cr, Ar = pyoracle.get_krylov(ref_h, psi[r], m)
+        assert abs(beta[r] - cr[0].real) < 1e-13 * abs(cr[0])
+        Qm = Q[r].T
+        assert max_rel(Hs @ Qm[:, :m - 1], Qm @ A[r][:, :m - 1]) < 1e-12
+        assert np.max(np.abs(Qm.conj().T @ Qm - np.eye(m))) < 1e-9
+        assert max_rel(A[r], Ar) < 1e-9
+        assert np.all(A[r][np.tril_indices(m, -2)] == 0)  # Hessenberg: zero below the subdiagonal
+
+
 def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch):
     monkeypatch.setenv("SPC_SSTEP", "6")
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=1.5)  # two steps
