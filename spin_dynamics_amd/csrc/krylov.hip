@@ -1615,10 +1615,13 @@ __global__ __launch_bounds__(kBlock) void k_sform(ColPtrs x, int nx, OutPtrs o, 
 }
 
 // Per lane: R from G - C^H C, the forming coefficients and the Hessenberg
-// columns K-1 .. K+n-2 (first block, K = 0: columns 0 .. n-2 and beta_0).
-// K == m: the last column, h(k, m-1) = q_k^H H q_{m-1}.  dots[x*n + t]: C for
-// x < K, G for x >= K; coef[x*n + j] as k_sform reads it.
-__global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, int Mp,
+// columns.  dots[x*n + t] (x < K + n, t < n): C = Q_K^H Y for x < K, G = Y^H Y
+// for x >= K.  The first nf = n - extra chain vectors are formed into basis
+// vectors (coef[x*nf + j] as k_sform reads it) and give the columns K-1 ..
+// K+nf-2 (first block, K = 0: columns 0 .. nf-2 and beta_0).  With extra = 1
+// (the last block) chain vector nf is not formed: its projection on the basis,
+// [C_nf; R^-H (G_{:nf,nf} - C^H C_nf)], gives the last column K+nf-1 = m-1.
+__global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m, int M, int Mp,
                                                   const double2 *__restrict__ dots, double2 *__restrict__ coef,
                                                   double2 *__restrict__ hess, double *__restrict__ beta0,
                                                   int32_t *__restrict__ flags)
@@ -1626,12 +1629,13 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, in
     constexpr int NL = kSCoefLanes, TP = kBlock / NL;
     static_assert(TP >= kMaxKrylov, "one thread per Hessenberg row");
     __shared__ double2 sD[NL][kSMaxCols * kSMaxNY];
-    __shared__ double2 sR[NL][kSMaxNY * kSMaxNY], sRi[NL][kSMaxNY * kSMaxNY];
+    __shared__ double2 sR[NL][kSMaxNY * kSMaxNY], sRi[NL][kSMaxNY * kSMaxNY], sW[NL][2 * kSMaxNY];
     __shared__ int sok[NL];
     const int l = threadIdx.x % NL, q = threadIdx.x / NL;
     const int r = blockIdx.x * NL + l;
     const bool active = r < M;
     const int nd = (K + n) * n;
+    const int nf = n - extra;
     for (int base = 0; base < nd; base += 8 * TP) { // eight loads in flight per thread
         double2 v[8];
 #pragma unroll
@@ -1641,77 +1645,84 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, in
             if (base + q + u * TP < nd) sD[l][base + q + u * TP] = v[u];
     }
     __syncthreads();
-    const double2 *D = sD[l];
-    double2 *R = sR[l], *Ri = sRi[l];
-    if (K == m) {
-        if (q < m) hess[((int64_t)(m - 1) * m + q) * Mp + r] = D[q];
-        return;
-    }
-    // G - C^H C
-    for (int idx = q; idx < n * n; idx += TP) {
+    const double2 *D = sD[l]; // C(c, t) = D[c*n + t], G(a, t) = D[(K + a)*n + t]
+    double2 *R = sR[l], *Ri = sRi[l], *W = sW[l];
+    // G - C^H C on rows a < nf, columns b < n (column nf: the extra vector's)
+    for (int idx = q; idx < nf * n; idx += TP) {
         const int a = idx / n, b = idx % n;
         double2 g = D[(K + a) * n + b];
         for (int c = 0; c < K; ++c) g = csub(g, cmul(cconj(D[c * n + a]), D[c * n + b]));
-        R[idx] = g;
+        if (b < nf)
+            R[a * nf + b] = g;
+        else
+            W[a] = g;
     }
     __syncthreads();
     if (q == 0) {
         int ok = active ? 1 : 0;
         if (ok) {
             double d[kSMaxNY];
-            for (int a = 0; a < n; ++a) {
-                const double x = R[a * n + a].x;
+            for (int a = 0; a < nf; ++a) {
+                const double x = R[a * nf + a].x;
                 if (!(x > 0.0) || !isfinite(x)) ok = 0;
                 d[a] = ok ? sqrt(x) : 1.0;
             }
-            for (int a = 0; ok && a < n; ++a)
-                for (int b = 0; b < n; ++b) R[a * n + b] = cscale(1.0 / (d[a] * d[b]), R[a * n + b]);
+            for (int a = 0; ok && a < nf; ++a)
+                for (int b = 0; b < nf; ++b) R[a * nf + b] = cscale(1.0 / (d[a] * d[b]), R[a * nf + b]);
             // upper Cholesky factor of the scaled matrix, row by row in place
-            for (int i = 0; ok && i < n; ++i) {
-                double x = R[i * n + i].x;
-                for (int k = 0; k < i; ++k) x -= cabs2(R[k * n + i]);
+            for (int i = 0; ok && i < nf; ++i) {
+                double x = R[i * nf + i].x;
+                for (int k = 0; k < i; ++k) x -= cabs2(R[k * nf + i]);
                 if (!(x > 0.0) || !isfinite(x)) {
                     ok = 0;
                     break;
                 }
                 const double rii = sqrt(x), inv = 1.0 / rii;
-                R[i * n + i] = cz(rii, 0.0);
-                for (int j = i + 1; j < n; ++j) {
-                    double2 s = R[i * n + j];
-                    for (int k = 0; k < i; ++k) s = csub(s, cmul(cconj(R[k * n + i]), R[k * n + j]));
-                    R[i * n + j] = cscale(inv, s);
+                R[i * nf + i] = cz(rii, 0.0);
+                for (int j = i + 1; j < nf; ++j) {
+                    double2 s = R[i * nf + j];
+                    for (int k = 0; k < i; ++k) s = csub(s, cmul(cconj(R[k * nf + i]), R[k * nf + j]));
+                    R[i * nf + j] = cscale(inv, s);
                 }
             }
             if (ok) {
-                for (int i = 0; i < n; ++i)
-                    for (int j = 0; j < n; ++j) R[i * n + j] = j < i ? cz(0, 0) : cscale(d[j], R[i * n + j]);
-                for (int j = 0; j < n; ++j) {
-                    Ri[j * n + j] = cz(1.0 / R[j * n + j].x, 0.0);
+                for (int i = 0; i < nf; ++i)
+                    for (int j = 0; j < nf; ++j) R[i * nf + j] = j < i ? cz(0, 0) : cscale(d[j], R[i * nf + j]);
+                for (int j = 0; j < nf; ++j) {
+                    Ri[j * nf + j] = cz(1.0 / R[j * nf + j].x, 0.0);
                     for (int i = j - 1; i >= 0; --i) {
                         double2 s = cz(0, 0);
-                        for (int k = i + 1; k <= j; ++k) s = cfma(R[i * n + k], Ri[k * n + j], s);
-                        Ri[i * n + j] = cscale(-1.0 / R[i * n + i].x, s);
+                        for (int k = i + 1; k <= j; ++k) s = cfma(R[i * nf + k], Ri[k * nf + j], s);
+                        Ri[i * nf + j] = cscale(-1.0 / R[i * nf + i].x, s);
                     }
-                    for (int i = j + 1; i < n; ++i) Ri[i * n + j] = cz(0, 0);
+                    for (int i = j + 1; i < nf; ++i) Ri[i * nf + j] = cz(0, 0);
+                }
+                // the extra vector's coordinates on the new vectors: R^-H w
+                for (int j = 0; extra && j < nf; ++j) {
+                    double2 p = cz(0, 0);
+                    for (int a = 0; a <= j; ++a) p = cfma_conj(Ri[a * nf + j], W[a], p);
+                    W[kSMaxNY + j] = p;
                 }
             } else {
                 flags[0] = 1; // Krylov breakdown (the block's vectors are dependent)
             }
         }
-        if (!ok)
-            for (int i = 0; i < n * n; ++i) R[i] = Ri[i] = cz(0, 0);
+        if (!ok) {
+            for (int i = 0; i < nf * nf; ++i) R[i] = Ri[i] = cz(0, 0);
+            for (int j = 0; j < nf; ++j) W[kSMaxNY + j] = cz(0, 0);
+        }
         sok[l] = ok;
     }
     __syncthreads();
     const bool ok = sok[l] != 0;
     // forming coefficients: x < K: -(C R^-1)[x][j]; x = K + t: (R^-1)[t][j]
-    for (int idx = q; idx < nd; idx += TP) {
-        const int xx = idx / n, j = idx % n;
+    for (int idx = q; idx < (K + nf) * nf; idx += TP) {
+        const int xx = idx / nf, j = idx % nf;
         double2 cf = cz(0, 0);
         if (xx < K) {
-            for (int t = 0; t <= j; ++t) cf = csub(cf, cmul(D[xx * n + t], Ri[t * n + j]));
+            for (int t = 0; t <= j; ++t) cf = csub(cf, cmul(D[xx * n + t], Ri[t * nf + j]));
         } else {
-            cf = Ri[(xx - K) * n + j];
+            cf = Ri[(xx - K) * nf + j];
         }
         coef[(int64_t)idx * Mp + r] = cf;
     }
@@ -1722,21 +1733,31 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, in
 #pragma unroll
         for (int j = 0; j < kSMaxNY - 1; ++j) {
             acc[j] = cz(0, 0);
-            if (ok && j < n - 1) {
-                double2 a = q < n ? R[q * n + j + 1] : cz(0, 0);
+            if (ok && j < nf - 1) {
+                double2 a = q < nf ? R[q * nf + j + 1] : cz(0, 0);
 #pragma unroll
-                for (int k = 0; k < j; ++k) a = csub(a, cmul(acc[k], R[k * n + j]));
-                acc[j] = cscale(1.0 / R[j * n + j].x, a);
+                for (int k = 0; k < j; ++k) a = csub(a, cmul(acc[k], R[k * nf + j]));
+                acc[j] = cscale(1.0 / R[j * nf + j].x, a);
             }
-            if (j < n - 1) hess[((int64_t)j * m + q) * Mp + r] = acc[j];
+            if (j < nf - 1) hess[((int64_t)j * m + q) * Mp + r] = acc[j];
         }
         if (q == 0) beta0[r] = ok ? R[0].x : 0.0;
         return;
     }
-    // y_t = Q_K C_t + Q_new R_t; H q_{K-1} = y_0 and H y_{t-1} = y_t
+    // y_t = Q_K C_t + Q_new R_t (t < nf; t = nf: the projection of the extra
+    // vector); H q_{K-1} = y_0 and H y_{t-1} = y_t
+    const int ncol = nf + extra;
 #pragma unroll
-    for (int t = 0; t < kSMaxNY; ++t)
-        acc[t] = ok && t < n ? (q < K ? D[q * n + t] : (q - K < n ? R[(q - K) * n + t] : cz(0, 0))) : cz(0, 0);
+    for (int t = 0; t < kSMaxNY; ++t) {
+        double2 a = cz(0, 0);
+        if (ok && t < ncol) {
+            if (q < K)
+                a = D[q * n + t];
+            else if (q - K < nf)
+                a = t < nf ? R[(q - K) * nf + t] : W[kSMaxNY + q - K];
+        }
+        acc[t] = a;
+    }
     if (ok) {
         // the old columns' entries of row q (zero below the subdiagonal), eight
         // loads in flight at a time
@@ -1750,23 +1771,23 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int m, int M, in
                 if (c0 + u < c_hi) {
 #pragma unroll
                     for (int t = 1; t < kSMaxNY; ++t)
-                        if (t < n) acc[t] = csub(acc[t], cmul(h[u], D[(c0 + u) * n + t - 1]));
+                        if (t < ncol) acc[t] = csub(acc[t], cmul(h[u], D[(c0 + u) * n + t - 1]));
                 }
             }
         }
 #pragma unroll
         for (int t = 1; t < kSMaxNY; ++t) {
-            if (t < n) {
+            if (t < ncol) {
                 double2 a = csub(acc[t], cmul(acc[0], D[(K - 1) * n + t - 1]));
 #pragma unroll
-                for (int k = 0; k < t - 1; ++k) a = csub(a, cmul(acc[k + 1], R[k * n + t - 1]));
-                acc[t] = cscale(1.0 / R[(t - 1) * n + t - 1].x, a);
+                for (int k = 0; k < t - 1; ++k) a = csub(a, cmul(acc[k + 1], R[k * nf + t - 1]));
+                acc[t] = cscale(1.0 / R[(t - 1) * nf + t - 1].x, a);
             }
         }
     }
 #pragma unroll
     for (int t = 0; t < kSMaxNY; ++t)
-        if (t < n) hess[((int64_t)(K - 1 + t) * m + q) * Mp + r] = acc[t];
+        if (t < ncol) hess[((int64_t)(K - 1 + t) * m + q) * Mp + r] = acc[t];
 }
 
 using SDotsFn = void (*)(ColPtrs, int, int, double2 *, int64_t, int, int64_t, int, int);
@@ -1817,10 +1838,10 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
     hipError_t e = hipSuccess;
     const int64_t rpf = 128; // rows per workgroup of the forming pass
     const int nbf = (int)ceil_div(w.N, rpf) * (w.Mp / kSLanes);
-    for (int K = 0;;) {
+    for (int K = 0; K < m;) {
         ColPtrs x{};
         OutPtrs o{};
-        int n;
+        int n, extra = 0;
         if (K == 0) {
             n = std::min(S, m - 1) + 1;
             x.p[0] = u0;
@@ -1829,29 +1850,29 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
                 if ((e = product(j == 1 ? u0 : slot(j - 1), slot(j))) != hipSuccess) return e;
                 x.p[j] = o.p[j] = slot(j);
             }
-        } else if (K < m) {
+        } else {
             n = std::min(S, m - K);
             for (int c = 0; c < K; ++c) x.p[c] = slot(c);
             for (int t = 0; t < n; ++t) {
                 if ((e = product(slot(K - 1 + t), slot(K + t))) != hipSuccess) return e;
                 x.p[K + t] = o.p[t] = slot(K + t);
             }
-        } else {
-            n = 1;
-            if ((e = product(slot(m - 1), v0)) != hipSuccess) return e;
-            for (int c = 0; c < m; ++c) x.p[c] = slot(c);
-            x.p[m] = v0;
+            if (K + n == m) { // one more chain vector, H q_{m-1}'s direction, gives the last column
+                extra = 1;
+                if ((e = product(slot(m - 1), v0)) != hipSuccess) return e;
+                x.p[m] = v0;
+            }
         }
-        const int nx = K + n;
-        if ((e = sdots_pass(w, x, nx, K, n, nrb, s)) != hipSuccess) return e;
+        const int nd = n + extra;
+        if ((e = sdots_pass(w, x, K + nd, K, nd, nrb, s)) != hipSuccess) return e;
         {
             ProfScope ps(w, PF_COEFFS, 0.0, s);
-            hipLaunchKernelGGL(k_scoef, dim3((unsigned)ceil_div(w.Mp, kSCoefLanes)), dim3(kBlock), 0, s, K, n, m, w.M,
-                               w.Mp, sdots, scoef, w.hess, w.beta0, w.flags);
+            hipLaunchKernelGGL(k_scoef, dim3((unsigned)ceil_div(w.Mp, kSCoefLanes)), dim3(kBlock), 0, s, K, nd, extra,
+                               m, w.M, w.Mp, sdots, scoef, w.hess, w.beta0, w.flags);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (K == m) break;
         {
+            const int nx = K + n;
             ProfScope ps(w, PF_SFORM, (nx + n) * vec_bytes(w), s);
             const size_t lds = sizeof(double2) * (size_t)nx * n * kSLanes;
             hipLaunchKernelGGL(kSForm[n - 1], dim3(nbf), dim3(kBlock), lds, s, x, nx, o, scoef, w.N, w.Mp, rpf);
