@@ -10,9 +10,12 @@
 // workgroup combine through LDS, workgroups through a deterministic tree pass.
 //
 // Orthogonalisation.  The reference uses modified Gram-Schmidt (i dependent
-// reductions per column).  Here each column costs one sparse product and one
-// streaming pass over the basis, with classical Gram-Schmidt plus a delayed
-// re-orthogonalisation (the "DCGS2" form of Swirydowicz et al., 2020):
+// reductions per column).  By default the basis is built a block of S columns
+// at a time (arnoldi_sstep below: a chain of products, one dots pass and one
+// forming pass per block).  With SPC_SSTEP=0 (and for m < 8) each column costs
+// one sparse product and one streaming pass over the basis, with classical
+// Gram-Schmidt plus a delayed re-orthogonalisation (the "DCGS2" form of
+// Swirydowicz et al., 2020):
 //   product  (column i): v = H u_i,  c = u_i^H v,  d = u_i^H u_i
 //   coefficients:        a = Q^H u_i and s = <P_S Q, P_S u_i> come from the
 //                        previous update pass; beta_i^2 = d - |a|^2; column i-1
@@ -1492,13 +1495,14 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 // [Q_K, Y] gives the dots C = Q_K^H Y and G = Y^H Y; per lane
 // R^H R = G - C^H C (Cholesky, columns scaled to unit diagonal first); one
 // more pass writes the new vectors q_{K+t} = (Y R^-1 - Q_K C R^-1)_t in place
-// of Y.  The first block takes Y = [psi, H psi, .., H^S psi] (K = 0, beta_0 =
-// R_00).  The Hessenberg columns follow from H y_{t-1} = y_t written in the
-// new basis (y_t = Q_K C_t + Q_new R_t).  Per block the basis is read twice
-// instead of once per column; a lane whose Gram matrix is not positive
-// definite raises the breakdown flag.
+// of Y.  The first block takes Y = [psi, H psi, .., H^(S-1) psi] (K = 0,
+// beta_0 = R_00).  The Hessenberg columns follow from H y_{t-1} = y_t written
+// in the new basis (y_t = Q_K C_t + Q_new R_t); the last block carries one
+// more chain vector, whose projection on the basis gives the last column.
+// Per block the basis is read twice instead of once per column; a lane whose
+// Gram matrix is not positive definite raises the breakdown flag.
 constexpr int kSMaxCols = 40;  // K + n columns of one pass (at most m + 1)
-constexpr int kSMaxNY = 8;     // new vectors per block (first block: S + 1)
+constexpr int kSMaxNY = 8;     // chain vectors per block (S, and the last block's extra one)
 constexpr int kSGroup = 4;     // dot columns per workgroup of the dots pass
 constexpr int kSLanes = 16;    // lanes per workgroup of the forming pass
 constexpr int kSCoefLanes = 8; // lanes per workgroup of the coefficient kernel
@@ -1842,8 +1846,8 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
         ColPtrs x{};
         OutPtrs o{};
         int n, extra = 0;
-        if (K == 0) {
-            n = std::min(S, m - 1) + 1;
+        if (K == 0) { // [psi, H psi, .., H^(n-1) psi]: blocks of S columns throughout (m = 24, S = 6: 6, 6, 6, 6)
+            n = std::min(S, m - 1);
             x.p[0] = u0;
             o.p[0] = slot(0);
             for (int j = 1; j < n; ++j) {
