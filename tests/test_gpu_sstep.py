@@ -81,6 +81,23 @@ def test_every_block_size_matches_the_column_path(ctx, monkeypatch, S):
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (S, k)
 
 
+@pytest.mark.parametrize("m,S", [(8, 6), (9, 6), (13, 6), (12, 2), (29, 7), (32, 6), (32, 7)])
+def test_block_schedules_match_the_column_path(ctx, monkeypatch, m, S):
+    # the smallest order the block form takes (8), orders that leave a short
+    # last block, the largest order (32) and the widest chain (S = 7: a last
+    # block of 7 + 1 vectors at m = 29)
+    p = _pair(3072)
+    sim = batch.orientation_sim(m=m, T_ns=3.5)  # four steps
+    s = copy.deepcopy(p.sys)
+    s.field_dir = (-0.2, 0.6, 0.77)
+    monkeypatch.setenv("SPC_SSTEP", "0")
+    col = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
+    monkeypatch.setenv("SPC_SSTEP", str(S))
+    blk = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
+    for k in KEYS:
+        assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (m, S, k)
+
+
 def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch):
     monkeypatch.setenv("SPC_SSTEP", "6")
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=39.5)  # 40 steps
