@@ -1634,12 +1634,14 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
     static_assert(TP >= kMaxKrylov, "one thread per Hessenberg row");
     __shared__ double2 sD[NL][kSMaxCols * kSMaxNY];
     __shared__ double2 sR[NL][kSMaxNY * kSMaxNY], sRi[NL][kSMaxNY * kSMaxNY], sW[NL][2 * kSMaxNY];
-    __shared__ int sok[NL];
+    __shared__ double sd[NL][kSMaxNY], sinv[NL];
+    __shared__ int sok[NL], sfail[NL];
     const int l = threadIdx.x % NL, q = threadIdx.x / NL;
     const int r = blockIdx.x * NL + l;
     const bool active = r < M;
     const int nd = (K + n) * n;
     const int nf = n - extra;
+    if (q == 0) sfail[l] = 0;
     for (int base = 0; base < nd; base += 8 * TP) { // eight loads in flight per thread
         double2 v[8];
 #pragma unroll
@@ -1662,63 +1664,63 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
             W[a] = g;
     }
     __syncthreads();
-    if (q == 0) {
-        int ok = active ? 1 : 0;
-        if (ok) {
-            double d[kSMaxNY];
-            for (int a = 0; a < nf; ++a) {
-                const double x = R[a * nf + a].x;
-                if (!(x > 0.0) || !isfinite(x)) ok = 0;
-                d[a] = ok ? sqrt(x) : 1.0;
-            }
-            for (int a = 0; ok && a < nf; ++a)
-                for (int b = 0; b < nf; ++b) R[a * nf + b] = cscale(1.0 / (d[a] * d[b]), R[a * nf + b]);
-            // upper Cholesky factor of the scaled matrix, row by row in place
-            for (int i = 0; ok && i < nf; ++i) {
-                double x = R[i * nf + i].x;
-                for (int k = 0; k < i; ++k) x -= cabs2(R[k * nf + i]);
-                if (!(x > 0.0) || !isfinite(x)) {
-                    ok = 0;
-                    break;
-                }
-                const double rii = sqrt(x), inv = 1.0 / rii;
-                R[i * nf + i] = cz(rii, 0.0);
-                for (int j = i + 1; j < nf; ++j) {
-                    double2 s = R[i * nf + j];
-                    for (int k = 0; k < i; ++k) s = csub(s, cmul(cconj(R[k * nf + i]), R[k * nf + j]));
-                    R[i * nf + j] = cscale(inv, s);
-                }
-            }
-            if (ok) {
-                for (int i = 0; i < nf; ++i)
-                    for (int j = 0; j < nf; ++j) R[i * nf + j] = j < i ? cz(0, 0) : cscale(d[j], R[i * nf + j]);
-                for (int j = 0; j < nf; ++j) {
-                    Ri[j * nf + j] = cz(1.0 / R[j * nf + j].x, 0.0);
-                    for (int i = j - 1; i >= 0; --i) {
-                        double2 s = cz(0, 0);
-                        for (int k = i + 1; k <= j; ++k) s = cfma(R[i * nf + k], Ri[k * nf + j], s);
-                        Ri[i * nf + j] = cscale(-1.0 / R[i * nf + i].x, s);
-                    }
-                    for (int i = j + 1; i < nf; ++i) Ri[i * nf + j] = cz(0, 0);
-                }
-                // the extra vector's coordinates on the new vectors: R^-H w
-                for (int j = 0; extra && j < nf; ++j) {
-                    double2 p = cz(0, 0);
-                    for (int a = 0; a <= j; ++a) p = cfma_conj(Ri[a * nf + j], W[a], p);
-                    W[kSMaxNY + j] = p;
-                }
-            } else {
-                flags[0] = 1; // Krylov breakdown (the block's vectors are dependent)
-            }
-        }
-        if (!ok) {
-            for (int i = 0; i < nf * nf; ++i) R[i] = Ri[i] = cz(0, 0);
-            for (int j = 0; j < nf; ++j) W[kSMaxNY + j] = cz(0, 0);
-        }
-        sok[l] = ok;
+    // scaled Cholesky (right-looking, the lane's 32 threads over the entries),
+    // then R^-1 and the extra vector's coordinates R^-H w, one column per thread
+    if (q == 0) sok[l] = active ? 1 : 0;
+    if (q < nf) {
+        const double x = R[q * nf + q].x;
+        const bool good = x > 0.0 && isfinite(x);
+        sd[l][q] = good ? sqrt(x) : 1.0;
+        if (!good) sfail[l] = 1;
     }
     __syncthreads();
-    const bool ok = sok[l] != 0;
+    for (int idx = q; idx < nf * nf; idx += TP)
+        R[idx] = cscale(1.0 / (sd[l][idx / nf] * sd[l][idx % nf]), R[idx]);
+    __syncthreads();
+    for (int i = 0; i < nf; ++i) {
+        if (q == 0) {
+            double x = R[i * nf + i].x;
+            if (!(x > 0.0) || !isfinite(x)) {
+                sfail[l] = 1;
+                x = 1.0;
+            }
+            const double rii = sqrt(x);
+            R[i * nf + i] = cz(rii, 0.0);
+            sinv[l] = 1.0 / rii;
+        }
+        __syncthreads();
+        if (q > i && q < nf) R[i * nf + q] = cscale(sinv[l], R[i * nf + q]);
+        __syncthreads();
+        for (int idx = q; idx < nf * nf; idx += TP) {
+            const int a = idx / nf, b = idx % nf;
+            if (a > i && b >= a) R[idx] = csub(R[idx], cmul(cconj(R[i * nf + a]), R[i * nf + b]));
+        }
+        __syncthreads();
+    }
+    const bool ok = sok[l] != 0 && sfail[l] == 0;
+    for (int idx = q; idx < nf * nf; idx += TP) {
+        const int a = idx / nf, b = idx % nf;
+        R[idx] = (b < a || !ok) ? cz(0, 0) : cscale(sd[l][b], R[idx]);
+    }
+    __syncthreads();
+    if (q < nf) {
+        const int j = q;
+        if (ok) {
+            Ri[j * nf + j] = cz(1.0 / R[j * nf + j].x, 0.0);
+            for (int i = j - 1; i >= 0; --i) {
+                double2 s = cz(0, 0);
+                for (int k = i + 1; k <= j; ++k) s = cfma(R[i * nf + k], Ri[k * nf + j], s);
+                Ri[i * nf + j] = cscale(-1.0 / R[i * nf + i].x, s);
+            }
+        }
+        for (int i = ok ? j + 1 : 0; i < nf; ++i) Ri[i * nf + j] = cz(0, 0);
+        double2 pj = cz(0, 0);
+        if (ok && extra)
+            for (int a = 0; a <= j; ++a) pj = cfma_conj(Ri[a * nf + j], W[a], pj);
+        W[kSMaxNY + j] = pj;
+    }
+    if (q == 0 && active && !ok) flags[0] = 1; // Krylov breakdown (the block's vectors are dependent)
+    __syncthreads();
     // forming coefficients: x < K: -(C R^-1)[x][j]; x = K + t: (R^-1)[t][j]
     for (int idx = q; idx < (K + nf) * nf; idx += TP) {
         const int xx = idx / nf, j = idx % nf;
