@@ -211,6 +211,7 @@ struct KrylovWork {
     double kS, kT;         // recombination rates (mT units), for Q^H H u via H^H = H + iK
     Prof *prof;            // optional kernel timing
     int sstep = 0;            // block size of the s-step Arnoldi (0 or 1: one column per pass)
+    bool no_dots = false;     // products may skip their u^H v, u^H u partials (tiled product only)
     double2 *sbuf = nullptr;  // its dots and forming coefficients, kSStepBuf x Mp
 };
 // Elements per lane of KrylovWork::sbuf.

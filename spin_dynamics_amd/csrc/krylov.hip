@@ -467,7 +467,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
 // pair the workgroup's own waves, so those neighbour rows also come from LDS
 // instead of L2.  Needs the last Q + BD nuclei spin-1/2.  Same summation order
 // as k_spmm_kron (bit-identical products).  rpg = tiles per workgroup.
-template <int G, int Q, int NW>
+// DOTS = false: v only (the block Arnoldi's chain products need no dots).
+template <int G, int Q, int NW, bool DOTS = true>
 __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__restrict__ kd,
                                                             const uint64_t *__restrict__ kmask,
                                                             const double2 *__restrict__ diags,
@@ -586,11 +587,14 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             ntstore(&v[rowb + toff + e * Mp], y[e]); // streamed past L2, which keeps u's rows
-            acc[0] = cfma_conj(x0[e], y[e], acc[0]);
-            acc[1].x += cabs2(x0[e]);
+            if constexpr (DOTS) {
+                acc[0] = cfma_conj(x0[e], y[e], acc[0]);
+                acc[1].x += cabs2(x0[e]);
+            }
         }
         __syncthreads(); // xl and dl are rewritten for the next tile
     }
+    if constexpr (!DOTS) return;
 #pragma unroll
     for (int s = W; s < kWave; s <<= 1)
 #pragma unroll
@@ -1449,15 +1453,22 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave, rpg = tile_rpg(w, nw);
             using TileFn = void (*)(const KronDev *, const uint64_t *, const double2 *, const double2 *, int64_t,
                                     int64_t, int, const double2 *, double2 *, double2 *, int64_t, int, int, int, int);
-            static const TileFn tfn[2][3][2] = {
-                {{k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>},
-                 {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>},
-                 {k_spmm_kron_tiles<2, 2, 16>, k_spmm_kron_tiles<3, 2, 16>}},
-                {{k_spmm_kron_tiles<2, 3, 4>, k_spmm_kron_tiles<3, 3, 4>},
-                 {k_spmm_kron_tiles<2, 3, 8>, k_spmm_kron_tiles<3, 3, 8>},
-                 {k_spmm_kron_tiles<2, 3, 16>, k_spmm_kron_tiles<3, 3, 16>}}};
+            static const TileFn tfn[2][2][3][2] = {
+                {{{k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>},
+                  {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>},
+                  {k_spmm_kron_tiles<2, 2, 16>, k_spmm_kron_tiles<3, 2, 16>}},
+                 {{k_spmm_kron_tiles<2, 3, 4>, k_spmm_kron_tiles<3, 3, 4>},
+                  {k_spmm_kron_tiles<2, 3, 8>, k_spmm_kron_tiles<3, 3, 8>},
+                  {k_spmm_kron_tiles<2, 3, 16>, k_spmm_kron_tiles<3, 3, 16>}}},
+                {{{k_spmm_kron_tiles<2, 2, 4, false>, k_spmm_kron_tiles<3, 2, 4, false>},
+                  {k_spmm_kron_tiles<2, 2, 8, false>, k_spmm_kron_tiles<3, 2, 8, false>},
+                  {k_spmm_kron_tiles<2, 2, 16, false>, k_spmm_kron_tiles<3, 2, 16, false>}},
+                 {{k_spmm_kron_tiles<2, 3, 4, false>, k_spmm_kron_tiles<3, 3, 4, false>},
+                  {k_spmm_kron_tiles<2, 3, 8, false>, k_spmm_kron_tiles<3, 3, 8, false>},
+                  {k_spmm_kron_tiles<2, 3, 16, false>, k_spmm_kron_tiles<3, 3, 16, false>}}}};
             const int ti = nw == 4 ? 0 : nw == 8 ? 1 : 2;
-            hipLaunchKernelGGL(tfn[q - 2][ti][std::min(group, 3) - 2], dim3(nrb * nsl), dim3(64 * nw), 0, s, w.kron,
+            hipLaunchKernelGGL(tfn[w.no_dots ? 1 : 0][q - 2][ti][std::min(group, 3) - 2], dim3(nrb * nsl), dim3(64 * nw),
+                               0, s, w.kron,
                                w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
                                w.Mp, rpg, nrb, tile_order(w, q + (nw == 16 ? 4 : nw == 8 ? 3 : 2), kWave >> q));
@@ -1836,9 +1847,11 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
     auto product = [&](double2 *in, double2 *out) {
         w.u = in;
         w.v = out;
+        w.no_dots = true;
         const hipError_t e = spmm_launch(w, s);
         w.u = u0;
         w.v = v0;
+        w.no_dots = false;
         return e;
     };
     hipError_t e = hipSuccess;
