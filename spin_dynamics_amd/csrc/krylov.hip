@@ -1514,7 +1514,7 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 // Gram matrix is not positive definite raises the breakdown flag.
 constexpr int kSMaxCols = 40;  // K + n columns of one pass (at most m + 1)
 constexpr int kSMaxNY = 8;     // chain vectors per block (S, and the last block's extra one)
-constexpr int kSGroup = 4;     // dot columns per workgroup of the dots pass
+constexpr int kSGroup = 6;     // dot columns per workgroup of the dots pass
 constexpr int kSLanes = 16;    // lanes per workgroup of the forming pass
 constexpr int kSCoefLanes = 8; // lanes per workgroup of the coefficient kernel
 struct ColPtrs {
