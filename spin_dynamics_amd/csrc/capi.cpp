@@ -365,18 +365,18 @@ struct LaneSet {
     std::vector<double> obs0, lt;
 };
 
-// Device buffers of lane set L; set 0 uses the plain names (shared with the
-// other entry points), sets 1-3 the same names with the suffix "#idx".
-// Block size of the s-step Arnoldi on the trace-sampling path: SPC_SSTEP
-// (0 or 1: one column per pass; at most 7), read at every run.  Default 6:
-// N = 36 864, M = 256, m = 24 on MI355X steps in 9.4 ms (14.2 ms column by
-// column; 9.9 at 5, 9.6 at 7).
+// Block size of the s-step Arnoldi (trace sampling, batches, gradient):
+// SPC_SSTEP (0 or 1: one column per pass; at most 7), read at every run.
+// Default 6: N = 36 864, M = 256, m = 24 on MI355X steps in 8.2 ms (14.2 ms
+// column by column; 9.6 at 4, 9.1 at 5, 8.7 at 7).
 int sstep_block()
 {
     const char *e = std::getenv("SPC_SSTEP");
     return e ? std::clamp(std::atoi(e), 0, 7) : 6;
 }
 
+// Device buffers of lane set L; set 0 uses the plain names (shared with the
+// other entry points), sets 1-3 the same names with the suffix "#idx".
 int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, int64_t n_out, bool need_gram,
                    int vkind)
 {
