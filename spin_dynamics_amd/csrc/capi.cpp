@@ -366,13 +366,13 @@ struct LaneSet {
 };
 
 // Block size of the s-step Arnoldi (trace sampling, batches, gradient):
-// SPC_SSTEP (0 or 1: one column per pass; at most 7), read at every run.
-// Default 6: N = 36 864, M = 256, m = 24 on MI355X steps in 8.2 ms (14.2 ms
-// column by column; 9.6 at 4, 9.1 at 5, 8.7 at 7).
+// SPC_SSTEP (0 or 1: one column per pass; at most 8), read at every run.
+// Default 8: N = 36 864, M = 256, m = 24 on MI355X steps in 7.4 ms (7.9 at 6,
+// 14.2 column by column).
 int sstep_block()
 {
     const char *e = std::getenv("SPC_SSTEP");
-    return e ? std::clamp(std::atoi(e), 0, 7) : 6;
+    return e ? std::clamp(std::atoi(e), 0, 8) : 8;
 }
 
 // Device buffers of lane set L; set 0 uses the plain names (shared with the

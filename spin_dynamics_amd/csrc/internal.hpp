@@ -215,7 +215,7 @@ struct KrylovWork {
     double2 *sbuf = nullptr;  // its dots and forming coefficients, kSStepBuf x Mp
 };
 // Elements per lane of KrylovWork::sbuf.
-constexpr int kSStepBuf = 640;
+constexpr int kSStepBuf = 720;
 // Row blocks of the s-step dots pass for dimension N (8 .. 256, N/128 rounded
 // up to a multiple of 8), and the partials it needs for order m, block size S.
 int sstep_row_blocks(int64_t N);

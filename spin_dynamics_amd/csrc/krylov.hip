@@ -1513,10 +1513,11 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 // Per block the basis is read twice instead of once per column; a lane whose
 // Gram matrix is not positive definite raises the breakdown flag.
 constexpr int kSMaxCols = 40;  // K + n columns of one pass (at most m + 1)
-constexpr int kSMaxNY = 8;     // chain vectors per block (S, and the last block's extra one)
-constexpr int kSGroup = 6;     // dot columns per workgroup of the dots pass
+constexpr int kSMaxNY = 9;     // chain vectors per block (S, and the last block's extra one)
+// dot columns per workgroup of the dots pass (4 for 8 and 9 chain vectors: registers)
+constexpr int sgroup(int ny) { return ny >= 8 ? 4 : 6; }
 constexpr int kSLanes = 16;    // lanes per workgroup of the forming pass
-constexpr int kSCoefLanes = 8; // lanes per workgroup of the coefficient kernel
+constexpr int kSCoefLanes = 4; // lanes per workgroup of the coefficient kernel
 struct ColPtrs {
     const double2 *p[kSMaxCols];
 };
@@ -1525,14 +1526,14 @@ struct OutPtrs {
 };
 
 // part[rb][x*NY + j][r] = sum over the block's rows of conj(x_x) y_j, for the
-// kSGroup columns x of group g; y_j = x_{K+j}.  Workgroup -> (g, row block,
+// sgroup(NY) columns x of group g; y_j = x_{K+j}.  Workgroup -> (g, row block,
 // lane chunk) with the groups of one row block consecutive on one XCD (they
 // read the same rows of Y).
 template <int NY>
 __global__ __launch_bounds__(kBlock) void k_sdots(ColPtrs x, int nx, int K, double2 *__restrict__ part, int64_t N,
                                                   int Mp, int64_t rpb, int nrb, int ngroups)
 {
-    constexpr int KG = kSGroup;
+    constexpr int KG = sgroup(NY);
     __shared__ double2 red[kWaves - 1][KG][kWave];
     const int nb = nrb * (Mp / kWave) * ngroups;
     int bid = blockIdx.x;
@@ -1809,10 +1810,10 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
 
 using SDotsFn = void (*)(ColPtrs, int, int, double2 *, int64_t, int, int64_t, int, int);
 using SFormFn = void (*)(ColPtrs, int, OutPtrs, const double2 *, int64_t, int, int64_t);
-const SDotsFn kSDots[kSMaxNY] = {k_sdots<1>, k_sdots<2>, k_sdots<3>, k_sdots<4>,
-                                 k_sdots<5>, k_sdots<6>, k_sdots<7>, k_sdots<8>};
-const SFormFn kSForm[kSMaxNY] = {k_sform<1>, k_sform<2>, k_sform<3>, k_sform<4>,
-                                 k_sform<5>, k_sform<6>, k_sform<7>, k_sform<8>};
+const SDotsFn kSDots[kSMaxNY] = {k_sdots<1>, k_sdots<2>, k_sdots<3>, k_sdots<4>, k_sdots<5>,
+                                 k_sdots<6>, k_sdots<7>, k_sdots<8>, k_sdots<9>};
+const SFormFn kSForm[kSMaxNY] = {k_sform<1>, k_sform<2>, k_sform<3>, k_sform<4>, k_sform<5>,
+                                 k_sform<6>, k_sform<7>, k_sform<8>, k_sform<9>};
 
 // Row blocks of the dots pass: a multiple of 8 (XCD mapping), as many as the
 // partials buffer holds for the largest pass, at most sstep_row_blocks(N).
@@ -1825,7 +1826,7 @@ int sstep_nrb(const KrylovWork &w, int S)
 
 hipError_t sdots_pass(KrylovWork &w, const ColPtrs &x, int nx, int K, int n, int nrb, hipStream_t s)
 {
-    const int ngroups = (nx + kSGroup - 1) / kSGroup;
+    const int ngroups = (nx + sgroup(n) - 1) / sgroup(n);
     const int64_t rpb = ceil_div(w.N, nrb);
     {
         ProfScope ps(w, PF_SDOTS, nx * vec_bytes(w), s);
@@ -1894,6 +1895,7 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
             const int nx = K + n;
             ProfScope ps(w, PF_SFORM, (nx + n) * vec_bytes(w), s);
             const size_t lds = sizeof(double2) * (size_t)nx * n * kSLanes;
+            if (lds > 65536) return hipErrorInvalidValue; // m <= 32, n <= 8: at most 64 KiB
             hipLaunchKernelGGL(kSForm[n - 1], dim3(nbf), dim3(kBlock), lds, s, x, nx, o, scoef, w.N, w.Mp, rpf);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -29,7 +29,7 @@ def _pair(N):
     return [q for q in batch.radical_pairs() if q.N == N][0]
 
 
-@pytest.mark.parametrize("S", ["2", "6", "7"])
+@pytest.mark.parametrize("S", ["2", "6", "7", "8"])
 def test_block_basis_arnoldi_relation(ctx, monkeypatch, S):
     # the block basis and Hessenberg matrix through spc_krylov_basis: the
     # Arnoldi relation H Q[:, :m-1] = Q A[:, :m-1] holds to rounding (an index
@@ -56,8 +56,9 @@ def test_block_basis_arnoldi_relation(ctx, monkeypatch, S):
         assert np.all(A[r][np.tril_indices(m, -2)] == 0)  # Hessenberg: zero below the subdiagonal
 
 
-def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch):
-    monkeypatch.setenv("SPC_SSTEP", "6")
+@pytest.mark.parametrize("S", ["6", "8"])
+def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch, S):
+    monkeypatch.setenv("SPC_SSTEP", S)
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=1.5)  # two steps
     res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     s = oracle_system(w.sys, w.B)
@@ -67,7 +68,7 @@ def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch):
         assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
 
 
-@pytest.mark.parametrize("S", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("S", [2, 3, 4, 5, 6, 7, 8])
 def test_every_block_size_matches_the_column_path(ctx, monkeypatch, S):
     p = _pair(3072)
     sim = batch.orientation_sim(T_ns=7.5)  # eight steps, m = 24, M = 64
@@ -81,7 +82,7 @@ def test_every_block_size_matches_the_column_path(ctx, monkeypatch, S):
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (S, k)
 
 
-@pytest.mark.parametrize("m,S", [(8, 6), (9, 6), (13, 6), (12, 2), (29, 7), (32, 6), (32, 7)])
+@pytest.mark.parametrize("m,S", [(8, 6), (9, 6), (13, 6), (12, 2), (29, 7), (32, 6), (32, 7), (9, 8), (24, 8), (32, 8)])
 def test_block_schedules_match_the_column_path(ctx, monkeypatch, m, S):
     # the smallest order the block form takes (8), orders that leave a short
     # last block, the largest order (32) and the widest chain (S = 7: a last
@@ -98,8 +99,9 @@ def test_block_schedules_match_the_column_path(ctx, monkeypatch, m, S):
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (m, S, k)
 
 
-def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch):
-    monkeypatch.setenv("SPC_SSTEP", "6")
+@pytest.mark.parametrize("S", ["6", "8"])
+def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch, S):
+    monkeypatch.setenv("SPC_SSTEP", S)
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=39.5)  # 40 steps
     k = 1.5 / GAMMA_E
     w.sys.kS = w.sys.kT = k
@@ -112,14 +114,15 @@ def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch):
     assert np.max(np.abs(tot - np.exp(-k * t))) < 1e-12
 
 
-def test_sstep_with_substep_outputs(ctx, monkeypatch):
+@pytest.mark.parametrize("S", ["6", "8"])
+def test_sstep_with_substep_outputs(ctx, monkeypatch, S):
     p = _pair(3072)
     sim = batch.orientation_sim(T_ns=7.5)
     sim.tol = 1e-8
     s = copy.deepcopy(p.sys)
     monkeypatch.setenv("SPC_SSTEP", "0")
     col = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
-    monkeypatch.setenv("SPC_SSTEP", "6")
+    monkeypatch.setenv("SPC_SSTEP", S)
     blk = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     for k in KEYS:
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-10, k
