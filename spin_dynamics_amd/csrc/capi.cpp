@@ -365,14 +365,32 @@ struct LaneSet {
     std::vector<double> obs0, lt;
 };
 
-// Block size of the s-step Arnoldi (trace sampling, batches, gradient):
-// SPC_SSTEP (0 or 1: one column per pass; at most 8), read at every run.
-// Default 8: N = 36 864, M = 256, m = 24 on MI355X steps in 7.4 ms (7.9 at 6,
-// 14.2 column by column).
-int sstep_block()
+// Vector passes of the block Arnoldi's dots and forming passes for order m
+// in blocks of S (first block S columns, then S at a time, the last block
+// with its extra chain vector): sum over blocks of (K + n + extra) + (K + 2n).
+int sstep_passes(int m, int S)
 {
-    const char *e = std::getenv("SPC_SSTEP");
-    return e ? std::clamp(std::atoi(e), 0, 8) : 8;
+    int K = std::min(S, m - 1), tot = 3 * K;
+    while (K < m) {
+        const int n = std::min(S, m - K), extra = K + n == m ? 1 : 0;
+        tot += (K + n + extra) + (K + 2 * n);
+        K += n;
+    }
+    return tot;
+}
+
+// Block size of the s-step Arnoldi for order m (trace sampling, batches,
+// gradient): SPC_SSTEP (0 or 1: one column per pass; at most 8), read at every
+// run; by default the S in 2..8 with the fewest vector passes (the larger on a
+// tie): m = 24 takes 8 (N = 36 864, M = 256 on MI355X: 7.4 ms per step; 7.9 at
+// 6, 14.2 column by column), m = 20 takes 7, m = 12 takes 6, m = 9 and 10 take 5.
+int sstep_block(int m)
+{
+    if (const char *e = std::getenv("SPC_SSTEP")) return std::clamp(std::atoi(e), 0, 8);
+    int best = 8;
+    for (int S = 8; S >= 2; --S)
+        if (sstep_passes(m, S) < sstep_passes(m, best)) best = S;
+    return best;
 }
 
 // Device buffers of lane set L; set 0 uses the plain names (shared with the
@@ -408,7 +426,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
     size_t part_need = partials_needed(w);
     if (need_gram) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
-    w.sstep = sstep_block();
+    w.sstep = sstep_block(m);
     if (w.sstep >= 2) part_need = std::max(part_need, sstep_partials(N, m, w.sstep, Mp));
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
@@ -959,7 +977,7 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
-    w.sstep = sstep_block();
+    w.sstep = sstep_block(m);
     const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
@@ -1551,7 +1569,7 @@ static int hook_work(spc_ctx *ctx, int M, int m, KrylovWork &w)
     w.kT = ctx->kT;
     w.rpw1 = w.rpw2 = w.rpwp = 4;
     // the hooks check the column form unless SPC_SSTEP is set explicitly
-    w.sstep = std::getenv("SPC_SSTEP") ? sstep_block() : 0;
+    w.sstep = std::getenv("SPC_SSTEP") ? sstep_block(w.m) : 0;
     int rc;
     if ((rc = ensure(ctx, "Q", (size_t)w.m * w.N * Mp, &w.Q))) return rc;
     if ((rc = ensure(ctx, "psi", (size_t)w.N * Mp, &w.psi))) return rc;
@@ -1987,7 +2005,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
-    w.sstep = sstep_block();
+    w.sstep = sstep_block(m);
     const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
     w.part_capacity = part_need;
