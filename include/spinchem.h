@@ -128,7 +128,7 @@ int spc_expm_small(spc_ctx *ctx, int32_t m, const double *A, double *E);
  * [M][N] in, Q [M][m][N] (vector-major), A [M][m][m] (column-major) and
  * c0 = ||psi|| [M] out.  The product path keeps these on the device; this
  * entry point exists to check the Arnoldi kernels: the one-column-per-pass
- * form, or the block s-step form when SPC_SSTEP is set (2..7). */
+ * form, or the block s-step form when SPC_SSTEP is set (2..8). */
 int spc_krylov_basis(spc_ctx *ctx, int32_t M, int32_t m, const double *psi, double *Q, double *A,
                      double *beta);
 
