@@ -1514,8 +1514,8 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
 // Gram matrix is not positive definite raises the breakdown flag.
 constexpr int kSMaxCols = 40;  // K + n columns of one pass (at most m + 1)
 constexpr int kSMaxNY = 9;     // chain vectors per block (S, and the last block's extra one)
-// dot columns per workgroup of the dots pass (4 for 8 and 9 chain vectors: registers)
-constexpr int sgroup(int ny) { return ny >= 8 ? 4 : 6; }
+// dot columns per workgroup of the dots pass (5 for 8 and 9 chain vectors: registers)
+constexpr int sgroup(int ny) { return ny >= 8 ? 5 : 6; }
 constexpr int kSLanes = 16;    // lanes per workgroup of the forming pass
 constexpr int kSCoefLanes = 4; // lanes per workgroup of the coefficient kernel
 struct ColPtrs {
