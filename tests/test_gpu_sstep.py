@@ -56,7 +56,7 @@ def test_block_basis_arnoldi_relation(ctx, monkeypatch, S):
         assert np.all(A[r][np.tril_indices(m, -2)] == 0)  # Hessenberg: zero below the subdiagonal
 
 
-@pytest.mark.parametrize("S", ["6", "8"])
+@pytest.mark.parametrize("S", ["0", "6", "8"])  # 0: the one-column-per-pass form
 def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch, S):
     monkeypatch.setenv("SPC_SSTEP", S)
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=1.5)  # two steps
