@@ -281,10 +281,7 @@ def get_kinetics(P_S, P_T0, P_Tp, P_Tm, dt: float, kS: float, kT: float):
     return iden, phis[:n // 2], phit[:n // 2]
 
 
-def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1), J: float = 0.0,
-                  kS: float = 0.0, kT: float = 0.0, n_samples: int, dt: float, n_steps_: int,
-                  seed=(42, 99), n_threads: int = 0, with_pt: bool = False):
-    """P_S(t) (and P_T(t) with with_pt) of the semiclassical model, n_steps_ + 1 values."""
+def _sc_system(a_iso1, a_iso2, mult1, mult2, B, direction, J, kS, kT):
     a = np.ascontiguousarray(list(a_iso1) + list(a_iso2), dtype=float)
     mult = np.ascontiguousarray(list(mult1) + list(mult2), dtype=np.intc)
     s = OrcScSystem()
@@ -294,6 +291,51 @@ def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1)
     s.B, s.J, s.kS, s.kT = B, J, kS, kT
     for i in range(3):
         s.dir[i] = direction[i]
+    return s, (a, mult)
+
+
+def semiclassical_trajectory(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1), J: float = 0.0,
+                             kS: float = 0.0, kT: float = 0.0, dt: float, n_steps_: int, sample: int = 0,
+                             seed=(42, 99)):
+    """States of one sample of the semiclassical run after every step:
+    x [n_steps_+1][16] (S1, S2, T12 column-major, x_16) and the nuclear
+    vectors I [n_steps_+1][n1+n2][3]."""
+    s, keep = _sc_system(a_iso1, a_iso2, mult1, mult2, B, direction, J, kS, kT)
+    nn = len(a_iso1) + len(a_iso2)
+    xs = np.zeros((n_steps_ + 1, 16))
+    Is = np.zeros((n_steps_ + 1, max(nn, 1), 3))
+    rc = lib().orc_semiclassical_trajectory(ctypes.byref(s), (ctypes.c_int64 * 2)(*seed), ctypes.c_int64(sample),
+                                            ctypes.c_double(dt), ctypes.c_int64(n_steps_), _dp(xs), _dp(Is))
+    if rc:
+        raise RuntimeError(f"orc_semiclassical_trajectory failed with code {rc}")
+    del keep
+    return xs, Is[:, :nn]
+
+
+SC_MUTATIONS = {"none": 0, "t12_transposed": 1, "nuclear_sign": 2, "nuclear_scale": 3, "field_scale": 4}
+
+
+class sc_mutation:
+    """Context manager planting one of the oracle's semiclassical mistakes
+    (tests that show a pin catches it)."""
+
+    def __init__(self, name: str):
+        self.which = SC_MUTATIONS[name]
+
+    def __enter__(self):
+        lib().orc_sc_set_mutation(ctypes.c_int(self.which))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_sc_set_mutation(ctypes.c_int(0))
+        return False
+
+
+def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1), J: float = 0.0,
+                  kS: float = 0.0, kT: float = 0.0, n_samples: int, dt: float, n_steps_: int,
+                  seed=(42, 99), n_threads: int = 0, with_pt: bool = False):
+    """P_S(t) (and P_T(t) with with_pt) of the semiclassical model, n_steps_ + 1 values."""
+    s, keep = _sc_system(a_iso1, a_iso2, mult1, mult2, B, direction, J, kS, kT)
     out = np.zeros(n_steps_ + 1)
     out_t = np.zeros(n_steps_ + 1)
     rc = lib().orc_semiclassical(ctypes.byref(s), ctypes.c_int64(n_samples), (ctypes.c_int64 * 2)(*seed),

@@ -1085,7 +1085,13 @@ static void sc_generator(double kS, double kT, double J, double *G)
 
 static double sc_ps(const double *x) { return 0.25 * x[15] - (x[6] + x[10] + x[14]); }
 
-/* Electron half step: x <- KJ x, then rotate S1, S2 and T12 columns/rows. */
+/* Planted mistakes for the tests that show the semiclassical pins catch them
+ * (0 = the model as stated; never set outside those tests). */
+static int sc_mutation = 0;
+void orc_sc_set_mutation(int which) { sc_mutation = which; }
+
+/* Electron half step: x <- KJ x, then rotate S1, S2 and T12 columns/rows
+ * (class_radical_pair.f90:100-118). */
 static void sc_electron_half(double *x, const double *KJ, const double *w1, const double *w2, double t)
 {
     double y[16];
@@ -1097,101 +1103,136 @@ static void sc_electron_half(double *x, const double *KJ, const double *w1, cons
     memcpy(x, y, sizeof y);
     sc_rotate(x + 0, w1, t);
     sc_rotate(x + 3, w2, t);
-    for (int col = 0; col < 3; ++col) sc_rotate(x + 6 + 3 * col, w1, t); /* T(:,b) rotates with S1 */
-    for (int row = 0; row < 3; ++row) {                                  /* T(a,:) rotates with S2 */
+    /* T(:,b) = S1 S2_b rotates with S1, T(a,:) = S1_a S2 with S2 */
+    const double *wc = sc_mutation == ORC_SC_MUT_T12_TRANSPOSED ? w2 : w1;
+    const double *wr = sc_mutation == ORC_SC_MUT_T12_TRANSPOSED ? w1 : w2;
+    for (int col = 0; col < 3; ++col) sc_rotate(x + 6 + 3 * col, wc, t);
+    for (int row = 0; row < 3; ++row) {
         double r[3] = {x[6 + row], x[9 + row], x[12 + row]};
-        sc_rotate(r, w2, t);
+        sc_rotate(r, wr, t);
         x[6 + row] = r[0];
         x[9 + row] = r[1];
         x[12 + row] = r[2];
     }
 }
 
+/* Shared set-up of a run: KJ = exp(dt/2 (K + J)) (sc_dynamics.f90:102-185),
+ * the Zeeman field -B n and the per-sample seeds (two next() draws each). */
+typedef struct {
+    double KJ[256];
+    double wz[3];
+    int nn;
+} sc_setup_t;
+
+static int sc_setup(const orc_sc_system *s, double dt, sc_setup_t *u)
+{
+    double G[256];
+    sc_generator(s->kS, s->kT, s->J, G);
+    double Gz[512], Ez[512];
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) { /* complex, column-major, scaled by dt/2 */
+            Gz[2 * (i + 16 * j)] = 0.5 * dt * G[16 * i + j];
+            Gz[2 * (i + 16 * j) + 1] = 0.0;
+        }
+    orc_expm(16, Gz, Ez);
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) u->KJ[16 * i + j] = Ez[2 * (i + 16 * j)];
+    const double dn = sqrt(s->dir[0] * s->dir[0] + s->dir[1] * s->dir[1] + s->dir[2] * s->dir[2]);
+    if (!(dn > 0.0)) return 1;
+    for (int a = 0; a < 3; ++a) u->wz[a] = -s->B * s->dir[a] / dn;
+    u->nn = s->n1 + s->n2;
+    return 0;
+}
+
+static void sc_master_seeds(const int64_t master_seed[2], int64_t n, int64_t *seeds)
+{
+    orc_rng r;
+    orc_rng_init_default(&r);
+    orc_rng_set_seed(&r, master_seed);
+    for (int64_t i = 0; i < n; ++i) {
+        seeds[2 * i] = orc_rng_next(&r);
+        seeds[2 * i + 1] = orc_rng_next(&r);
+    }
+}
+
+/* One sample: S1, S2 on spheres of radius sqrt(3/4), I_k of radius
+ * sqrt(I(I+1)) (m_random.f90:300-318), T12 = S1 S2^T, x_16 = 1, then n_steps
+ * Strang steps: electron half step, nuclear full step about
+ * a_k S_e sqrt(3/4)/|S_e| (class_radical_pair.f90:123-145), electron half
+ * step.  out[st] = P_S(0) P_S(x(st)), out_t[st] = P_S(0) x_16(st); with xs/Is
+ * non-NULL the state after every step ([n_steps+1][16], [n_steps+1][3 nn]). */
+static void sc_sample(const orc_sc_system *s, const sc_setup_t *u, const int64_t seed[2], double dt, int64_t n_steps,
+                      double *out, double *out_t, double *xs, double *Is)
+{
+    const int nn = u->nn;
+    const double shalf = sqrt(0.75);
+    double x[16];
+    double *Iv = (double *)malloc(sizeof(double) * 3 * (nn ? nn : 1));
+    orc_rng r;
+    orc_rng_init_default(&r);
+    orc_rng_set_seed(&r, seed);
+    orc_rng_sphere(&r, shalf, x + 0);
+    orc_rng_sphere(&r, shalf, x + 3);
+    for (int k = 0; k < nn; ++k) {
+        double I = 0.5 * (s->mult[k] - 1);
+        orc_rng_sphere(&r, sqrt(I * (I + 1.0)), Iv + 3 * k);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) x[6 + a + 3 * b] = x[a] * x[3 + b];
+    x[15] = 1.0;
+    const double p0 = sc_ps(x);
+    const double nuc_sign = sc_mutation == ORC_SC_MUT_NUCLEAR_SIGN ? -1.0 : 1.0;
+    const double nuc_scale = sc_mutation == ORC_SC_MUT_NUCLEAR_SCALE ? 2.0 : 1.0;
+    const double el_scale = sc_mutation == ORC_SC_MUT_FIELD_SCALE ? 0.5 : 1.0;
+    out[0] = p0 * p0;
+    out_t[0] = p0 * x[15];
+    if (xs) memcpy(xs, x, sizeof x);
+    if (Is) memcpy(Is, Iv, sizeof(double) * 3 * nn);
+    for (int64_t st = 1; st <= n_steps; ++st) {
+        for (int half = 0; half < 2; ++half) {
+            double w1[3] = {u->wz[0], u->wz[1], u->wz[2]}, w2[3] = {u->wz[0], u->wz[1], u->wz[2]};
+            for (int k = 0; k < nn; ++k) {
+                double *w = k < s->n1 ? w1 : w2;
+                for (int a = 0; a < 3; ++a) w[a] += el_scale * s->a_iso[k] * Iv[3 * k + a];
+            }
+            sc_electron_half(x, u->KJ, w1, w2, 0.5 * dt);
+            if (half == 0) {
+                double n1 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                double n2 = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+                double f1 = n1 > 0.0 ? shalf / n1 : 0.0, f2 = n2 > 0.0 ? shalf / n2 : 0.0;
+                for (int k = 0; k < nn; ++k) {
+                    const double *S = k < s->n1 ? x : x + 3;
+                    double f = nuc_sign * nuc_scale * (k < s->n1 ? f1 : f2) * s->a_iso[k];
+                    double fld[3] = {f * S[0], f * S[1], f * S[2]};
+                    sc_rotate(Iv + 3 * k, fld, dt);
+                }
+            }
+        }
+        out[st] = p0 * sc_ps(x);
+        out_t[st] = p0 * x[15];
+        if (xs) memcpy(xs + 16 * st, x, sizeof x);
+        if (Is) memcpy(Is + 3 * nn * st, Iv, sizeof(double) * 3 * nn);
+    }
+    free(Iv);
+}
+
 int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t master_seed[2], double dt,
                       int64_t n_steps, double *P_S, double *P_T, int n_threads)
 {
-    const int nn = s->n1 + s->n2;
     if (n_samples < 1 || n_steps < 0) return 1;
-    double G[256], KJ[256];
-    sc_generator(s->kS, s->kT, s->J, G);
-    {
-        double Gz[512];
-        for (int i = 0; i < 16; ++i)
-            for (int j = 0; j < 16; ++j) { /* complex, column-major, scaled by dt/2 */
-                Gz[2 * (i + 16 * j)] = 0.5 * dt * G[16 * i + j];
-                Gz[2 * (i + 16 * j) + 1] = 0.0;
-            }
-        double Ez[512];
-        orc_expm(16, Gz, Ez);
-        for (int i = 0; i < 16; ++i)
-            for (int j = 0; j < 16; ++j) KJ[16 * i + j] = Ez[2 * (i + 16 * j)];
-    }
-    /* the field is B along the unit vector of dir */
-    const double dn = sqrt(s->dir[0] * s->dir[0] + s->dir[1] * s->dir[1] + s->dir[2] * s->dir[2]);
-    if (!(dn > 0.0)) return 1;
-    const double wz[3] = {-s->B * s->dir[0] / dn, -s->B * s->dir[1] / dn, -s->B * s->dir[2] / dn};
-    const double shalf = sqrt(0.75);
+    sc_setup_t u;
+    if (sc_setup(s, dt, &u)) return 1;
     int64_t *seeds = (int64_t *)malloc(sizeof(int64_t) * 2 * n_samples);
-    {
-        orc_rng r;
-        orc_rng_init_default(&r);
-        orc_rng_set_seed(&r, master_seed);
-        for (int64_t i = 0; i < n_samples; ++i) {
-            seeds[2 * i] = orc_rng_next(&r);
-            seeds[2 * i + 1] = orc_rng_next(&r);
-        }
-    }
+    sc_master_seeds(master_seed, n_samples, seeds);
     double *contrib = (double *)malloc(sizeof(double) * (n_steps + 1) * (size_t)n_samples);
     double *contrib_t = (double *)malloc(sizeof(double) * (n_steps + 1) * (size_t)n_samples);
 #ifdef _OPENMP
     int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
 #endif
-    for (int64_t smp = 0; smp < n_samples; ++smp) {
-        double x[16];
-        double *Iv = (double *)malloc(sizeof(double) * 3 * (nn ? nn : 1));
-        orc_rng r;
-        orc_rng_init_default(&r);
-        orc_rng_set_seed(&r, seeds + 2 * smp);
-        orc_rng_sphere(&r, shalf, x + 0);
-        orc_rng_sphere(&r, shalf, x + 3);
-        for (int k = 0; k < nn; ++k) {
-            double I = 0.5 * (s->mult[k] - 1);
-            orc_rng_sphere(&r, sqrt(I * (I + 1.0)), Iv + 3 * k);
-        }
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) x[6 + a + 3 * b] = x[a] * x[3 + b];
-        x[15] = 1.0;
-        double p0 = sc_ps(x);
-        double *out = contrib + (n_steps + 1) * (size_t)smp;
-        double *out_t = contrib_t + (n_steps + 1) * (size_t)smp;
-        out[0] = p0 * p0;
-        out_t[0] = p0 * x[15];
-        for (int64_t st = 1; st <= n_steps; ++st) {
-            for (int half = 0; half < 2; ++half) {
-                double w1[3] = {wz[0], wz[1], wz[2]}, w2[3] = {wz[0], wz[1], wz[2]};
-                for (int k = 0; k < nn; ++k) {
-                    double *w = k < s->n1 ? w1 : w2;
-                    for (int a = 0; a < 3; ++a) w[a] += s->a_iso[k] * Iv[3 * k + a];
-                }
-                sc_electron_half(x, KJ, w1, w2, 0.5 * dt);
-                if (half == 0) { /* nuclear full step about a_k S_e normalised to sqrt(3/4) */
-                    double n1 = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-                    double n2 = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-                    double f1 = n1 > 0.0 ? shalf / n1 : 0.0, f2 = n2 > 0.0 ? shalf / n2 : 0.0;
-                    for (int k = 0; k < nn; ++k) {
-                        const double *S = k < s->n1 ? x : x + 3;
-                        double f = (k < s->n1 ? f1 : f2) * s->a_iso[k];
-                        double fld[3] = {f * S[0], f * S[1], f * S[2]};
-                        sc_rotate(Iv + 3 * k, fld, dt);
-                    }
-                }
-            }
-            out[st] = p0 * sc_ps(x);
-            out_t[st] = p0 * x[15];
-        }
-        free(Iv);
-    }
+    for (int64_t smp = 0; smp < n_samples; ++smp)
+        sc_sample(s, &u, seeds + 2 * smp, dt, n_steps, contrib + (n_steps + 1) * (size_t)smp,
+                  contrib_t + (n_steps + 1) * (size_t)smp, NULL, NULL);
     /* P_T = x_16 - P_S per sample (class_radical_pair.f90:174-175) */
     for (int64_t st = 0; st <= n_steps; ++st) {
         double acc = 0.0, acc_t = 0.0;
@@ -1204,6 +1245,21 @@ int orc_semiclassical(const orc_sc_system *s, int64_t n_samples, const int64_t m
     }
     free(contrib);
     free(contrib_t);
+    free(seeds);
+    return 0;
+}
+
+int orc_semiclassical_trajectory(const orc_sc_system *s, const int64_t master_seed[2], int64_t sample, double dt,
+                                 int64_t n_steps, double *xs, double *Is)
+{
+    if (sample < 0 || n_steps < 0 || !xs) return 1;
+    sc_setup_t u;
+    if (sc_setup(s, dt, &u)) return 1;
+    int64_t *seeds = (int64_t *)malloc(sizeof(int64_t) * 2 * (sample + 1));
+    sc_master_seeds(master_seed, sample + 1, seeds);
+    double *o = (double *)malloc(sizeof(double) * 2 * (n_steps + 1));
+    sc_sample(s, &u, seeds + 2 * sample, dt, n_steps, o, o + n_steps + 1, xs, Is);
+    free(o);
     free(seeds);
     return 0;
 }

@@ -22,8 +22,11 @@
  *   - orc_trace_sampling,
  *     orc_exact_dynamics   : dense scipy propagation, kS=kT exact decay,
  *                            no-nuclei closed form
- *   - orc_semiclassical    : Rodrigues closed form, norm conservation, Larmor
- *                            precession (no reference output exists)
+ *   - orc_semiclassical    : no reference output exists; pinned by the uniform
+ *                            decay at kS = kT, T12 = S1 S2^T in every sample
+ *                            at K = J = 0, conservation of S + I for one
+ *                            electron and one nucleus at zero field, and the
+ *                            Schulten-Wolynes frozen-nuclei closed form
  */
 #ifndef SPINCHEM_ORACLE_H
 #define SPINCHEM_ORACLE_H
@@ -185,6 +188,24 @@ typedef struct {
 int orc_semiclassical(const orc_sc_system *s, int64_t n_samples,
                       const int64_t master_seed[2], double dt, int64_t n_steps,
                       double *P_S, double *P_T, int n_threads);
+
+/* The state of sample `sample` of the same run after every step: xs
+ * [n_steps+1][16] (S1, S2, T12 column-major, x_16) and, if non-NULL, Is
+ * [n_steps+1][3 (n1+n2)] the nuclear spin vectors. */
+int orc_semiclassical_trajectory(const orc_sc_system *s, const int64_t master_seed[2],
+                                 int64_t sample, double dt, int64_t n_steps, double *xs,
+                                 double *Is);
+
+/* Planted mistakes, for the tests that show the pins above catch them:
+ * T12's columns turned with S2's axis and its rows with S1's, the nuclear
+ * axis with the wrong sign or twice its length, the electron hyperfine field
+ * at half its length.  0 restores the model. */
+#define ORC_SC_MUT_NONE 0
+#define ORC_SC_MUT_T12_TRANSPOSED 1
+#define ORC_SC_MUT_NUCLEAR_SIGN 2
+#define ORC_SC_MUT_NUCLEAR_SCALE 3
+#define ORC_SC_MUT_FIELD_SCALE 4
+void orc_sc_set_mutation(int which);
 
 #ifdef __cplusplus
 }
