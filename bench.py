@@ -34,10 +34,20 @@ METRIC = ("state·steps/s (dim×M×steps) at 1e-10 yield err, %HBM roofline; gra
 HBM_PEAK_GBS = 8000.0
 
 
+def _host_cpus() -> int:
+    """CPUs this process may run on (the affinity mask; os.cpu_count() counts the machine)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def _cpu_threads() -> int:
+    """OMP_NUM_THREADS when it is set (the host's allotment of threads for this
+    process), else every CPU the process may run on."""
     env = os.environ.get("OMP_NUM_THREADS")
-    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
-    return max(1, min(n, 16))
+    n = int(env) if env and env.isdigit() else _host_cpus()
+    return max(1, n)
 
 
 def _cpu_model() -> str:
@@ -74,8 +84,11 @@ def cpu_baseline(w, target_s: float):
     steps = max(1, int(target_s / max(t1, 1e-3)))
     k, t = run(steps) if steps > 1 else (k1, t1)
     return {"value": w.N * nt * k / t, "unit": "state·steps/s", "cores": nt, "kind": "port",
-            "sample": f"CPU oracle (C, OpenMP, one trace vector per thread): N={w.N}, {nt} vectors x {k} steps, "
-                      f"m={w.sim.N_krylov}, MGS Arnoldi + Pade expm + convert_to_krylov, {t:.1f} s on {_cpu_model()}"}
+            "host_cpus": _host_cpus(), "machine_cpus": os.cpu_count(), "cpu_model": _cpu_model(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "sample": f"CPU oracle (C, OpenMP, one trace vector per thread, {nt} threads): N={w.N}, {nt} vectors x "
+                      f"{k} steps, m={w.sim.N_krylov}, MGS Arnoldi + Pade expm + convert_to_krylov, {t:.1f} s on "
+                      f"{_cpu_model()}"}
 
 
 def side_workloads(ctx, a):
@@ -87,16 +100,23 @@ def side_workloads(ctx, a):
     import copy
     from spin_dynamics_amd import batch, dynamics, gradients, sc_dynamics, workloads
     out = {}
-    # adjoint gradient of the yield vs the forward run (step-uniform cost at tol = 0)
-    w4 = workloads.anisotropic_36864(M=64, m=24, T_ns=float(a.extra_steps) - 0.5)
-    dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)  # warm
+    # adjoint gradient of the yield vs the forward run: the differentiation workload,
+    # T = 5 us (5001 steps), checkpoints every 64 steps, earlier segments recomputed
+    w4 = workloads.anisotropic_36864(M=64, m=24, T_ns=a.grad_T_ns)
+    warm = workloads.anisotropic_36864(M=64, m=24, T_ns=63.5)
+    dynamics.trace_sampling_para(warm.sys, warm.sim, B=warm.B, ctx=ctx)  # first launches
+    gradients.yield_gradient(warm.sys, warm.sim, B=warm.B, ctx=ctx)
     dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)
     fms, _, steps = dynamics.last_run_stats(ctx)
+    t0 = time.perf_counter()
     g = gradients.yield_gradient(w4.sys, w4.sim, B=w4.B, ctx=ctx)
+    g_wall = time.perf_counter() - t0
     out["grad_fwd"] = {"ratio": round(g.device_ms / fms, 3), "forward_ms": round(fms, 1),
-                       "gradient_ms": round(g.device_ms, 1), "steps": steps,
-                       "sample": f"N=36864 (2x14N+6x1H | 4x1H, anisotropic), M=64, m=24, dt=1 ns, {steps} steps, "
-                                 f"gradient w.r.t. 78 parameters; T=5 us (5001 steps) costs ~{5001 / steps:.0f}x"}
+                       "gradient_ms": round(g.device_ms, 1), "gradient_wall_ms": round(1000 * g_wall, 1),
+                       "steps": steps,
+                       "sample": f"N=36864 (2x14N+6x1H | 4x1H, anisotropic), M=64, m=24, dt=1 ns, T={a.grad_T_ns} ns "
+                                 f"({steps} steps), gradient of Phi_S(T) w.r.t. 78 parameters, checkpoints every 64 "
+                                 f"steps (segments beyond the resident ones recomputed)"}
     # batched orientations: one pair per dimension class, 96 Fibonacci orientations
     steps_b = 8
     dirs = batch.fibonacci_sphere(96)
@@ -176,7 +196,8 @@ def main(argv=None):
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the gradient, batched-orientation and semiclassical measurements")
-    ap.add_argument("--extra-steps", type=int, default=64, help="steps of the gradient/forward comparison")
+    ap.add_argument("--grad-T-ns", type=float, default=5000.0,
+                    help="simulated time of the gradient/forward comparison (5000 ns: 5001 steps)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",

@@ -89,9 +89,18 @@ const char *spc_version(void);
  * covers anisotropic tensors, tilted fields and full D, which the reference
  * stops on (qm_dynamics.f90:300). */
 int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys);
-/* Dimensions of the current Hamiltonian. */
+/* Dimensions of the current Hamiltonian; nnz = the entries
+ * spc_hamiltonian_copy writes. */
 int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz);
-/* Copies the device CSR (0-based rowptr[N+1], col[nnz], val[2*nnz]). */
+/* Copies H as CSR in the reference basis order (0-based rowptr[N+1],
+ * col[nnz], val[2*nnz]), the output of getHiso_sp + coo_to_csr
+ * (hamiltonian.f90:17-205, utils.f90:52-89).  When H has getHiso_sp's form
+ * (isotropic tensors, field along z, axial coupling) the rows hold that
+ * routine's structure, zeros it stores included (nnz = N + N/2 +
+ * sum_k (g_k-1) N/g_k, hamiltonian.f90:62-73), in coo_to_csr's order: reverse
+ * COO order, i.e. the last nucleus's S-I+ first and the diagonal last.  Other
+ * Hamiltonians (which the reference does not build) give their stored
+ * entries, ordered by the same term classes and by column within a class. */
 int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *val);
 
 /* ---------------------------------------------------------------- dynamics */

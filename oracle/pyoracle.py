@@ -162,6 +162,14 @@ class Csr:
     def scipy(self):
         return _csr_to_scipy(self.h)
 
+    def arrays(self):
+        """(rowptr, col, val) as stored, 0-based, val complex."""
+        n, nnz = self.h.n, self.h.nnz
+        rowptr = np.ctypeslib.as_array(self.h.rowptr, shape=(n + 1,)).copy()
+        col = np.ctypeslib.as_array(self.h.col, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(self.h.val, shape=(max(2 * nnz, 2),))[:2 * nnz].copy().view(np.complex128)
+        return rowptr, col, val
+
     @property
     def n(self):
         return self.h.n

@@ -80,6 +80,8 @@ struct spc_ctx {
     DevSystem hsys{};
     double kS = 0.0, kT = 0.0;
     bool iso_applied = false; // the isotropy rule replaced the tensors by trace/3
+    bool ref_form = false;    // H has getHiso_sp's form: isotropic tensors, field along z, axial T
+    int64_t nnz_export = 0;   // entries spc_hamiltonian_copy writes
     HamDev ham;               // H
     HamDev adj;               // H' = -H^H (Hermitian part negated), for the adjoint
     bool adj_valid = false;
@@ -445,6 +447,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)n_out * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
+    if (w.sstep >= 2 && (rc = ensure(ctx, nm("cond").c_str(), (size_t)1, &w.cond))) return rc;
     if (vkind == 0 || vkind == 1)
         if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &L.d_seeds))) return rc;
     if (vkind == 2)
@@ -485,6 +488,7 @@ int start_pass(spc_ctx *ctx, LaneSet &L, const double pop[4])
     HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, L.s));
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, L.s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, L.s));
+    if (w.cond) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), L.s));
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * w.n_out * 4 * Mp, L.s));
     std::fill(L.obs0.begin(), L.obs0.end(), 0.0);
     for (int x = 0; x < 4; ++x)
@@ -502,6 +506,31 @@ int rebuild(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, bool gram)
     if (gram) HIP_TRY(gram_projectors(w, L.s));
     HIP_TRY(step_lanes(w, sim->dt, sim->t_end, sim->tol, gram, L.s));
     HIP_TRY(psi_update(w, !gram, L.s));
+    return SPC_OK;
+}
+
+// The smallest pivot of a block's unit-diagonal Cholesky factor below which a
+// chunk runs column by column instead.  The block form's monomial chain
+// [q, Hq, .., H^S q] is ill-conditioned where the spectrum clusters far from
+// zero (fields of tens of mT); the column form keeps Q orthonormal to about
+// 1e-13 whatever the spectrum.  In a model of this schedule (S = 8, m = 24,
+// anisotropic systems) the pivot was 1.2e-2 with ||Q^H Q - I|| = 2e-10 at
+// 0.05-2 mT (N = 384 and 36 864 alike) and 8e-4 with 1.6e-6 at 25 mT; the
+// loss grows like p^-3.2, so the limit keeps it below about 5e-9.
+constexpr double kBlockPivotLimit = 5e-3;
+
+// Whether L's block Arnoldi, since the chunk started, built a block with a
+// pivot below kBlockPivotLimit.  Waits for L's stream.
+int block_poor(spc_ctx *ctx, LaneSet &L, bool &poor)
+{
+    poor = false;
+    if (!L.w.cond || L.w.sstep < 2) return SPC_OK;
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, L.w.cond, sizeof(bits), hipMemcpyDeviceToHost, L.s));
+    HIP_TRY(hipStreamSynchronize(L.s));
+    double inv_p = 0.0;
+    std::memcpy(&inv_p, &bits, sizeof(inv_p));
+    poor = inv_p * kBlockPivotLimit > 1.0;
     return SPC_OK;
 }
 
@@ -524,6 +553,16 @@ int gram_pass(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, const double 
     for (int64_t round = 0;; ++round) {
         if ((rc = rebuild(ctx, L, sim, true))) return rc;
         ++rebuilds;
+        if (round == 0) { // a poorly conditioned block basis: the chunk again, column by column
+            bool poor = false;
+            if ((rc = block_poor(ctx, L, poor))) return rc;
+            if (poor) {
+                w.sstep = 0;
+                if ((rc = start_pass(ctx, L, pop))) return rc;
+                round = -1;
+                continue;
+            }
+        }
         HIP_TRY(hipMemcpyAsync(L.lt.data(), L.w.lane_t, sizeof(double) * L.w.Mp, hipMemcpyDeviceToHost, L.s));
         HIP_TRY(hipStreamSynchronize(L.s));
         bool done = true;
@@ -658,20 +697,52 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
                 if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
                 int32_t flags[4];
                 if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                if (flags[0] && sets[j].w.sstep >= 2) { // the block form broke down: column by column
+                    sets[j].w.sstep = 0;
+                    if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
+                    if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                }
                 breakdown = breakdown || flags[0];
             }
         } else {
             // direct mode: one step per rebuild, the chunks' rebuilds interleaved
+            auto direct_pass = [&](int j) -> int {
+                int r2;
+                if ((r2 = start_pass(ctx, sets[j], pop))) return r2;
+                for (int64_t round = 0; round < K; ++round) {
+                    if ((r2 = rebuild(ctx, sets[j], sim, false))) return r2;
+                    ++rebuilds;
+                }
+                return SPC_OK;
+            };
             for (int j = 0; j < used; ++j)
                 if ((rc = start_pass(ctx, sets[j], pop))) return rc;
-            for (int64_t round = 0; round < K; ++round)
+            for (int64_t round = 0; round < K; ++round) {
                 for (int j = 0; j < used; ++j) {
                     if ((rc = rebuild(ctx, sets[j], sim, false))) return rc;
                     ++rebuilds;
                 }
+                if (round > 0) continue;
+                // after the first rebuild: a chunk whose block basis is poorly
+                // conditioned starts again column by column
+                for (int j = 0; j < used; ++j) {
+                    bool poor = false;
+                    if ((rc = block_poor(ctx, sets[j], poor))) return rc;
+                    if (!poor) continue;
+                    sets[j].w.sstep = 0;
+                    if ((rc = start_pass(ctx, sets[j], pop))) return rc;
+                    if ((rc = rebuild(ctx, sets[j], sim, false))) return rc;
+                    ++rebuilds;
+                }
+            }
             for (int j = 0; j < used; ++j) {
                 int32_t flags[4];
                 if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                if (flags[0] && sets[j].w.sstep >= 2) { // the block form broke down: column by column
+                    sets[j].w.sstep = 0;
+                    if ((rc = direct_pass(j))) return rc;
+                    if ((rc = read_flags(ctx, sets[j], flags))) return rc;
+                }
                 breakdown = breakdown || flags[0];
                 if (!flags[1]) continue;
                 // a lane needed two steps per rebuild: redo the chunk with projector Grams
@@ -909,6 +980,7 @@ struct BatchPlan {
     bool graph = true; // replay one captured step
     int slots = 1;     // chunks in flight at once, each on its own stream
     bool kron = true;  // per-orientation Kronecker tables (else the block product)
+    bool columns = false; // one column per pass (after the block form broke down or lost orthogonality)
 };
 
 // One stream of the batched run: its workspace and the step graph of its
@@ -977,7 +1049,7 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("psi").c_str(), (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
-    w.sstep = sstep_block(m);
+    w.sstep = plan.columns ? 0 : sstep_block(m);
     const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
@@ -995,6 +1067,10 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
+    if (w.sstep >= 2) { // the pivot monitor, over every chunk of the slot
+        if ((rc = ensure(ctx, nm("cond").c_str(), (size_t)1, &w.cond))) return rc;
+        if (!sl.n) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), sl.s));
+    }
     if ((rc = ensure(ctx, nm("bvals").c_str(), (size_t)n * nb * 16, &vals))) return rc;
     w.bval = vals;
     w.kron = nullptr;
@@ -1244,6 +1320,18 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         }
         iv.emplace_back(ta, tb);
     }
+    bool poor = false; // a block basis below the pivot limit (see kBlockPivotLimit)
+    for (int j = 0; j < S; ++j) {
+        if (!slots[j].w.cond || slots[j].w.sstep < 2) continue;
+        unsigned long long bits = 0;
+        if (hipMemcpy(&bits, slots[j].w.cond, sizeof(bits), hipMemcpyDeviceToHost) != hipSuccess) {
+            cleanup();
+            return fail(ctx, SPC_ERR_HIP, "pivot monitor copy");
+        }
+        double inv_p = 0.0;
+        std::memcpy(&inv_p, &bits, sizeof(inv_p));
+        poor = poor || inv_p * kBlockPivotLimit > 1.0;
+    }
     cleanup();
     std::vector<int32_t> flags((size_t)4 * nchunks), bad((size_t)nchunks);
     HIP_TRY(hipMemcpy(flags.data(), sh.flags, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost));
@@ -1252,6 +1340,13 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         // an orientation's H did not match its Kronecker tables: the block product
         BatchPlan p2 = plan;
         p2.kron = false;
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms);
+    }
+    bool broke = false;
+    for (int c = 0; c < nchunks; ++c) broke = broke || flags[4 * c] != 0;
+    if ((poor || broke) && !plan.columns) { // the whole batch again, column by column
+        BatchPlan p2 = plan;
+        p2.columns = true;
         return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms);
     }
     for (int c = 0; c < nchunks; ++c) {
@@ -1426,6 +1521,23 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     ctx->ham = h;
     ctx->iso_applied = iso;
     ctx->nnz = h.nnz;
+    // getHiso_sp's form (hamiltonian.f90:17-205): every tensor a 1, the field
+    // along z, the electron coupling T diagonal with T11 = T22.  Its COO
+    // stores the flip-flop and ladder entries even where their values vanish
+    // (the nnz formula of hamiltonian.f90:62-73); the export does the same.
+    {
+        bool rf = S.w[0] == 0.0 && S.w[1] == 0.0 && S.dir[0] == 0.0 && S.dir[1] == 0.0;
+        for (int k = 0; k < nn && rf; ++k)
+            for (int q = 0; q < 9; ++q)
+                if ((q % 4 == 0) ? S.A[k][q] != S.A[k][0] : S.A[k][q] != 0.0) rf = false;
+        for (int q = 0; q < 9 && rf; ++q)
+            if (q % 4 != 0 && (S.Tre[q] != 0.0 || S.Tim[q] != 0.0)) rf = false;
+        if (rf && (S.Tre[0] != S.Tre[4] || S.Tim[0] != S.Tim[4])) rf = false;
+        ctx->ref_form = rf;
+        int64_t ne = N + N / 2;
+        for (int k = 0; k < nn; ++k) ne += (int64_t)(S.mult[k] - 1) * (N / S.mult[k]);
+        ctx->nnz_export = rf ? ne : h.nnz;
+    }
     ctx->hsys = S;
     ctx->kS = sys->kS;
     ctx->kT = sys->kT;
@@ -1440,7 +1552,7 @@ int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz)
     if (!ctx || !ctx->have_sys) return SPC_ERR_INVALID;
     if (N) *N = ctx->N;
     if (Z) *Z = ctx->Z;
-    if (nnz) *nnz = ctx->nnz;
+    if (nnz) *nnz = ctx->nnz_export;
     return SPC_OK;
 }
 
@@ -1455,22 +1567,76 @@ int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *va
     HIP_TRY(hipMemcpy(rp.data(), ctx->ham.rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cl.data(), ctx->ham.col, sizeof(int32_t) * ctx->nnz, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vl.data(), ctx->ham.val, sizeof(double2) * ctx->nnz, hipMemcpyDeviceToHost));
-    std::vector<std::pair<int32_t, double2>> row;
+    // Within a row, getHiso_sp's COO lists the diagonal, S1+S2-, S1-S2+, then
+    // per nucleus (electron 1's first) S+I- and S-I+ (hamiltonian.f90:79-203),
+    // and coo_to_csr places a row's entries in reverse COO order
+    // (utils.f90:77-84): the last nucleus's S-I+ first, the diagonal last.
+    // In getHiso_sp's form the rows are its structure, with the zeros it
+    // stores; otherwise (anisotropic tensors, tilted field: terms the
+    // reference lacks) the stored entries, each keyed by its term class, and
+    // in column order within a class.
+    const DevSystem &hs = ctx->hsys;
+    auto term_key = [&](int64_t r, int64_t c) -> int {
+        if (r == c) return 0;
+        const int64_t er = r / Z, ec = c / Z, nr = r % Z, nc = c % Z;
+        if (nr == nc) return er == 1 && ec == 2 ? 1 : 2; // S1+S2- maps beta-alpha to alpha-beta
+        for (int k = 0; k < hs.nn; ++k) {
+            const int64_t dr = nr / hs.stride[k] % hs.mult[k], dc = nc / hs.stride[k] % hs.mult[k];
+            if (dr == dc) continue;
+            const int sh = k < hs.n1 ? 1 : 0; // spin of nucleus k's electron: bit sh of e = 2 s1 + s2
+            const bool raised = ((er >> sh) & 1) == 0 && ((ec >> sh) & 1) == 1;
+            return 3 + 2 * k + (raised && dr == dc + 1 ? 0 : 1);
+        }
+        return 2;
+    };
+    struct Ent {
+        int32_t col;
+        int key;
+        double2 v;
+    };
+    std::vector<Ent> row;
     int64_t pos = 0;
     for (int64_t n = 0; n < N; ++n) {
         const int64_t d = dev_row(n, Z);
         row.clear();
-        for (int32_t e = rp[d]; e < rp[d + 1]; ++e) {
-            const int32_t c = cl[e];
-            row.push_back({(int32_t)((c & 3) * Z + (c >> 2)), vl[e]});
+        if (ctx->ref_form) {
+            const int64_t e = n / Z, nuc = n % Z;
+            row.push_back({(int32_t)n, 0, double2{0.0, 0.0}});
+            if (e == 1 || e == 2) row.push_back({(int32_t)((3 - e) * Z + nuc), (int)e, double2{0.0, 0.0}});
+            for (int k = 0; k < hs.nn; ++k) {
+                const int sh = k < hs.n1 ? 1 : 0;
+                const int64_t dg = nuc / hs.stride[k] % hs.mult[k];
+                const int64_t ef = e ^ (1 << sh);
+                if (((e >> sh) & 1) == 0 && dg >= 1) // S+ I-: electron up from beta, nucleus down
+                    row.push_back({(int32_t)(ef * Z + nuc - hs.stride[k]), 3 + 2 * k, double2{0.0, 0.0}});
+                if (((e >> sh) & 1) == 1 && dg + 1 < hs.mult[k]) // S- I+
+                    row.push_back({(int32_t)(ef * Z + nuc + hs.stride[k]), 4 + 2 * k, double2{0.0, 0.0}});
+            }
+            for (int32_t q = rp[d]; q < rp[d + 1]; ++q) {
+                const int32_t c = cl[q];
+                const int32_t cr = (int32_t)((c & 3) * Z + (c >> 2));
+                auto it = std::find_if(row.begin(), row.end(), [&](const Ent &x) { return x.col == cr; });
+                if (it == row.end())
+                    return fail(ctx, SPC_ERR_INVALID, "Hamiltonian entry outside the getHiso_sp structure");
+                it->v = vl[q];
+            }
+        } else {
+            for (int32_t q = rp[d]; q < rp[d + 1]; ++q) {
+                const int32_t c = cl[q];
+                const int32_t cr = (int32_t)((c & 3) * Z + (c >> 2));
+                row.push_back({cr, term_key(n, cr), vl[q]});
+            }
         }
-        std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::sort(row.begin(), row.end(),
+                  [](const Ent &a, const Ent &b) { return a.key != b.key ? a.key > b.key : a.col < b.col; });
+        if (pos + (int64_t)row.size() > ctx->nnz_export)
+            return fail(ctx, SPC_ERR_INVALID, "Hamiltonian export exceeds its entry count");
         if (rowptr) rowptr[n] = (int32_t)pos;
         for (const auto &x : row) {
-            if (col) col[pos] = x.first;
+            if (col) col[pos] = x.col;
             if (val) {
-                val[2 * pos] = x.second.x;
-                val[2 * pos + 1] = x.second.y;
+                val[2 * pos] = x.v.x;
+                val[2 * pos + 1] = x.v.y;
             }
             ++pos;
         }
@@ -1884,10 +2050,14 @@ int spc_semiclassical_tangent(spc_ctx *ctx, const spc_sc_system *sys, int64_t n_
                      dPhi_T);
 }
 
-int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
-                          const double *trace_vectors, int32_t checkpoint_every, double *P_S, int64_t n_out,
-                          double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
-                          double *grad_kS, double *grad_kT)
+// Internal return code of trace_sample_grad: a block Krylov basis fell below
+// the pivot limit (kBlockPivotLimit); the caller runs again column by column.
+constexpr int kRetryColumns = -2;
+
+static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
+                             const double *trace_vectors, int32_t checkpoint_every, double *P_S, int64_t n_out,
+                             double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
+                             double *grad_kS, double *grad_kT, bool columns)
 {
     if (!ctx) return SPC_ERR_INVALID;
     if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
@@ -2005,10 +2175,14 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if ((rc = ensure(ctx, "psi", (size_t)N * Mp, &w.psi))) return rc;
     w.u = w.psi;
     if ((rc = ensure(ctx, "v", (size_t)N * Mp, &w.v))) return rc;
-    w.sstep = sstep_block(m);
+    w.sstep = columns ? 0 : sstep_block(m);
     const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
     if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
     w.part_capacity = part_need;
+    if (w.sstep >= 2) { // the pivot monitor, shared with the adjoint sweep's basis
+        if ((rc = ensure(ctx, "cond", (size_t)1, &w.cond))) return rc;
+        HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), ctx->stream));
+    }
     if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
     if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
     if ((rc = ensure(ctx, "coef", (size_t)(2 * SPC_MAX_KRYLOV + 4) * Mp, &w.coef))) return rc;
@@ -2145,6 +2319,7 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     std::vector<double> acc((size_t)4 * nout);
     HIP_TRY(hipMemcpyAsync(acc.data(), d_acc, sizeof(double) * 4 * nout, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (flags[0] && w.sstep >= 2) return kRetryColumns;
     if (flags[0])
         return fail(ctx, SPC_ERR_BREAKDOWN,
                     "Krylov breakdown: the Krylov space of a trace vector closed before N_krylov vectors");
@@ -2221,6 +2396,13 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     ctx->last_ms = ms;
     ctx->last_rebuilds = K + ntop + kstart;
     ctx->last_steps = K;
+    if (w.cond && w.sstep >= 2) {
+        unsigned long long bits = 0;
+        HIP_TRY(hipMemcpy(&bits, w.cond, sizeof(bits), hipMemcpyDeviceToHost));
+        double inv_p = 0.0;
+        std::memcpy(&inv_p, &bits, sizeof(inv_p));
+        if (inv_p * kBlockPivotLimit > 1.0 || flags[0] || flags_b[0]) return kRetryColumns;
+    }
     if (flags[0] || flags_b[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
 
     double gf[3], gT[9], gkS, gkT;
@@ -2252,6 +2434,19 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
         for (int q = 0; q < 9; ++q)
             if (!std::isfinite(grad_hfc[9 * k + q])) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite gradient");
     return SPC_OK;
+}
+
+int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds,
+                          const double *trace_vectors, int32_t checkpoint_every, double *P_S, int64_t n_out,
+                          double *Phi_S, double *grad_field, double *grad_hfc, double *grad_J, double *grad_D,
+                          double *grad_kS, double *grad_kT)
+{
+    int rc = trace_sample_grad(ctx, sim, sample_seeds, trace_vectors, checkpoint_every, P_S, n_out, Phi_S, grad_field,
+                               grad_hfc, grad_J, grad_D, grad_kS, grad_kT, false);
+    if (rc == kRetryColumns) // a block basis broke down or lost orthogonality: column by column
+        rc = trace_sample_grad(ctx, sim, sample_seeds, trace_vectors, checkpoint_every, P_S, n_out, Phi_S,
+                               grad_field, grad_hfc, grad_J, grad_D, grad_kS, grad_kT, true);
+    return rc;
 }
 
 int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,

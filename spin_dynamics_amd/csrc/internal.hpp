@@ -213,6 +213,11 @@ struct KrylovWork {
     int sstep = 0;            // block size of the s-step Arnoldi (0 or 1: one column per pass)
     bool no_dots = false;     // products may skip their u^H v, u^H u partials (tiled product only)
     double2 *sbuf = nullptr;  // its dots and forming coefficients, kSStepBuf x Mp
+    // largest 1/p over the lanes and blocks since the last reset, p the
+    // smallest pivot of a block's unit-diagonal Cholesky factor (bits of a
+    // positive double; 0 = none yet): the block basis loses orthogonality as
+    // p falls
+    unsigned long long *cond = nullptr;
 };
 // Elements per lane of KrylovWork::sbuf.
 constexpr int kSStepBuf = 720;

@@ -1640,7 +1640,7 @@ __global__ __launch_bounds__(kBlock) void k_sform(ColPtrs x, int nx, OutPtrs o, 
 __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m, int M, int Mp,
                                                   const double2 *__restrict__ dots, double2 *__restrict__ coef,
                                                   double2 *__restrict__ hess, double *__restrict__ beta0,
-                                                  int32_t *__restrict__ flags)
+                                                  int32_t *__restrict__ flags, unsigned long long *__restrict__ cond)
 {
     constexpr int NL = kSCoefLanes, TP = kBlock / NL;
     static_assert(TP >= kMaxKrylov, "one thread per Hessenberg row");
@@ -1689,6 +1689,7 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
     for (int idx = q; idx < nf * nf; idx += TP)
         R[idx] = cscale(1.0 / (sd[l][idx / nf] * sd[l][idx % nf]), R[idx]);
     __syncthreads();
+    double pmin = 1.0; // smallest pivot of the unit-diagonal factor (thread q = 0)
     for (int i = 0; i < nf; ++i) {
         if (q == 0) {
             double x = R[i * nf + i].x;
@@ -1697,6 +1698,7 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
                 x = 1.0;
             }
             const double rii = sqrt(x);
+            pmin = fmin(pmin, rii);
             R[i * nf + i] = cz(rii, 0.0);
             sinv[l] = 1.0 / rii;
         }
@@ -1732,6 +1734,7 @@ __global__ __launch_bounds__(kBlock) void k_scoef(int K, int n, int extra, int m
         W[kSMaxNY + j] = pj;
     }
     if (q == 0 && active && !ok) flags[0] = 1; // Krylov breakdown (the block's vectors are dependent)
+    if (q == 0 && active && ok && cond) atomicMax(cond, (unsigned long long)__double_as_longlong(1.0 / pmin));
     __syncthreads();
     // forming coefficients: x < K: -(C R^-1)[x][j]; x = K + t: (R^-1)[t][j]
     for (int idx = q; idx < (K + nf) * nf; idx += TP) {
@@ -1888,7 +1891,7 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
         {
             ProfScope ps(w, PF_COEFFS, 0.0, s);
             hipLaunchKernelGGL(k_scoef, dim3((unsigned)ceil_div(w.Mp, kSCoefLanes)), dim3(kBlock), 0, s, K, nd, extra,
-                               m, w.M, w.Mp, sdots, scoef, w.hess, w.beta0, w.flags);
+                               m, w.M, w.Mp, sdots, scoef, w.hess, w.beta0, w.flags, w.cond);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         {

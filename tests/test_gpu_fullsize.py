@@ -35,6 +35,26 @@ def test_first_steps_match_the_oracle(ctx):
         assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
 
 
+@pytest.mark.parametrize("S", ["default", "0"])
+def test_first_16_vectors_over_500_steps_match_oracle_fixture(ctx, monkeypatch, S):
+    # the oracle's populations and Simpson yield for the first 16 random-phase
+    # vectors of the master seed over T = 500 ns (501 steps), written by
+    # tests/golden/make_oracle_fixtures.py; the default block Arnoldi (blocks
+    # of 8) and the column form both stay within the yield tolerance
+    import os
+    if S != "default":
+        monkeypatch.setenv("SPC_SSTEP", S)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_aniso36864_m16_500ns.npz"))
+    w = workloads.anisotropic_36864(M=16, m=24, T_ns=float(d["T_ns"]))
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    _, _, steps = dynamics.last_run_stats(ctx)
+    assert steps == 501 and len(res.P_S) == len(d["P_S"])
+    for k in KEYS:
+        assert max_rel(getattr(res, k), d[k]) < 1e-10, k
+    res.get_kinetics(w.sim.dt, w.sys.kS, w.sys.kT)
+    assert abs(res.Phi_S[-1] - d["Phi_S"][-1]) <= 1e-10 * abs(d["Phi_S"][-1])
+
+
 def test_equal_rates_decay_exactly(ctx):
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=39.5)  # 40 steps
     k = 1.5 / GAMMA_E

@@ -239,3 +239,25 @@ def test_gradient_on_the_full_size_system_matches_fd(ctx):
         adj.append(_adj_value(g, name, idx))
     adj, fd = np.array(adj), np.array(fd)
     assert np.max(np.abs(adj - fd)) <= 1e-6 * np.max(np.abs(fd)), (adj, fd)
+
+
+def test_gradient_workload_shape_with_recomputed_segments(ctx, monkeypatch):
+    # the differentiation workload's system and width (N = 36864, M = 64,
+    # checkpoints every 64 steps) over 192 steps, with only the last 64 steps
+    # kept resident: the two earlier segments are recomputed from checkpoints
+    w = workloads.anisotropic_36864(M=64, m=24, T_ns=191.5)
+    monkeypatch.setenv("SPC_ADJOINT_KEEP", "64")
+    g = gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx, checkpoint_every=64)
+    monkeypatch.delenv("SPC_ADJOINT_KEEP")
+    picks = [("hfc", (0, 0, 0)), ("hfc", (1, 0, 2)), ("hfc", (7, 2, 1)), ("hfc", (11, 1, 1)), ("field", 0),
+             ("field", 2), ("J", None), ("kS", None), ("kT", None)]
+    adj, fd = [], []
+    for name, idx in picks:
+        h = 1e-6 * getattr(w.sys, name) if name in ("kS", "kT") else 1e-6
+        sp, Bp = _perturbed(w.sys, w.B, name, idx, h)
+        sm, Bm = _perturbed(w.sys, w.B, name, idx, -h)
+        fd.append((_forward_yield(sp, w.sim, Bp, ctx) - _forward_yield(sm, w.sim, Bm, ctx)) / (2 * h))
+        adj.append(_adj_value(g, name, idx))
+    adj, fd = np.array(adj), np.array(fd)
+    assert abs(g.Phi_S - _forward_yield(w.sys, w.sim, w.B, ctx)) <= 1e-12 * abs(g.Phi_S)
+    assert np.max(np.abs(adj - fd)) <= 1e-6 * np.max(np.abs(fd)), (adj, fd)

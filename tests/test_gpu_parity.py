@@ -21,18 +21,44 @@ def test_hamiltonian_isotropic_matches_reference_builder(ctx):
     w = workloads.isotropic_192()
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
     ref = pyoracle.build_iso_csr(oracle_system(w.sys, w.B))
-    assert H.N == 192 and H.nnz <= ref.nnz
+    assert H.N == 192 and H.nnz == ref.nnz == 800
     assert max_rel(H.scipy().toarray(), ref.scipy().toarray()) < 1e-14
+
+
+@pytest.mark.parametrize("case", ["iso192", "mixed", "exchange"])
+def test_hamiltonian_export_is_coo_to_csr_order(ctx, case):
+    """getHiso_sp + coo_to_csr (hamiltonian.f90:17-205, utils.f90:52-89): the
+    exported rowptr and col equal the oracle's bit for bit (reverse COO order
+    within a row, the zeros the reference stores included)."""
+    import copy
+    from spin_dynamics_amd.spinsys import Electron
+    w = workloads.isotropic_192()  # kS = kT, J = 0: its flip-flop entries are stored zeros
+    sys = copy.deepcopy(w.sys)
+    if case != "iso192":
+        m1, m2 = ((3, 2, 2), (2, 3)) if case == "mixed" else ((2, 2), (3,))
+        a = np.random.default_rng(3).uniform(-1, 1, len(m1) + len(m2))
+        sys.e1 = Electron(g_I=list(m1), A=np.array([x * np.eye(3) for x in a[:len(m1)]]))
+        sys.e2 = Electron(g_I=list(m2), A=np.array([x * np.eye(3) for x in a[len(m1):]]))
+        if case == "exchange":
+            sys.J, sys.kS, sys.kT = 0.2, 2.0 / 176.08596, 0.5 / 176.08596
+    H = dynamics.Hamiltonian(sys, w.B, ctx)
+    rowptr, col, val = H.csr()
+    rr, rc, rv = pyoracle.build_iso_csr(oracle_system(sys, w.B)).arrays()
+    assert H.nnz == len(rc)
+    assert np.array_equal(rowptr, rr) and np.array_equal(col, rc)
+    assert np.max(np.abs(val - rv)) <= 1e-14 * np.max(np.abs(rv))
+    if case == "iso192":
+        assert np.count_nonzero(rv == 0) >= H.N // 2
 
 
 def test_hamiltonian_anisotropic_tilted_matches_oracle_and_dense(ctx):
     w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 3))
     s = oracle_system(w.sys, w.B)
     H = dynamics.Hamiltonian(w.sys, w.B, ctx)
-    rowptr, col, val = H.csr()
     g = pyoracle.build_general_csr(s).scipy()
-    assert np.array_equal(rowptr, g.indptr) and np.array_equal(col, g.indices)
-    assert max_rel(val, g.data) < 1e-14
+    h = H.scipy().sorted_indices()  # the export orders a row by term, the general builder by column
+    assert np.array_equal(h.indptr, g.indptr) and np.array_equal(h.indices, g.indices)
+    assert max_rel(h.data, g.data) < 1e-14
     Hd = dm.hamiltonian(s.mult1, s.mult2, s.tensors(), B=s.B, direction=s.direction, J=s.J, D=s.D, kS=s.kS, kT=s.kT)
     assert max_rel(H.scipy().toarray(), Hd) < 1e-14
 
@@ -91,6 +117,25 @@ def test_trace_sampling_injected_vs_oracle(ctx, M, tol):
         assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
     _, phs, _ = pyoracle.get_kinetics(ref["P_S"], ref["P_T0"], ref["P_Tp"], ref["P_Tm"], w.sim.dt, s.kS, s.kT)
     assert abs(res.Phi_S[-1] - phs[-1]) < 1e-10 * abs(phs[-1])
+
+
+@pytest.mark.parametrize("state", ["T0", "Tp", "Tm"])
+@pytest.mark.parametrize("tol", [0.0, 1e-8])
+def test_triplet_initial_states_vs_oracle(ctx, state, tol):
+    """init_el_state's triplet branches (qm_dynamics.f90:395-406) on the device."""
+    import copy
+    w = workloads.small_anisotropic(M=8, tol=tol)
+    sim = copy.deepcopy(w.sim)
+    sim.init_state = state
+    s = oracle_system(w.sys, w.B)
+    vecs = np.array([pyoracle.sample_vector(sd, s.Z, 1) for sd in pyoracle.sample_seeds((7, 13), 8)])
+    res = dynamics.trace_sampling_para(w.sys, sim, B=w.B, trace_vectors=vecs, ctx=ctx)
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=8, m=sim.N_krylov, t_end=sim.t_end,
+                                  dt=sim.dt, tol=tol, vec_kind=2, injected=vecs, init_state=_lib.STATE[state])
+    bound = 1e-10 if tol == 0.0 else 1e-9
+    for k in ("P_S", "P_T0", "P_Tp", "P_Tm"):
+        assert max_rel(getattr(res, k), ref[k]) < bound, k
+    assert getattr(res, {"T0": "P_T0", "Tp": "P_Tp", "Tm": "P_Tm"}[state])[0] == 1.0
 
 
 def test_trace_sampling_tol_gram_mode(ctx):

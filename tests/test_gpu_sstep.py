@@ -126,3 +126,61 @@ def test_sstep_with_substep_outputs(ctx, monkeypatch, S):
     blk = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     for k in KEYS:
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-10, k
+
+
+def _high_field_case(B, T_ns=300.0, M=8):
+    # N = 384 (3x2x2 | 2x2x2), anisotropic tensors, tilted field, m = 24:
+    # at B = 25 mT the spectrum clusters at -B, 0 and +B, where the monomial
+    # chain of the block form is ill-conditioned (numpy model of this
+    # schedule: smallest Cholesky pivot 8e-4, ||Q^H Q - I|| = 1.6e-6 at
+    # S = 8, against 1.2e-2 and 2e-10 at 0.05 mT)
+    w = workloads.small_anisotropic(m1=(3, 2, 2), m2=(2, 2, 2), M=M, m=24, T_ns=T_ns)
+    w.B = B
+    w.sys.e1.w = w.sys.e2.w = -B
+    s = oracle_system(w.sys, w.B)
+    vecs = np.array([pyoracle.sample_vector(sd, s.Z, 1) for sd in pyoracle.sample_seeds((4, 8), M)])
+    return w, s, vecs
+
+
+@pytest.mark.parametrize("B", [0.05, 25.0])
+def test_high_field_runs_column_by_column(ctx, monkeypatch, B):
+    w, s, vecs = _high_field_case(B)
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
+    monkeypatch.setenv("SPC_SSTEP", "0")
+    col = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
+    monkeypatch.delenv("SPC_SSTEP")
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=len(vecs), m=w.sim.N_krylov,
+                                  t_end=w.sim.t_end, dt=w.sim.dt, vec_kind=2, injected=vecs)
+    same = all(np.array_equal(getattr(res, k), getattr(col, k)) for k in KEYS)
+    # 25 mT: the first rebuild's pivot is below the limit, so the chunk starts
+    # again column by column and gives the column form's results bit for bit;
+    # 0.05 mT keeps the block form (within rounding of the column form)
+    assert same == (B > 1.0)
+    for k in KEYS:
+        assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
+        assert max_rel(getattr(res, k), getattr(col, k)) < 1e-11, k
+
+
+def test_high_field_batch_and_gradient_run_column_by_column(ctx, monkeypatch):
+    import copy as cp
+    w, s, _ = _high_field_case(25.0, T_ns=40.0, M=64)
+    dirs = batch.fibonacci_sphere(3)
+    phi = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, ctx=ctx)
+    g = gradients_of(w, ctx)
+    monkeypatch.setenv("SPC_SSTEP", "0")
+    phi_c = batch.orientation_yields(w.sys, w.sim, dirs, B=w.B, ctx=ctx)
+    g_c = gradients_of(w, ctx)
+    monkeypatch.delenv("SPC_SSTEP")
+    assert np.array_equal(phi, phi_c)
+    assert np.array_equal(g.hfc, g_c.hfc) and np.array_equal(g.field, g_c.field) and g.Phi_S == g_c.Phi_S
+    for o, d in enumerate(dirs):
+        sd = cp.deepcopy(w.sys)
+        sd.field_dir = tuple(d)
+        r = dynamics.trace_sampling_para(sd, w.sim, B=w.B, ctx=ctx)
+        r.get_kinetics(w.sim.dt, sd.kS, sd.kT)
+        assert abs(phi[o] - r.Phi_S[-1]) <= 1e-12 * abs(r.Phi_S[-1])
+
+
+def gradients_of(w, ctx):
+    from spin_dynamics_amd import gradients
+    return gradients.yield_gradient(w.sys, w.sim, B=w.B, ctx=ctx)

@@ -42,6 +42,24 @@ def test_trace_sampling_matches_dense_propagation():
     assert res["P_S"][0] == 1.0 and res["P_T0"][0] == 0.0
 
 
+@pytest.mark.parametrize("state,code", [("singlet", 0), ("T0", 1), ("Tp", 2), ("Tm", 3)])
+def test_every_initial_state_matches_dense_propagation(state, code):
+    """init_el_state (qm_dynamics.f90:386-414): theta for S, T0, T+, T- and the
+    one-hot initial populations, against dense propagation from the textbook
+    states (alpha beta -+ beta alpha)/sqrt2, alpha alpha, beta beta."""
+    s = _aniso()
+    h = pyoracle.build_general_csr(s)
+    M, T = 3, 60 * DT
+    vecs = np.array([pyoracle.sample_vector(sd, s.Z, 1) for sd in pyoracle.sample_seeds((9, 2), M)])
+    res = pyoracle.trace_sampling(h, s.Z, M=M, m=14, t_end=T, dt=DT, vec_kind=2, injected=vecs, init_state=code)
+    K = pyoracle.steps_taken(T, DT)
+    Hd = dm.hamiltonian(s.mult1, s.mult2, s.tensors(), B=s.B, direction=s.direction, J=s.J, kS=s.kS, kT=s.kT)
+    ref = sum(dm.propagate_dense(Hd, np.kron(dm.theta(state), v), DT, K) for v in vecs) / M
+    for x, key in enumerate(["P_S", "P_T0", "P_Tp", "P_Tm"]):
+        assert res[key][0] == (1.0 if x == code else 0.0)
+        assert np.max(np.abs(res[key][:K + 1] - ref[x])) < 1e-13
+
+
 def test_equal_rates_give_exact_exponential_decay():
     k = 3 / GE
     s = _aniso(kS=k, kT=k)
