@@ -139,6 +139,15 @@ int ensure(spc_ctx *ctx, const char *name, size_t count, T **out)
     return SPC_OK;
 }
 
+// Frees a named buffer (after a call that sized it by the free memory).
+void release_buf(spc_ctx *ctx, const char *name)
+{
+    auto it = ctx->bufs.find(name);
+    if (it == ctx->bufs.end()) return;
+    if (it->second.p) (void)hipFree(it->second.p);
+    ctx->bufs.erase(it);
+}
+
 // ------------------------------------------------------------ host RNG
 // Master stream for per-sample seeds (main.f90:50, qm_dynamics.f90:315-320).
 struct HostRng {
@@ -2446,6 +2455,13 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     if (rc == kRetryColumns) // a block basis broke down or lost orthogonality: column by column
         rc = trace_sample_grad(ctx, sim, sample_seeds, trace_vectors, checkpoint_every, P_S, n_out, Phi_S,
                                grad_field, grad_hfc, grad_J, grad_D, grad_kS, grad_kT, true);
+    // the stored step vectors and checkpoints take most of the free memory:
+    // returned to the device for the next calls
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        release_buf(ctx, "adj_store");
+        release_buf(ctx, "adj_ckpt");
+    }
     return rc;
 }
 
