@@ -117,6 +117,25 @@ int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed
                      const double *trace_vectors, double *P_S, double *P_T0, double *P_Tp,
                      double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T);
 
+/* Size of the scratch area of spc_trace_sample_noalloc, per GPU. */
+#define SPC_SCRATCH_BYTES (64u << 20)
+
+/* spc_trace_sample without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops that must not call hipMalloc, code around graph capture).  Same
+ * arguments, results and errors as spc_trace_sample; every device buffer of
+ * the call (Krylov basis, vectors, partial sums, populations, staging) is
+ * carved from a scratch area of SPC_SCRATCH_BYTES that the library holds on
+ * each GPU for the life of the process, and a workspace beyond it returns
+ * SPC_ERR_UNSUPPORTED (about 20 MiB for N = 192, M = 32, m = 20 over 5001
+ * steps).  The lanes run as one chunk on the context stream: the call creates
+ * no streams or events and allocates nothing on the device; the event
+ * profiler must be off.  Every call on a GPU shares that GPU's scratch area:
+ * calls that overlap on one GPU (two contexts or threads) race. */
+int spc_trace_sample_noalloc(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2],
+                             const double *trace_vectors, double *P_S, double *P_T0, double *P_Tp,
+                             double *P_Tm, int64_t n_out, double *Phi_S, double *Phi_T);
+
 /* As spc_trace_sample, with the per-sample seeds given ([M][2]): the pairs
  * trace_sampling_para draws from the caller's generator with two next() calls
  * per sample (qm_dynamics.f90:315-320) and passes to set_seed (:326). */

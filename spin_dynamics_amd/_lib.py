@@ -30,7 +30,7 @@ PROF_FAMILIES = ["pass1_spmm_dots", "pass2_update", "final_dots", "psi_update", 
 # Exported symbols, in the order of include/spinchem.h.
 EXPORTS = [
     "spc_ctx_create", "spc_ctx_destroy", "spc_last_error", "spc_version", "spc_system_set", "spc_system_info",
-    "spc_hamiltonian_copy", "spc_n_steps", "spc_trace_sample", "spc_trace_sample_seeds", "spc_exact_trace", "spc_expm_small",
+    "spc_hamiltonian_copy", "spc_n_steps", "spc_trace_sample", "spc_trace_sample_noalloc", "spc_trace_sample_seeds", "spc_exact_trace", "spc_expm_small",
     "spc_krylov_basis", "spc_spmm", "spc_trace_sample_grad", "spc_orientation_batch", "spc_batch_tune", "spc_semiclassical_run", "spc_semiclassical_tangent", "spc_last_run_stats", "spc_profile_enable", "spc_profile_read", "spc_ini_open", "spc_ini_close",
     "spc_ini_num_sections", "spc_ini_has_section", "spc_ini_count_values", "spc_ini_get_string",
     "spc_ini_get_doubles", "spc_ini_get_int64s",
@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
     lib.spc_n_steps.restype = ctypes.c_int64
     lib.spc_trace_sample.argtypes = [vp, ctypes.POINTER(SimParams), _i64p, _dp, _dp, _dp, _dp, _dp,
                                      ctypes.c_int64, _dp, _dp]
+    lib.spc_trace_sample_noalloc.argtypes = lib.spc_trace_sample.argtypes
     lib.spc_trace_sample_seeds.argtypes = [vp, ctypes.POINTER(SimParams), _i64p, _dp, _dp, _dp, _dp, ctypes.c_int64,
                                            _dp, _dp]
     lib.spc_exact_trace.argtypes = [vp, ctypes.POINTER(SimParams), _dp, _dp, _dp, _dp, ctypes.c_int64, _dp, _dp]

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -22,6 +23,12 @@
 #include "internal.hpp"
 
 using namespace spc;
+
+// The scratch area of spc_trace_sample_noalloc: one copy per GPU, held for
+// the whole process.  In host code `scratch` only stands for the symbol: its
+// device address comes from hipGetSymbolAddress (scratch_address), once per
+// device, and only that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[SPC_SCRATCH_BYTES];
 
 namespace {
 
@@ -32,6 +39,17 @@ constexpr double kGamma14N = 0.0193297792;
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+};
+
+constexpr size_t kScratchBytes = (size_t)SPC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+// Named buffers carved from the scratch area for the duration of one call
+// (each name keeps its region while the call runs).
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t used = 0;
+    std::map<std::string, DevBuf> bufs;
 };
 
 // One Hamiltonian on the device: its parameters, CSR (device basis order) and
@@ -92,6 +110,8 @@ struct spc_ctx {
     bool fbasis_valid = false;
     // workspace
     std::map<std::string, DevBuf> bufs;
+    unsigned char *scratch = nullptr; // this device's scratch area (hipGetSymbolAddress)
+    Arena *arena = nullptr;           // set: buffers come from the scratch area, not hipMalloc
     // stats
     Prof prof;
     double last_ms = 0.0;
@@ -99,6 +119,28 @@ struct spc_ctx {
 };
 
 namespace {
+
+// This device's scratch address, looked up once per device with the device
+// current (each GPU holds its own copy of the array at its own address).
+int scratch_address(int device, unsigned char **out)
+{
+    static std::mutex mu;
+    static unsigned char *addr[kMaxDevices] = {};
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return SPC_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return SPC_ERR_HIP;
+        }
+        addr[device] = (unsigned char *)p;
+    }
+    *out = addr[device];
+    return SPC_OK;
+}
 
 int fail(spc_ctx *ctx, int code, const std::string &msg)
 {
@@ -117,12 +159,29 @@ int hip_fail(spc_ctx *ctx, hipError_t e, const char *where)
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
 
-// Grow-only named device buffer.
+// Grow-only named device buffer; inside a scratch call (ctx->arena), a region
+// of the scratch area instead, 256-byte aligned.
 template <class T>
 int ensure(spc_ctx *ctx, const char *name, size_t count, T **out)
 {
-    DevBuf &b = ctx->bufs[name];
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (Arena *a = ctx->arena) {
+        DevBuf &b = a->bufs[name];
+        if (b.bytes < bytes) {
+            const size_t need = (bytes + 255) & ~(size_t)255;
+            if (need > kScratchBytes - a->used)
+                return fail(ctx, SPC_ERR_UNSUPPORTED,
+                            std::string("the workspace exceeds the scratch area of ") +
+                                std::to_string(kScratchBytes >> 20) + " MiB (at " + name + ": " +
+                                std::to_string(a->used + need) + " bytes needed)");
+            b.p = a->base + a->used;
+            b.bytes = need;
+            a->used += need;
+        }
+        *out = (T *)b.p;
+        return SPC_OK;
+    }
+    DevBuf &b = ctx->bufs[name];
     if (b.bytes < bytes) {
         if (b.p) hipFree(b.p);
         b.p = nullptr;
@@ -646,7 +705,8 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     max_lanes = (max_lanes / 64) * 64;
     if (max_lanes < 64) return fail(ctx, SPC_ERR_OOM, "not enough device memory for one 64-vector chunk");
     int nsets = 1;
-    if (!need_gram) nsets = (int)std::min<int64_t>({lane_streams(), (M_total + 63) / 64, max_lanes / 64});
+    const bool arena = ctx->arena != nullptr; // scratch call: one chunk, no streams or events created
+    if (!need_gram && !arena) nsets = (int)std::min<int64_t>({lane_streams(), (M_total + 63) / 64, max_lanes / 64});
     const int64_t Mchunk =
         std::min<int64_t>(max_lanes / (64 * nsets) * 64, (M_total + 64 * nsets - 1) / (64 * nsets) * 64);
     const int Mp = (int)((Mchunk + 63) / 64 * 64);
@@ -678,13 +738,16 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
                 if (*e) hipEventDestroy(*e);
         }
     } drain{sets, nsets, {&ev0, &ev1, &evr}};
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
-    HIP_TRY(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
+    if (!arena) {
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        HIP_TRY(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
+    }
     for (int j = 0; j < nsets; ++j)
         if ((rc = alloc_lane_set(ctx, sets[j], j, nsets, Mp, m, n_out, need_gram, vi.kind))) return rc;
+    if (arena) sets[0].w.prof = nullptr;
     // the timed region starts after the allocations, on every stream
-    HIP_TRY(hipEventRecord(ev0, s));
+    if (ev0) HIP_TRY(hipEventRecord(ev0, s));
     for (int j = 1; j < nsets; ++j)
         if (sets[j].s != s) HIP_TRY(hipStreamWaitEvent(sets[j].s, ev0, 0));
 
@@ -769,12 +832,12 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
                 if (sets[j].s != s) HIP_TRY(hipStreamWaitEvent(sets[j].s, evr, 0));
         }
     }
-    HIP_TRY(hipEventRecord(ev1, s));
+    if (ev1) HIP_TRY(hipEventRecord(ev1, s));
     std::vector<double> acc((size_t)4 * n_out);
     HIP_TRY(hipMemcpyAsync(acc.data(), d_acc, sizeof(double) * 4 * n_out, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    if (ev0 && ev1) HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     ctx->last_ms = ms;
     ctx->last_rebuilds = rebuilds;
     ctx->last_steps = K;
@@ -1430,6 +1493,9 @@ int spc_ctx_create(int device, spc_ctx **out)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->lds_limit = prop.sharedMemPerBlock;
+    // the scratch area's address on this device (spc_trace_sample_noalloc
+    // reports the failure if there is none)
+    if (scratch_address(device, &ctx->scratch) != SPC_OK) ctx->scratch = nullptr;
     *out = ctx;
     return SPC_OK;
 }
@@ -1654,11 +1720,10 @@ int spc_hamiltonian_copy(spc_ctx *ctx, int32_t *rowptr, int32_t *col, double *va
     return SPC_OK;
 }
 
-int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2], const double *trace_vectors,
-                     double *P_S, double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S,
-                     double *Phi_T)
+static int trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2], const double *trace_vectors,
+                        double *P_S, double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S,
+                        double *Phi_T)
 {
-    if (!ctx) return SPC_ERR_INVALID;
     if (!sim) return fail(ctx, SPC_ERR_INVALID, "sim is NULL");
     HIP_TRY(hipSetDevice(ctx->device));
     const int64_t M = sim->n_samples;
@@ -1681,6 +1746,32 @@ int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed
         return fail(ctx, SPC_ERR_INVALID, "unknown vec_kind");
     }
     return propagate(ctx, sim, vi, M, seeds, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
+int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2], const double *trace_vectors,
+                     double *P_S, double *P_T0, double *P_Tp, double *P_Tm, int64_t n_out, double *Phi_S,
+                     double *Phi_T)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    return trace_sample(ctx, sim, seed, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
+int spc_trace_sample_noalloc(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2],
+                             const double *trace_vectors, double *P_S, double *P_T0, double *P_Tp, double *P_Tm,
+                             int64_t n_out, double *Phi_S, double *Phi_T)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    if (!ctx->scratch)
+        return fail(ctx, SPC_ERR_HIP, "no scratch area on this device (hipGetSymbolAddress failed at set-up)");
+    if (ctx->prof.on) return fail(ctx, SPC_ERR_INVALID, "the event profiler creates events: switch it off first");
+    Arena a;
+    a.base = ctx->scratch;
+    struct Leave {
+        spc_ctx *c;
+        ~Leave() { c->arena = nullptr; }
+    } leave{ctx};
+    ctx->arena = &a;
+    return trace_sample(ctx, sim, seed, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
 }
 
 int spc_trace_sample_seeds(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, double *P_S,
