@@ -89,6 +89,37 @@ const char *spc_version(void);
  * covers anisotropic tensors, tilted fields and full D, which the reference
  * stops on (qm_dynamics.f90:300). */
 int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys);
+/* The system of one symmetry block of symmetrised_dynamics: replaces
+ * reduce_system + getHiso_sp (symmetry.f90:146-178, hamiltonian.f90:17-205)
+ * for a reduced system.  Each nucleus stands for a set of equivalent spin-1/2
+ * nuclei in one total-spin multiplet: any multiplicity >= 1 (1, a singlet
+ * set, drops out; getHiso_sp stops above 3), isotropic couplings (the
+ * isotropy rule as iso_rule = 1), the nuclear Zeeman term of 1H for every
+ * set.  Otherwise as spc_system_set. */
+int spc_system_set_block(spc_ctx *ctx, const spc_spin_system *sys);
+
+/* shrink (moments.f90:15-93): the M sets of equivalent nuclei (couplings
+ * a_bar[M], counts n_bar[M], sum n_bar = n) whose first M power sums match
+ * those of the n couplings a_iso, via the M-point Gauss rule of the discrete
+ * measure on a_iso (quad_rule, moments.f90:95-150), integer counts from its
+ * weights and Newton's method on the moment equations (moments.f90:184-239).
+ * *M is set to 0 when n = 0.  SPC_ERR_INVALID if M >= n ('Error: Symetry
+ * reduction requires M<N'); SPC_ERR_BREAKDOWN if no integer choice converges.
+ * Host only: needs no context or device. */
+int spc_shrink(int32_t n, const double *a_iso, int32_t *M, double *a_bar, int32_t *n_bar);
+
+/* The blocks of symmetrised_dynamics for n_sets sets of n_bar[i] equivalent
+ * spin-1/2 nuclei (cartesian_product, sort_col_prod and weight of
+ * symmetry.f90:61-65, 187-226, utils.f90:25-50): every combination of one
+ * total-spin multiplicity per set (1 + n mod 2, 3 + n mod 2, .., n + 1, the
+ * first set varying fastest), sorted stably by nuclear dimension.  Out, for
+ * block b: mult[b][n_sets], Z_block[b] = the product of its multiplicities,
+ * w_block[b] = the number of copies of that multiplet combination (its weight
+ * in the full trace is w_block Z_block / Z).  With mult == NULL only
+ * *n_blocks is set.  Host only. */
+int spc_symmetry_blocks(int32_t n_sets, const int32_t *n_bar, int64_t capacity, int64_t *n_blocks,
+                        int32_t *mult, int64_t *Z_block, int64_t *w_block);
+
 /* Dimensions of the current Hamiltonian; nnz = the entries
  * spc_hamiltonian_copy writes. */
 int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz);

@@ -7,6 +7,7 @@ baseline leg of bench.py, always as the checker, never as the product path.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -36,7 +37,7 @@ class OrcSystem(ctypes.Structure):
     _fields_ = [("n1", ctypes.c_int), ("n2", ctypes.c_int), ("mult", _c_int_p),
                 ("hfc", _c_double_p), ("J", ctypes.c_double), ("D", ctypes.c_double * 9),
                 ("kS", ctypes.c_double), ("kT", ctypes.c_double), ("B", ctypes.c_double),
-                ("dir", ctypes.c_double * 3)]
+                ("dir", ctypes.c_double * 3), ("nuclear_1h", ctypes.c_int)]
 
 
 class OrcCsr(ctypes.Structure):
@@ -107,6 +108,7 @@ class System:
     kT: float = 0.0
     B: float = 0.0
     direction: Sequence[float] = (0.0, 0.0, 1.0)
+    nuclear_1h: bool = False  # 1H's gyromagnetic ratio for every nucleus (symmetry blocks)
 
     def tensors(self) -> np.ndarray:
         n1, n2 = len(self.mult1), len(self.mult2)
@@ -134,6 +136,7 @@ class System:
         s.kS, s.kT, s.B = self.kS, self.kT, self.B
         for i in range(3):
             s.dir[i] = float(self.direction[i])
+        s.nuclear_1h = int(bool(self.nuclear_1h))
         s._keep = (mult, hfc)
         return s
 
@@ -352,3 +355,147 @@ def semiclassical(a_iso1, a_iso2, mult1, mult2, *, B: float, direction=(0, 0, 1)
     if rc:
         raise RuntimeError(f"orc_semiclassical failed with code {rc}")
     return (out, out_t) if with_pt else out
+
+
+# ------------------------------------------------------- symmetry reduction
+# symmetry.f90:16-246 and moments.f90:15-240, restated with numpy/scipy for
+# the tridiagonal eigenproblem (stdlib steqr) and the linear solve (solve_lu).
+
+def quad_rule(w_init, x_init, n):
+    """moments.f90:95-150: discrete Stieltjes procedure on diag(x) from
+    sqrt(w), then the n-point Gauss rule (nodes ascending, weights)."""
+    from scipy.linalg import eigh_tridiagonal
+    x_init = np.asarray(x_init, float)
+    q = np.sqrt(np.asarray(w_init, float))
+    P = np.zeros((len(x_init), n))
+    W = np.zeros(n)
+    X = np.zeros(n)
+    for k in range(n):
+        q2 = float(np.linalg.norm(q))
+        if q2 == 0.0:
+            raise RuntimeError("quad_rule 1")
+        P[:, k] = q / q2
+        q = x_init * P[:, k]
+        for j in range(k, -1, -1):
+            pq = float(P[:, j] @ q)
+            if j == k:
+                W[k], X[k] = q2, pq
+            q = q - pq * P[:, j]
+    weight = W[0]
+    E = np.roll(W, -1)  # cshift(W, 1)
+    lam, Zv = eigh_tridiagonal(X, E[:n - 1])
+    return (weight * Zv[0, :]) ** 2, lam
+
+
+def newton(aw_bar, w_bar, a_bar, n_bar, m):
+    """moments.f90:184-239; returns (a_bar, icode)."""
+    a = np.array(a_bar, float)
+    n_bar = np.asarray(n_bar, float)
+    dap = 0.0
+    for it in range(1, 31):
+        f = np.array([n_bar @ a ** k - w_bar @ aw_bar ** k for k in range(1, m + 1)])
+        G = np.array([k * n_bar * a ** (k - 1) for k in range(1, m + 1)])
+        try:
+            q = np.linalg.solve(G, f)
+        except np.linalg.LinAlgError:  # a set of size 0: stdlib's solve_lu stops; the candidate is dropped
+            return a, 1
+        a_abs, d_abs = np.sum(np.abs(a)), np.sum(np.abs(q))
+        prev = a.copy()
+        a = a - q
+        da = d_abs / a_abs
+        if da > 1.0:
+            return a, 1
+        if 1.0 + da == 1.0:
+            return a, 0
+        if it > 1 and da >= dap:
+            return prev, 0
+        dap = da
+    return a, 2
+
+
+def shrink(a_iso, M):
+    """moments.f90:15-93: (a_bar, n_bar) of M sets of equivalent nuclei."""
+    a_iso = np.asarray(a_iso, float)
+    if a_iso.size == 0:
+        return np.zeros(0), np.zeros(0, int)
+    if M >= a_iso.size:
+        raise ValueError("Error: Symetry reduction requires M<N")
+    w_bar, aw_bar = quad_rule(np.ones(a_iso.size), a_iso, M)
+    nfloor = w_bar.astype(int)
+    ndiff = a_iso.size - int(nfloor.sum())
+    best, dmom = None, np.inf
+    for ib in range(1 << M):
+        if bin(ib).count("1") != ndiff:
+            continue
+        nt = nfloor + np.array([(ib >> i) & 1 for i in range(M)])
+        at, icode = newton(aw_bar, w_bar, aw_bar.copy(), nt, M)
+        if icode != 0:
+            continue
+        emom = sum(abs(np.sum(at ** k) - np.sum(a_iso ** k)) for k in range(1, M + 2))
+        if emom < dmom:
+            dmom, best = emom, (at, nt)
+    if best is None:
+        raise RuntimeError("shrink: no integer weights converged")
+    return best
+
+
+def symmetry_blocks(n_bar):
+    """cartesian_product + sort_col_prod + weight (symmetry.f90:61-65,
+    187-226, utils.f90:25-50): (K [blocks][sets], Z [blocks], w [blocks])."""
+    n_bar = [int(x) for x in n_bar]
+    combos = [[]]
+    for n in n_bar:  # the first set varies fastest
+        opts = [2 * j + 1 + n % 2 for j in range(n // 2 + 1)]
+        combos = [c + [g] for g in opts for c in combos]
+    Z = [int(np.prod(c, dtype=np.int64)) for c in combos]
+    order = sorted(range(len(combos)), key=lambda i: Z[i])  # Python's sort is stable
+    K = [combos[i] for i in order]
+    Zs = [Z[i] for i in order]
+    W = []
+    for k in K:
+        w = 1
+        for n, g in zip(n_bar, k):
+            w *= math.comb(n, (n + g - 1) // 2) * (2 * g) // (n + g + 1)
+        W.append(w)
+    return K, Zs, W
+
+
+def symmetrised_dynamics(a1, a2, M1, M2, *, J, kS, kT, B, m, t_end, dt, N_samples, master_seed, block_tol=0.0,
+                         tol=0.0, vec_kind=0, init_state=0, D=None):
+    """symmetry.f90:16-139 with the n_bar copy and multiplicities above 3 as
+    the method intends: per block, trace sampling of the reduced system with
+    its own master seed (two next() draws per block from master_seed), each
+    scaled by w Z_block / Z; returns the summed traces and the block list."""
+    a1_bar, n1_bar = shrink(a1, M1)
+    a2_bar, n2_bar = shrink(a2, M2)
+    m1 = len(a1_bar)
+    n_bar = list(n1_bar) + list(n2_bar)
+    K, Zb, W = symmetry_blocks(n_bar)
+    Zfull = 2 ** (len(a1) + len(a2))
+    r = lib()
+    rng = OrcRng()
+    r.orc_rng_init_default(ctypes.byref(rng))
+    r.orc_rng_set_seed(ctypes.byref(rng), (ctypes.c_int64 * 2)(*master_seed))
+    seeds = [(r.orc_rng_next(ctypes.byref(rng)), r.orc_rng_next(ctypes.byref(rng))) for _ in K]
+    n_out = n_steps(t_end, dt) + 1
+    tot = {k: np.zeros(n_out) for k in ("P_S", "P_T0", "P_Tp", "P_Tm")}
+    blocks = []
+    for i, (k, z, w) in enumerate(zip(K, Zb, W)):
+        wk = w * z / Zfull
+        blocks.append((k, z, w, wk))
+        if wk < block_tol:
+            continue
+        g1 = [g for g in k[:m1]]
+        g2 = [g for g in k[m1:]]
+        keep1 = [j for j, g in enumerate(g1) if g > 1]
+        keep2 = [j for j, g in enumerate(g2) if g > 1]
+        s = System(mult1=[g1[j] for j in keep1], mult2=[g2[j] for j in keep2],
+                   A1=np.array([a1_bar[j] * np.eye(3) for j in keep1]) if keep1 else None,
+                   A2=np.array([a2_bar[j] * np.eye(3) for j in keep2]) if keep2 else None,
+                   J=J, D=np.zeros((3, 3)) if D is None else D, kS=kS, kT=kT, B=B, nuclear_1h=True)
+        h = build_general_csr(s)
+        res = trace_sampling(h, s.Z, M=N_samples, m=m, t_end=t_end, dt=dt, tol=tol, init_state=init_state,
+                             vec_kind=vec_kind, seed=seeds[i])
+        for key in tot:
+            tot[key] += res[key] * wk
+    return tot, blocks, (a1_bar, n1_bar, a2_bar, n2_bar)

@@ -400,7 +400,7 @@ int orc_build_iso_csr(const orc_system *s, orc_csr *out)
             int64_t Zr = el == 0 ? prod_range(g, k + 1, n1) * Z2 : prod_range(g, k + 1, n1 + n2);
             const double *A = s->hfc + 9 * k;
             double a_iso = (A[0] + A[4] + A[8]) / 3.0; /* spinsys.f90:232 */
-            double gam = g[k] == 3 ? ORC_GAMMA_14N : ORC_GAMMA_1H;
+            double gam = g[k] == 3 && !s->nuclear_1h ? ORC_GAMMA_14N : ORC_GAMMA_1H;
             double zeeman_factor = -w * gam / ORC_GAMMA_E;
             coo_t Iz, Ip, Im;
             get_spinop_sp(g[k], &Iz, &Ip, &Im);
@@ -497,7 +497,7 @@ static int general_row(const orc_system *s, const int64_t *stride, int64_t Z, in
     for (int k = 0; k < nn; ++k) {
         const double *A = s->hfc + 9 * k;
         int se = k < s->n1 ? s1 : s2;
-        double gam = s->mult[k] == 3 ? ORC_GAMMA_14N : ORC_GAMMA_1H;
+        double gam = s->mult[k] == 3 && !s->nuclear_1h ? ORC_GAMMA_14N : ORC_GAMMA_1H;
         double zeta = s->B * gam / ORC_GAMMA_E;
         double mz = 0.5 * (s->mult[k] - 1) - d[k];
         diag += A[8] * spin_elem(2, 2, se, se) * mz + zeta * s->dir[2] * mz;
@@ -543,7 +543,7 @@ static int general_row(const orc_system *s, const int64_t *stride, int64_t Z, in
         int g = s->mult[k];
         int el = k < s->n1 ? 0 : 1;
         int se = el == 0 ? s1 : s2;
-        double gam = g == 3 ? ORC_GAMMA_14N : ORC_GAMMA_1H;
+        double gam = g == 3 && !s->nuclear_1h ? ORC_GAMMA_14N : ORC_GAMMA_1H;
         double zeta = s->B * gam / ORC_GAMMA_E;
         for (int dd = -1; dd <= 1; dd += 2) {
             int dc = d[k] + dd;

@@ -73,6 +73,9 @@ typedef struct {
     double kS, kT;       /* recombination rates in mT units                     */
     double B;            /* field magnitude, mT (w = -B, main.f90:55-56)        */
     double dir[3];       /* unit field direction; (0,0,1) is the reference      */
+    int nuclear_1h;      /* 0: 14N's gyromagnetic ratio for multiplicity 3, 1H's
+                            otherwise (hamiltonian.f90:132-138); 1: 1H's for
+                            every nucleus (the multiplets of a symmetry block) */
 } orc_system;
 
 typedef struct {

@@ -29,7 +29,8 @@ PROF_FAMILIES = ["pass1_spmm_dots", "pass2_update", "final_dots", "psi_update", 
 
 # Exported symbols, in the order of include/spinchem.h.
 EXPORTS = [
-    "spc_ctx_create", "spc_ctx_destroy", "spc_last_error", "spc_version", "spc_system_set", "spc_system_info",
+    "spc_ctx_create", "spc_ctx_destroy", "spc_last_error", "spc_version", "spc_system_set", "spc_system_set_block",
+    "spc_shrink", "spc_symmetry_blocks", "spc_system_info",
     "spc_hamiltonian_copy", "spc_n_steps", "spc_trace_sample", "spc_trace_sample_noalloc", "spc_trace_sample_seeds", "spc_exact_trace", "spc_expm_small",
     "spc_krylov_basis", "spc_spmm", "spc_trace_sample_grad", "spc_orientation_batch", "spc_batch_tune", "spc_semiclassical_run", "spc_semiclassical_tangent", "spc_last_run_stats", "spc_profile_enable", "spc_profile_read", "spc_ini_open", "spc_ini_close",
     "spc_ini_num_sections", "spc_ini_has_section", "spc_ini_count_values", "spc_ini_get_string",
@@ -85,6 +86,9 @@ def load() -> ctypes.CDLL:
     lib.spc_last_error.restype = ctypes.c_char_p
     lib.spc_version.restype = ctypes.c_char_p
     lib.spc_system_set.argtypes = [vp, ctypes.POINTER(SpinSystem)]
+    lib.spc_system_set_block.argtypes = [vp, ctypes.POINTER(SpinSystem)]
+    lib.spc_shrink.argtypes = [ctypes.c_int32, _dp, _i32p, _dp, _i32p]
+    lib.spc_symmetry_blocks.argtypes = [ctypes.c_int32, _i32p, ctypes.c_int64, _i64p, _i32p, _i64p, _i64p]
     lib.spc_system_info.argtypes = [vp, _i64p, _i64p, _i64p]
     lib.spc_hamiltonian_copy.argtypes = [vp, _i32p, _i32p, _dp]
     lib.spc_n_steps.argtypes = [ctypes.c_double, ctypes.c_double]

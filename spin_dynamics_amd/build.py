@@ -32,9 +32,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-f
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", os.path.join(ROOT, "include"), "-Wno-unused-result"]
 HIP_SOURCES = ["hamiltonian.hip", "krylov.hip", "sampling.hip", "semiclassical.hip", "adjoint.hip"]
-CXX_SOURCES = ["capi.cpp", "ini.cpp"]
-FORTRAN_SOURCES = ["spinchem_capi.f90", "spc_random.f90", "variables.f90", "class_observables.f90", "matrix_exponential.f90",
-                   "dynamics.f90", "sc_dynamics.f90", "main.f90"]
+CXX_SOURCES = ["capi.cpp", "ini.cpp", "symmetry.cpp"]
+FORTRAN_SOURCES = ["spinchem_capi.f90", "spc_random.f90", "class_observables.f90", "variables.f90", "matrix_exponential.f90",
+                   "dynamics.f90", "sc_dynamics.f90", "symmetry.f90", "main.f90"]
 
 
 def _run(cmd):

@@ -393,8 +393,15 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     HIP_TRY(hipStreamSynchronize(s));
     h.nnz = nnz;
     h.nblocks = nb;
-    // the Kronecker form used by the product
+    // the Kronecker form used by the product (its tables hold multiplicities
+    // up to 3; larger ones use the block product)
     const KronDev hk = kron_of(S);
+    if (std::any_of(S.mult, S.mult + S.nn, [](int g) { return g > 3; })) {
+        h.kq = 0;
+        h.slow_mult = hk.nn > 0 ? hk.mult[0] : 1;
+        h.slow_stride = hk.nn > 0 ? hk.stride[0] : 0;
+        return SPC_OK;
+    }
     h.kq = kron_inwave(hk);
     h.slow_mult = hk.nn > 0 ? hk.mult[0] : 1;
     h.slow_stride = hk.nn > 0 ? hk.stride[0] : 0;
@@ -971,7 +978,7 @@ void contract_gradient(const DevSystem &sys, const std::vector<double2> &S, doub
         gI[0] = 0.5 * (tp + tm);
         gI[1] = cd(0, -0.5) * tp + cd(0, 0.5) * tm;
         gI[2] = tz;
-        const double gam = (sys.mult[k] == 3 ? kGamma14N : kGamma1H) / kGammaE;
+        const double gam = sys.gamma[k] / kGammaE;
         for (int a = 0; a < 3; ++a) field[a] += f * gam * std::imag(gI[a]);
     }
     cd PT[4][4];
@@ -1000,7 +1007,7 @@ int build_field_basis(spc_ctx *ctx)
             S.w[a] = -b[a];
             S.dir[a] = nb > 0.0 ? b[a] / nb : (a == 2 ? 1.0 : 0.0);
         }
-        for (int k = 0; k < hs.nn; ++k) S.zeta[k] = nb * (hs.mult[k] == 3 ? kGamma14N : kGamma1H) / kGammaE;
+        for (int k = 0; k < hs.nn; ++k) S.zeta[k] = nb * hs.gamma[k] / kGammaE;
         return S;
     };
     // generic direction: every Zeeman term present
@@ -1520,10 +1527,33 @@ void spc_ctx_destroy(spc_ctx *ctx)
 
 const char *spc_last_error(const spc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
+// spc_system_set (block = false) and spc_system_set_block (block = true: the
+// nuclei are total-spin multiplets of sets of equivalent spin-1/2 nuclei, of
+// any multiplicity; multiplicity 1 drops out; the nuclear Zeeman is 1H's).
+static int system_set(spc_ctx *ctx, const spc_spin_system *sys, bool block)
 {
     if (!ctx || !sys) return fail(ctx, SPC_ERR_INVALID, "NULL argument");
     HIP_TRY(hipSetDevice(ctx->device));
+    spc_spin_system red;
+    std::vector<int32_t> red_mult;
+    std::vector<double> red_hfc;
+    if (block) {
+        const int n0 = sys->n_nuc1 + sys->n_nuc2;
+        if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || (n0 > 0 && (!sys->mult || !sys->hfc)))
+            return fail(ctx, SPC_ERR_INVALID, "bad nucleus counts or NULL mult/hfc");
+        red = *sys;
+        red.n_nuc1 = red.n_nuc2 = 0;
+        for (int k = 0; k < n0; ++k) {
+            if (sys->mult[k] < 1) return fail(ctx, SPC_ERR_INVALID, "multiplicity must be >= 1");
+            if (sys->mult[k] == 1) continue; // a singlet set: dimension 1, no coupling
+            red_mult.push_back(sys->mult[k]);
+            red_hfc.insert(red_hfc.end(), sys->hfc + 9 * k, sys->hfc + 9 * k + 9);
+            ++(k < sys->n_nuc1 ? red.n_nuc1 : red.n_nuc2);
+        }
+        red.mult = red_mult.data();
+        red.hfc = red_hfc.data();
+        sys = &red;
+    }
     const int nn = sys->n_nuc1 + sys->n_nuc2;
     if (sys->n_nuc1 < 0 || sys->n_nuc2 < 0 || nn > SPC_MAX_NUCLEI)
         return fail(ctx, SPC_ERR_UNSUPPORTED, "number of nuclei outside 0.." + std::to_string(SPC_MAX_NUCLEI));
@@ -1534,7 +1564,8 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     int64_t Z = 1;
     for (int k = 0; k < nn; ++k) {
         const int g = sys->mult[k];
-        if (g != 2 && g != 3) // hamiltonian.f90:132-138
+        if (block && g > 64) return fail(ctx, SPC_ERR_UNSUPPORTED, "block multiplicity above 64");
+        if (!block && g != 2 && g != 3) // hamiltonian.f90:132-138
             return fail(ctx, SPC_ERR_UNSUPPORTED, std::string("Error: sys%e") + (k < sys->n_nuc1 ? "1" : "2") +
                                                       "%g_I(i) must be 3 or 2");
         S.mult[k] = g;
@@ -1563,7 +1594,10 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
         } else {
             for (int q = 0; q < 9; ++q) S.A[k][q] = A[q];
         }
-        const double gam = S.mult[k] == 3 ? kGamma14N : kGamma1H;
+        // hamiltonian.f90:132-138: 14N for multiplicity 3, else 1H; a block's
+        // multiplets are made of 1H
+        const double gam = !block && S.mult[k] == 3 ? kGamma14N : kGamma1H;
+        S.gamma[k] = gam;
         S.zeta[k] = sys->B * gam / kGammaE; // -w gamma_N/gamma_e with w = -B
     }
     double nrm = std::sqrt(sys->field_dir[0] * sys->field_dir[0] + sys->field_dir[1] * sys->field_dir[1] +
@@ -1621,6 +1655,10 @@ int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys)
     ctx->have_sys = true;
     return SPC_OK;
 }
+
+int spc_system_set(spc_ctx *ctx, const spc_spin_system *sys) { return system_set(ctx, sys, false); }
+
+int spc_system_set_block(spc_ctx *ctx, const spc_spin_system *sys) { return system_set(ctx, sys, true); }
 
 int spc_system_info(const spc_ctx *ctx, int64_t *N, int64_t *Z, int64_t *nnz)
 {

@@ -18,6 +18,7 @@ struct DevSystem {
     int64_t stride[64];      // digit stride in the nuclear index
     double A[64][9];         // A(a,b) at A[k][3*b + a]
     double zeta[64];         // nuclear Zeeman B*gamma_k/gamma_e
+    double gamma[64];        // gamma_k (rad s^-1 T^-1 / 1e9, the reference's units)
     double w[3];             // electron Zeeman vector, -B n
     double dir[3];           // field direction n
     double Tre[9], Tim[9];   // S1.T.S2 coupling T(a,b) at [3*b + a], T = (2J + 2i dk) 1 + D

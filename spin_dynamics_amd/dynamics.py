@@ -27,10 +27,13 @@ def context(device: int = 0) -> _lib.Context:
 class Hamiltonian:
     """The device CSR of one spin system at field B (getHiso_sp + coo_to_csr)."""
 
-    def __init__(self, sys: SysParam, B: float, ctx: Optional[_lib.Context] = None):
+    def __init__(self, sys: SysParam, B: float, ctx: Optional[_lib.Context] = None, block: bool = False):
+        """block: sys is a symmetry block (spc_system_set_block: multiplets of
+        equivalent spin-1/2 nuclei of any multiplicity, 1H nuclear Zeeman)."""
         self.ctx = ctx or context()
         s, self._keep = to_c_system(sys, B)
-        self.ctx.check(self.ctx.lib.spc_system_set(self.ctx.h, s))
+        setter = self.ctx.lib.spc_system_set_block if block else self.ctx.lib.spc_system_set
+        self.ctx.check(setter(self.ctx.h, s))
         N, Z, nnz = (np.zeros(1, np.int64) for _ in range(3))
         self.ctx.check(self.ctx.lib.spc_system_info(self.ctx.h, _lib.ptr(N, _lib._i64p), _lib.ptr(Z, _lib._i64p),
                                                     _lib.ptr(nnz, _lib._i64p)))
@@ -92,13 +95,14 @@ def _run(fn, ctx, sim, n_out, *args):
 
 def trace_sampling_para(sys: SysParam, sim: SimParam, seed=None, out: Optional[str] = None, *,
                         B: Optional[float] = None, trace_vectors: Optional[np.ndarray] = None,
-                        ctx: Optional[_lib.Context] = None) -> Observables:
+                        ctx: Optional[_lib.Context] = None, block: bool = False) -> Observables:
     """qm_dynamics.f90:257-384.  The field is B (default -sys.e1.w, as main.f90
     sets w = -B); seed is the master seed set before the call (main.f90:50).
-    trace_vectors ([M][Z] nuclear vectors) replaces the sampled ones."""
+    trace_vectors ([M][Z] nuclear vectors) replaces the sampled ones; block:
+    sys is a symmetry block (see Hamiltonian)."""
     ctx = ctx or context()
     B = -sys.e1.w if B is None else B
-    Hamiltonian(sys, B, ctx)
+    Hamiltonian(sys, B, ctx, block=block)
     seed = sim.seed if seed is None else seed
     if trace_vectors is not None:
         kind = _lib.VEC_INJECTED
@@ -118,11 +122,12 @@ def trace_sampling_para(sys: SysParam, sim: SimParam, seed=None, out: Optional[s
 
 
 def exact_dynamics(sys: SysParam, sim: SimParam, out: Optional[str] = None, *, B: Optional[float] = None,
-                   ctx: Optional[_lib.Context] = None) -> Observables:
-    """qm_dynamics.f90:24-137: the Z nuclear basis states as trace vectors."""
+                   ctx: Optional[_lib.Context] = None, block: bool = False) -> Observables:
+    """qm_dynamics.f90:24-137: the Z nuclear basis states as trace vectors
+    (block: sys is a symmetry block, see Hamiltonian)."""
     ctx = ctx or context()
     B = -sys.e1.w if B is None else B
-    Hamiltonian(sys, B, ctx)
+    Hamiltonian(sys, B, ctx, block=block)
     n_out = n_steps(sim.t_end, sim.dt) + 1
     res = _run(ctx.lib.spc_exact_trace, ctx, sim, n_out, _sim_c(sim, 0, 0))
     if out is not None:

@@ -4,7 +4,14 @@ module class_observables
     use, intrinsic :: iso_fortran_env, only: dp => real64
     implicit none
     private
-    public :: observables, write_data
+    public :: observables, write_data, output_style
+
+    ! how write_data renders a value: 0, this compiler's list-directed output
+    ! (write(u,*), as observables.f90:143); 1, gfortran's list-directed output
+    ! of a real(8), the form the reference's own builds write: a blank, then
+    ! G25.17E3 where F editing applies (0.1 <= |x| < 1e17, and 0) and 1PE25.16E3
+    ! elsewhere (17 significant digits either way)
+    integer :: output_style = 0
 
     type observables
         real(dp), allocatable :: P_S(:), P_T0(:), P_Tp(:), P_Tm(:)
@@ -87,14 +94,22 @@ contains
         if (allocated(self%Phi_T)) call write_data(self%Phi_T, path, 'T_yield.data')
     end subroutine
 
-    ! one list-directed value per line
+    ! one list-directed value per line (observables.f90:133-148)
     subroutine write_data(x, path, name)
         real(dp), intent(in) :: x(:)
         character(*), intent(in) :: path, name
         integer :: i, u
         open(newunit=u, file=trim(path) // '/' // trim(name), status='replace')
         do i = 1, size(x)
-            write(u, *) x(i)
+            if (output_style == 1) then
+                if (x(i) == 0.0_dp .or. (abs(x(i)) >= 0.1_dp .and. abs(x(i)) < 1.0e17_dp)) then
+                    write(u, '(1X,G25.17E3)') x(i)
+                else
+                    write(u, '(1X,1PE25.16E3)') x(i)
+                end if
+            else
+                write(u, *) x(i)
+            end if
         end do
         close(u)
     end subroutine

@@ -33,9 +33,12 @@ contains
         end select
     end function
 
-    ! sys_param (field B = -w, main.f90:55-56) -> spc_system_set
-    subroutine set_system(sys)
+    ! sys_param (field B = -w, main.f90:55-56) -> spc_system_set, or
+    ! spc_system_set_block for a symmetry block (multiplets of equivalent
+    ! spin-1/2 nuclei)
+    subroutine set_system(sys, block)
         type(sys_param), intent(in) :: sys
+        logical, intent(in), optional :: block
         type(spc_spin_system) :: s
         integer :: n1, n2, k, a, b
         n1 = size(sys%e1%g_I)
@@ -69,6 +72,12 @@ contains
         s%B = -sys%e1%w
         s%field_dir = sys%field_dir
         s%iso_rule = 1
+        if (present(block)) then
+            if (block) then
+                call check(spc_system_set_block(device_context(), s))
+                return
+            end if
+        end if
         call check(spc_system_set(device_context(), s))
     end subroutine
 
@@ -99,18 +108,19 @@ contains
     end subroutine
 
     ! qm_dynamics.f90:257-384: per-sample seeds drawn from rng, then the
-    ! device propagation of all samples.
-    subroutine trace_sampling_para(sys, sim, rng, res, out)
+    ! device propagation of all samples.  block: sys is a symmetry block.
+    subroutine trace_sampling_para(sys, sim, rng, res, out, block)
         type(sys_param), intent(in)            :: sys
         type(sim_param), intent(inout)         :: sim
         type(rng_t), intent(inout)             :: rng
         type(observables), intent(out)         :: res
         character(*), optional, intent(in)     :: out
+        logical, optional, intent(in)          :: block
         integer(c_int64_t), allocatable :: seeds(:,:)
         real(dp), allocatable :: phis(:), phit(:)
         integer(c_int64_t) :: n_out
         integer :: i, kind
-        call set_system(sys)
+        call set_system(sys, block)
         n_out = spc_n_steps(sim%t_end, sim%dt) + 1
         call res%malloc(int(n_out))
         allocate(seeds(2, max(sim%N_samples, 1)), phis(n_out), phit(n_out))

@@ -8,6 +8,7 @@ program main
     use class_observables
     use dynamics
     use sc_dynamics
+    use symmetry
     implicit none
 
     character(len=600)        :: arg
@@ -49,7 +50,8 @@ program main
         case ('semi_classical')
             call semi_classical(sim, sys, res, folder)
         case ('symmetry_dynamics')
-            print *, 'method not yet implemented'
+            print *, sim%B(i)
+            call symmetrised_dynamics(sys, sim, rng, folder)
         case default
             stop 'Error: Invalid simulation type entered.'
         end select

@@ -29,7 +29,8 @@ module spinchem_capi
         integer(c_int32_t) :: init_state = 0, vec_kind = 0
     end type
 
-    public :: spc_ctx_create, spc_ctx_destroy, spc_last_error, spc_system_set, spc_n_steps, &
+    public :: spc_ctx_create, spc_ctx_destroy, spc_last_error, spc_system_set, spc_system_set_block, spc_shrink, &
+              spc_symmetry_blocks, spc_n_steps, &
               spc_trace_sample, spc_trace_sample_seeds, spc_exact_trace, spc_expm_small, spc_semiclassical_run, &
               spc_ini_open, spc_ini_close, spc_ini_num_sections, spc_ini_has_section, &
               spc_ini_count_values, spc_ini_get_string, spc_ini_get_doubles, spc_ini_get_int64s, &
@@ -53,6 +54,28 @@ module spinchem_capi
             import :: c_int, c_ptr, spc_spin_system
             type(c_ptr), value :: ctx
             type(spc_spin_system), intent(in) :: sys
+        end function
+        integer(c_int) function spc_system_set_block(ctx, sys) bind(C, name='spc_system_set_block')
+            import :: c_int, c_ptr, spc_spin_system
+            type(c_ptr), value :: ctx
+            type(spc_spin_system), intent(in) :: sys
+        end function
+        integer(c_int) function spc_shrink(n, a_iso, M, a_bar, n_bar) bind(C, name='spc_shrink')
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: a_iso(*)
+            integer(c_int32_t), intent(inout) :: M
+            real(c_double), intent(out) :: a_bar(*)
+            integer(c_int32_t), intent(out) :: n_bar(*)
+        end function
+        integer(c_int) function spc_symmetry_blocks(n_sets, n_bar, capacity, n_blocks, mult, Z_block, w_block) &
+                bind(C, name='spc_symmetry_blocks')
+            import :: c_int, c_int32_t, c_int64_t, c_ptr
+            integer(c_int32_t), value :: n_sets
+            integer(c_int32_t), intent(in) :: n_bar(*)
+            integer(c_int64_t), value :: capacity
+            integer(c_int64_t), intent(out) :: n_blocks
+            type(c_ptr), value :: mult, Z_block, w_block
         end function
         integer(c_int64_t) function spc_n_steps(t_end, dt) bind(C, name='spc_n_steps')
             import :: c_int64_t, c_double

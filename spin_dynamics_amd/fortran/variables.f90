@@ -5,6 +5,7 @@ module variables
     use, intrinsic :: iso_fortran_env, only: dp => real64, i8 => int64
     use, intrinsic :: iso_c_binding
     use spinchem_capi
+    use class_observables, only: output_style
     implicit none
     private
     public :: g_e, gamma_e, gamma_1H, gamma_14N, electron, sys_param, sim_param, read_inp
@@ -168,6 +169,10 @@ contains
         if (has(ini, 'simulation parameters', 'field_direction')) &
             call get_real(ini, 'simulation parameters', 'field_direction', sys%field_dir)
 
+        if (has(ini, 'simulation parameters', 'output_format')) then
+            ! extension key: 'gfortran' writes the .data files as the reference's gfortran builds do
+            if (get_string(ini, 'simulation parameters', 'output_format') == 'gfortran') output_style = 1
+        end if
         sim%output_folder = get_string(ini, 'simulation parameters', 'output_folder')
         inquire(file=sim%output_folder // '/.', exist=dirExists)
         if (.not. dirExists) stop 'Error: Output folder does not exist.'

@@ -152,6 +152,24 @@ def small_anisotropic(m1=(3, 2), m2=(2,), M=8, m=12, T_ns=200.0, tol=0.0, vec_ki
                     1.0)
 
 
+def symmetry_protons(a1=(1.0, 0.3, 0.5, 0.2, 0.1, 0.5, 0.5, 0.5), a2=(), *, M1=4, M2=1, B=0.0, kS_us=20.0,
+                     kT_us=5.0, J=0.0, M=10, m=7, T_ns=300.0, dt_ns=0.3, tol=1e-5, block_tol=1e-5,
+                     vec_kind="SUZ") -> Workload:
+    """The reference's symmetry_dynamics input (input.ini): spin-1/2 nuclei
+    with isotropic couplings a1 (electron 1) and a2 (electron 2) in mT,
+    reduced to M1 and M2 sets of equivalent nuclei; kS = 20, kT = 5 us^-1,
+    N_samples = 10, N_krylov = 7, dt = 0.3 ns, T = 300 ns, tol = 1e-5,
+    block_tolerance = 1e-5."""
+    e1 = _electron([2] * len(a1), [a * np.eye(3) for a in a1]) if a1 else Electron()
+    e2 = _electron([2] * len(a2), [a * np.eye(3) for a in a2]) if a2 else Electron()
+    sys = SysParam(e1=e1, e2=e2, J=J, kS=kS_us / GAMMA_E, kT=kT_us / GAMMA_E)
+    sys.e1.w = sys.e2.w = -B
+    sim = _sim(M, m, T_ns, dt_ns, tol=tol, vec_kind=vec_kind)
+    sim.type = "symmetry_dynamics"
+    sim.M1, sim.M2, sim.block_tol = M1, M2, block_tol
+    return Workload("symmetry_protons", sys, sim, B, kS_us, kT_us, T_ns, dt_ns)
+
+
 @dataclass
 class ScWorkload:
     """A semiclassical system (isotropic couplings) and its run parameters."""
@@ -229,5 +247,7 @@ def write_ini(w: Workload, path: str, output_folder: str, *, simulation_type: st
               f"N_samples = {sim.N_samples}", f"simulation_time = {w.T_ns!r}", f"dt = {w.dt_ns!r}",
               f"N_krylov = {sim.N_krylov}", f"integrator_tolerance = {sim.tol!r}",
               f"field_direction = {fmt(s.field_dir)}", f"trace_vectors = {sim.vec_kind}"]
+    if sim.M1 or sim.M2:
+        lines += [f"M1 = {sim.M1}", f"M2 = {sim.M2}", f"block_tolerance = {sim.block_tol!r}"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -37,8 +37,10 @@ def embed(op, site, dims):
 
 
 def hamiltonian(mult1, mult2, A, *, B=0.0, direction=(0, 0, 1), J=0.0, D=None, kS=0.0, kT=0.0,
-                nuclear_zeeman=True):
-    """A: (n1+n2, 3, 3) hyperfine tensors A[k][a][b] (mT); kS/kT in mT units."""
+                nuclear_zeeman=True, gammas=None):
+    """A: (n1+n2, 3, 3) hyperfine tensors A[k][a][b] (mT); kS/kT in mT units;
+    gammas: per-nucleus gyromagnetic ratios (default 14N's for multiplicity 3,
+    else 1H's)."""
     mult = list(mult1) + list(mult2)
     dims = [2, 2] + mult
     N = int(np.prod(dims))
@@ -56,7 +58,7 @@ def hamiltonian(mult1, mult2, A, *, B=0.0, direction=(0, 0, 1), J=0.0, D=None, k
             for b in range(3):
                 H += A[k][a][b] * Se[a] @ Ik[b]
         if nuclear_zeeman:
-            gam = GAMMA_14N if g == 3 else GAMMA_1H
+            gam = gammas[k] if gammas is not None else (GAMMA_14N if g == 3 else GAMMA_1H)
             for b in range(3):
                 H += (B * gam / GAMMA_E) * n[b] * Ik[b]
     T = 2 * J * np.eye(3) + D
