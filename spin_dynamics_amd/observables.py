@@ -9,6 +9,25 @@ from typing import Optional
 import numpy as np
 
 
+def gfortran_real(x: float) -> str:
+    """One value as gfortran's list-directed output writes a real(8), the form
+    of the reference's .data files from its gfortran builds
+    (observables.f90:143): a blank, then 17 significant digits, as G25.17E3
+    where F editing applies (0.1 <= |x| < 1e17, and 0) and as 1PE25.16E3
+    elsewhere."""
+    x = float(x)
+    if x == 0.0:
+        return "   0.0000000000000000     "
+    mant, exp = f"{abs(x):.16e}".split("e")
+    e = int(exp)
+    if -1 <= e <= 16:  # F editing: 10^(n-1) <= |x| < 10^n, 17 - n decimals
+        n = e + 1
+        body = f"{x:.{17 - n}f}" + ("." if n == 17 else "")  # F editing keeps the point
+        return " " + body.rjust(20) + " " * 5
+    body = f"{'-' if x < 0 else ''}{mant}E{'-' if e < 0 else '+'}{abs(e):03d}"
+    return " " + body.rjust(25)
+
+
 @dataclass
 class Observables:
     P_S: np.ndarray
@@ -47,8 +66,9 @@ class Observables:
             self.Phi_S[i] = s
             self.Phi_T[i] = t
 
-    def output(self, path: str):
-        """observables.f90:103-148: one value per line."""
+    def output(self, path: str, style: str = "default"):
+        """observables.f90:103-148: one value per line; style "gfortran"
+        writes each as gfortran's list-directed output does (gfortran_real)."""
         if self.P_T is None:
             pops = [("S_prob.data", self.P_S), ("Tp_prob.data", self.P_Tp), ("T0_prob.data", self.P_T0),
                     ("Tm_prob.data", self.P_Tm)]
@@ -60,4 +80,4 @@ class Observables:
                 continue
             with open(os.path.join(path, name), "w") as f:
                 for x in data:
-                    f.write(f"   {x!r}\n")
+                    f.write((gfortran_real(x) if style == "gfortran" else f"   {x!r}") + "\n")

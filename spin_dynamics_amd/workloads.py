@@ -221,9 +221,9 @@ def semiclassical_48(n_samples: int = 1 << 20, T_ns: float = 2000.0, dt_ns: floa
 
 
 def write_ini(w: Workload, path: str, output_folder: str, *, simulation_type: str = "trace_sampling",
-              fields: Sequence[float] = None):
+              fields: Sequence[float] = None, output_format: str = None):
     """Writes the workload with the reference's keys (spinsys.f90:60-235);
-    field_direction and trace_vectors are extension keys."""
+    field_direction, trace_vectors and output_format are extension keys."""
     s, sim = w.sys, w.sim
     fmt = lambda v: " ".join(repr(float(x)) for x in v)
     lines = ["[system variables]", f"J = {s.J!r}", f"D = {fmt(np.asarray(s.D).reshape(3, 3).T.reshape(-1))}",
@@ -247,6 +247,8 @@ def write_ini(w: Workload, path: str, output_folder: str, *, simulation_type: st
               f"N_samples = {sim.N_samples}", f"simulation_time = {w.T_ns!r}", f"dt = {w.dt_ns!r}",
               f"N_krylov = {sim.N_krylov}", f"integrator_tolerance = {sim.tol!r}",
               f"field_direction = {fmt(s.field_dir)}", f"trace_vectors = {sim.vec_kind}"]
+    if output_format:
+        lines.append(f"output_format = {output_format}")
     if sim.M1 or sim.M2:
         lines += [f"M1 = {sim.M1}", f"M2 = {sim.M2}", f"block_tolerance = {sim.block_tol!r}"]
     with open(path, "w") as f:

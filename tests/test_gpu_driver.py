@@ -16,11 +16,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "spin_dynamics_amd", "bin", "spinchem")
 
 
-def run_driver(w, tmp_path, sim_type="trace_sampling", fields=None):
+def run_driver(w, tmp_path, sim_type="trace_sampling", fields=None, output_format=None):
     out = tmp_path / "out"
     out.mkdir(exist_ok=True)
     ini = tmp_path / "input.ini"
-    workloads.write_ini(w, str(ini), str(out), simulation_type=sim_type, fields=fields)
+    workloads.write_ini(w, str(ini), str(out), simulation_type=sim_type, fields=fields, output_format=output_format)
     r = subprocess.run([DRIVER, str(ini)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return out
@@ -90,3 +90,23 @@ def test_driver_semi_classical_matches_oracle(tmp_path):
     assert np.max(np.abs(got["T_prob"] - pt)) < 1e-10 * max(np.max(np.abs(pt)), 1e-3)
     _, phis, phit = pyoracle.get_kinetics(ps, pt, np.zeros_like(ps), np.zeros_like(ps), w.sim.dt, w.sys.kS, w.sys.kT)
     assert abs(got["S_yield"][-1] - phis[-1]) <= 1e-10 * abs(phis[-1])
+
+
+def test_driver_gfortran_output_and_parse_output(tmp_path):
+    # output_format = gfortran: every line as gfortran's list-directed output of
+    # the value it holds, and the reference's reader (example/utils.py:420-456)
+    # reads the same traces as from the default output
+    from spin_dynamics_amd.observables import gfortran_real
+    from test_output_format import parse_output
+    w = workloads.small_anisotropic(M=6, T_ns=60.0, vec_kind="SUZ")
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    a = run_driver(w, tmp_path / "a", fields=[0.05])
+    b = run_driver(w, tmp_path / "b", fields=[0.05], output_format="gfortran")
+    for name in ("S_prob.data", "T0_prob.data", "S_yield.data", "total_prob.data"):
+        lines = (b / ".050" / name).read_text().splitlines()
+        assert lines and all(line == gfortran_real(float(line)) for line in lines), name
+    pa, pb = parse_output(a, ".050", 1e-9), parse_output(b, ".050", 1e-9)
+    assert sorted(pa) == sorted(pb) == ["S_prob", "T0_prob", "Tm_prob", "Tp_prob", "time", "total_prob"]
+    for k in pa:
+        assert np.array_equal(pa[k], pb[k]), k
