@@ -1188,6 +1188,22 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
     const int Mp = w.Mp, m = w.m;
     const int64_t Z = ctx->Z, nb = ctx->fpat.nblocks;
     hipStream_t s = sl.s;
+    // host-side checks of what the enqueued work (and a replayed graph) will
+    // touch: the slot's workspace is set up for this width, and its buffers
+    // hold the lanes, values and populations of n orientations
+    {
+        const std::string sfx = sl.idx ? "#" + std::to_string(sl.idx) : "";
+        auto holds = [&](const char *base, size_t bytes) {
+            auto it = ctx->bufs.find(std::string(base) + sfx);
+            return it != ctx->bufs.end() && it->second.bytes >= bytes;
+        };
+        if (c < 0 || o0 < 0 || n < 1 || sl.n != n || Mp != n * M || w.m != plan.m ||
+            !holds("bvals", (size_t)n * nb * 16 * sizeof(double2)) ||
+            !holds("obs", (size_t)nout * 4 * Mp * sizeof(double)) ||
+            !holds("psi", (size_t)ctx->N * Mp * sizeof(double2)) ||
+            !holds("Q", (size_t)m * w.qs * sizeof(double2)) || !holds("part", w.part_capacity * sizeof(double2)))
+            return fail(ctx, SPC_ERR_INVALID, "batched orientations: a chunk does not fit its slot's workspace");
+    }
     HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], sh.fields + 3 * (size_t)o0, nb * 16, n,
                            const_cast<double2 *>(w.bval), s));
     if (w.kron)
