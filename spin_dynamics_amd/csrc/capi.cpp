@@ -576,7 +576,9 @@ int start_pass(spc_ctx *ctx, LaneSet &L, const double pop[4])
 int rebuild(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, bool gram)
 {
     KrylovWork &w = L.w;
+    w.fold_psi = !gram; // one step per basis: nothing reads the basis after the psi pass
     HIP_TRY(arnoldi(w, L.s));
+    w.fold_psi = false;
     HIP_TRY(expm_lanes(w, sim->dt, L.s));
     if (gram) HIP_TRY(gram_projectors(w, L.s));
     HIP_TRY(step_lanes(w, sim->dt, sim->t_end, sim->tol, gram, L.s));
@@ -1224,7 +1226,10 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
                              sizeof(double) * Mp, 4, hipMemcpyDeviceToDevice, s));
     auto one_step = [&]() -> hipError_t {
         hipError_t e;
-        if ((e = arnoldi(w, s)) != hipSuccess) return e;
+        w.fold_psi = true; // one step per basis
+        e = arnoldi(w, s);
+        w.fold_psi = false;
+        if (e != hipSuccess) return e;
         if ((e = expm_lanes(w, sim->dt, s)) != hipSuccess) return e;
         if ((e = step_lanes(w, sim->dt, sim->t_end, 0.0, false, s)) != hipSuccess) return e;
         return psi_update(w, true, s);
@@ -2454,8 +2459,12 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
         if (n < kstart && n % L == 0)
             HIP_TRY(hipMemcpyAsync(ckpt + (n / L) * NMp, w.psi, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
         if (n == ntop) HIP_TRY(singlet_source(w.psi, cw[ntop] * wscale, lam, Z, Mp, s));
-        HIP_TRY(arnoldi(w, s));
-        if (backward && n >= kstart && n < ntop) {
+        const bool keep = backward && n >= kstart && n < ntop; // the step's vectors are stored from the basis
+        w.fold_psi = !keep;
+        const hipError_t ea = arnoldi(w, s);
+        w.fold_psi = false;
+        HIP_TRY(ea);
+        if (keep) {
             HIP_TRY(expm_coeffs(w, tk, Cb, s));
             CombineOut out{};
             out.n = kGL + 1;

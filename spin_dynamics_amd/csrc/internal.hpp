@@ -219,6 +219,12 @@ struct KrylovWork {
     // positive double; 0 = none yet): the block basis loses orthogonality as
     // p falls
     unsigned long long *cond = nullptr;
+    // Set by a caller that reads nothing of the basis after psi_update (one
+    // step per rebuild): the block Arnoldi then leaves the last block's chain
+    // vectors unformed (fold_K, fold_nf > 0 after arnoldi) and psi_update
+    // folds the forming coefficients into c instead.
+    bool fold_psi = false;
+    int fold_K = 0, fold_nf = 0;
 };
 // Elements per lane of KrylovWork::sbuf.
 constexpr int kSStepBuf = 720;

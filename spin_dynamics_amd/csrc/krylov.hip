@@ -1894,6 +1894,11 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
                                m, w.M, w.Mp, sdots, scoef, w.hess, w.beta0, w.flags, w.cond);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (extra && w.fold_psi) { // psi_update forms psi from [Q_K, Y] directly
+            w.fold_K = K;
+            w.fold_nf = n;
+            break;
+        }
         {
             const int nx = K + n;
             ProfScope ps(w, PF_SFORM, (nx + n) * vec_bytes(w), s);
@@ -1905,6 +1910,28 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
         K += n;
     }
     return hipSuccess;
+}
+
+// psi's coordinates when the last block (basis columns K .. K+nf-1) was left
+// as its chain vectors y_0 .. y_{nf-1} in those slots: q_{K+j} = sum_x
+// coef[x*nf + j] x_x over x = [q_0 .. q_{K-1}, y_0 .. y_{nf-1}] (k_sform's
+// coefficients), so sum_k c_k q_k = sum_x d_x x_x with d_x = [x < K] c_x +
+// sum_j coef[x*nf + j] c_{K+j}.
+__global__ void k_fold(int K, int nf, int Mp, const double2 *__restrict__ c, const double2 *__restrict__ coef,
+                       double2 *__restrict__ d)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= Mp) return;
+    double2 cn[kSMaxNY];
+#pragma unroll
+    for (int j = 0; j < kSMaxNY; ++j) cn[j] = j < nf ? c[(int64_t)(K + j) * Mp + r] : cz(0, 0);
+    for (int x = 0; x < K + nf; ++x) {
+        double2 acc = x < K ? c[(int64_t)x * Mp + r] : cz(0, 0);
+#pragma unroll
+        for (int j = 0; j < kSMaxNY; ++j)
+            if (j < nf) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], cn[j], acc);
+        d[(int64_t)x * Mp + r] = acc;
+    }
 }
 
 } // namespace
@@ -1938,6 +1965,7 @@ size_t partials_needed(const KrylovWork &w)
 hipError_t arnoldi(KrylovWork &w, hipStream_t s)
 {
     static_assert(2 * kSMaxCols * kSMaxNY == kSStepBuf, "s-step buffer size");
+    w.fold_K = w.fold_nf = 0;
     if (w.sstep >= 2 && w.sbuf && w.m >= 8) {
         const int S = std::min(w.sstep, kSMaxNY - 1);
         const int nrb = sstep_nrb(w, S);
@@ -2030,12 +2058,23 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s)
     const int nch = w.Mp / kWave;
     const int nrbp = nrb_for(w.Z, w.rpwp);
     const PsiFn fn = observe ? kPsiObs[w.m - 1] : kPsiPlain[w.m - 1];
+    const double2 *cvec = w.c;
+    hipError_t e = hipSuccess;
+    if (w.fold_nf > 0) { // the last block unformed: psi's coordinates over [Q_K, Y] (the dots area is free)
+        if (w.fold_K + w.fold_nf != w.m || w.fold_nf >= kSMaxNY) return hipErrorInvalidValue;
+        double2 *const d = w.sbuf, *const coef = w.sbuf + (size_t)kSMaxCols * kSMaxNY * w.Mp;
+        hipLaunchKernelGGL(k_fold, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.fold_K, w.fold_nf, w.Mp,
+                           w.c, coef, d);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        cvec = d;
+        w.fold_K = w.fold_nf = 0;
+    }
     {
         ProfScope ps(w, PF_PSI, (w.m + 1) * vec_bytes(w), s);
-        hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, w.c, w.psi, w.part, w.Z, w.Mp, w.rpwp,
+        hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, cvec, w.psi, w.part, w.Z, w.Mp, w.rpwp,
                            nrbp, qstride(w));
     }
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess || !observe) return e;
     if ((e = reduce(w, 2, nrbp, w.dots1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_obs_store, dim3((unsigned)ceil_div(w.M, 64)), dim3(64), 0, s, w.M, w.Mp, w.dots1, w.lane_k,
