@@ -32,7 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-f
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", os.path.join(ROOT, "include"), "-Wno-unused-result"]
 HIP_SOURCES = ["hamiltonian.hip", "krylov.hip", "sampling.hip", "semiclassical.hip", "adjoint.hip"]
-CXX_SOURCES = ["capi.cpp", "ini.cpp", "symmetry.cpp"]
+CXX_SOURCES = ["capi.cpp", "ini.cpp", "symmetry.cpp", "newton.cpp"]
 FORTRAN_SOURCES = ["spinchem_capi.f90", "spc_random.f90", "class_observables.f90", "variables.f90", "matrix_exponential.f90",
                    "dynamics.f90", "sc_dynamics.f90", "symmetry.f90", "main.f90"]
 
@@ -53,7 +53,7 @@ def build_library(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(OBJ_DIR, src + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") or src == "capi.cpp" else ["-x", "c++"]
+        lang = ["-x", "hip"] if src.endswith(".hip") or src in ("capi.cpp", "newton.cpp") else ["-x", "c++"]
         jobs.append((src, [HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj], obj))
     n = min(len(jobs), int(os.environ.get("MAX_JOBS", "8")))
     with ThreadPoolExecutor(max_workers=max(n, 1)) as ex:

@@ -67,6 +67,7 @@ struct HamDev {
     int slow_mult = 1;        // multiplicity of the slowest nucleus (digit of largest stride)
     int64_t slow_stride = 0;  // its stride in nuclear indices
     uint64_t *kmask = nullptr; // [Z][3] neighbour masks of the Kronecker product
+    std::vector<double> shifts; // Newton shifts (ham_shifts; empty until first use)
     void release()
     {
         hipFree(kmask);
@@ -430,6 +431,27 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
     return SPC_OK;
 }
 
+// The Newton shifts of h (newton_shifts_csr on a host copy of its CSR),
+// computed on first use and kept with h; null if they cannot be had (the
+// block Arnoldi then keeps the monomial chain).
+const double *ham_shifts(spc_ctx *ctx, HamDev &h, int64_t N)
+{
+    if (!h.shifts.empty()) return h.shifts.data();
+    if (!h.rowptr || !h.col || !h.val || N < 1 || h.nnz < 1) return nullptr;
+    std::vector<int32_t> rp((size_t)N + 1), cl((size_t)h.nnz);
+    std::vector<double2> vl((size_t)h.nnz);
+    hipStream_t s = ctx->stream;
+    if (hipMemcpyAsync(rp.data(), h.rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(cl.data(), h.col, sizeof(int32_t) * cl.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(vl.data(), h.val, sizeof(double2) * vl.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return nullptr;
+    std::vector<double> sh(SPC_MAX_KRYLOV);
+    if (newton_shifts_csr(N, rp.data(), cl.data(), vl.data(), SPC_MAX_KRYLOV, sh.data())) return nullptr;
+    h.shifts = std::move(sh);
+    return h.shifts.data();
+}
+
 // Workspace, stream and host staging of one lane chunk of propagate().
 struct LaneSet {
     KrylovWork w{};
@@ -483,6 +505,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     w.m = m;
     w.n_out = n_out;
     set_ham(w, ctx->ham);
+    w.shifts = ham_shifts(ctx, ctx->ham, N);
     w.prof = &ctx->prof;
     w.kS = ctx->kS;
     w.kT = ctx->kT;
@@ -522,7 +545,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)n_out * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
-    if (w.sstep >= 2 && (rc = ensure(ctx, nm("cond").c_str(), (size_t)1, &w.cond))) return rc;
+    if (w.sstep >= 2 && (rc = ensure(ctx, nm("cond").c_str(), (size_t)kCondWords, &w.cond))) return rc;
     if (vkind == 0 || vkind == 1)
         if ((rc = ensure(ctx, nm("seeds").c_str(), (size_t)2 * Mp, &L.d_seeds))) return rc;
     if (vkind == 2)
@@ -563,7 +586,7 @@ int start_pass(spc_ctx *ctx, LaneSet &L, const double pop[4])
     HIP_TRY(hipMemsetAsync(w.lane_t, 0, sizeof(double) * Mp, L.s));
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, L.s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, L.s));
-    if (w.cond) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), L.s));
+    if (w.cond) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, L.s));
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * w.n_out * 4 * Mp, L.s));
     std::fill(L.obs0.begin(), L.obs0.end(), 0.0);
     for (int x = 0; x < 4; ++x)
@@ -596,18 +619,35 @@ int rebuild(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, bool gram)
 // loss grows like p^-3.2, so the limit keeps it below about 5e-9.
 constexpr double kBlockPivotLimit = 5e-3;
 
+// The condition estimate of a Newton block above which a chunk runs column by
+// column instead.  The estimate m ||L^-1||_F^2 (L the Cholesky factor of the
+// block's Gram matrix scaled to unit diagonal) bounds the scaled Gram
+// matrix's condition number; in a model of the single Newton block (m = 24,
+// anisotropic systems, 0.05-100 mT) ||Q^H Q - I|| stayed within 0.01-0.03 of
+// eps times the estimate (2e-12 at 0.05 mT, estimate 1e6; 7e-8 at 15 mT,
+// 1.4e10), so the limit keeps the loss below about 1e-9.
+constexpr double kNewtonCondLimit = 1e8;
+
+// Whether the conditioning monitor's words (KrylovWork::cond) show a block
+// basis past either limit.
+bool cond_poor(const unsigned long long bits[kCondWords])
+{
+    double inv_p = 0.0, kappa = 0.0;
+    std::memcpy(&inv_p, &bits[0], sizeof(inv_p));
+    std::memcpy(&kappa, &bits[1], sizeof(kappa));
+    return inv_p * kBlockPivotLimit > 1.0 || kappa > kNewtonCondLimit;
+}
+
 // Whether L's block Arnoldi, since the chunk started, built a block with a
 // pivot below kBlockPivotLimit.  Waits for L's stream.
 int block_poor(spc_ctx *ctx, LaneSet &L, bool &poor)
 {
     poor = false;
     if (!L.w.cond || L.w.sstep < 2) return SPC_OK;
-    unsigned long long bits = 0;
-    HIP_TRY(hipMemcpyAsync(&bits, L.w.cond, sizeof(bits), hipMemcpyDeviceToHost, L.s));
+    unsigned long long bits[kCondWords] = {};
+    HIP_TRY(hipMemcpyAsync(bits, L.w.cond, sizeof(bits), hipMemcpyDeviceToHost, L.s));
     HIP_TRY(hipStreamSynchronize(L.s));
-    double inv_p = 0.0;
-    std::memcpy(&inv_p, &bits, sizeof(inv_p));
-    poor = inv_p * kBlockPivotLimit > 1.0;
+    poor = cond_poor(bits);
     return SPC_OK;
 }
 
@@ -1113,6 +1153,7 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     w.m = m;
     w.n_out = nout;
     set_ham(w, ctx->fpat);
+    w.shifts = ham_shifts(ctx, ctx->ham, N); // the orientations share |B|: the spectrum of the set field's H
     w.bval_stride = nb * 16;
     w.chunks_per_bval = M / 64;
     w.prof = nullptr;
@@ -1149,8 +1190,8 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
     if (w.sstep >= 2) { // the pivot monitor, over every chunk of the slot
-        if ((rc = ensure(ctx, nm("cond").c_str(), (size_t)1, &w.cond))) return rc;
-        if (!sl.n) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), sl.s));
+        if ((rc = ensure(ctx, nm("cond").c_str(), (size_t)kCondWords, &w.cond))) return rc;
+        if (!sl.n) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, sl.s));
     }
     if ((rc = ensure(ctx, nm("bvals").c_str(), (size_t)n * nb * 16, &vals))) return rc;
     w.bval = vals;
@@ -1423,14 +1464,12 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
     bool poor = false; // a block basis below the pivot limit (see kBlockPivotLimit)
     for (int j = 0; j < S; ++j) {
         if (!slots[j].w.cond || slots[j].w.sstep < 2) continue;
-        unsigned long long bits = 0;
-        if (hipMemcpy(&bits, slots[j].w.cond, sizeof(bits), hipMemcpyDeviceToHost) != hipSuccess) {
+        unsigned long long bits[kCondWords] = {};
+        if (hipMemcpy(bits, slots[j].w.cond, sizeof(bits), hipMemcpyDeviceToHost) != hipSuccess) {
             cleanup();
             return fail(ctx, SPC_ERR_HIP, "pivot monitor copy");
         }
-        double inv_p = 0.0;
-        std::memcpy(&inv_p, &bits, sizeof(inv_p));
-        poor = poor || inv_p * kBlockPivotLimit > 1.0;
+        poor = poor || cond_poor(bits);
     }
     cleanup();
     std::vector<int32_t> flags((size_t)4 * nchunks), bad((size_t)nchunks);
@@ -2321,6 +2360,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     w.m = m;
     w.n_out = nout;
     set_ham(w, ctx->ham);
+    w.shifts = ham_shifts(ctx, ctx->ham, N);
     w.prof = &ctx->prof;
     ctx->prof.clear();
     w.kS = ctx->kS;
@@ -2339,8 +2379,8 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     if ((rc = ensure(ctx, "part", part_need, &w.part))) return rc;
     w.part_capacity = part_need;
     if (w.sstep >= 2) { // the pivot monitor, shared with the adjoint sweep's basis
-        if ((rc = ensure(ctx, "cond", (size_t)1, &w.cond))) return rc;
-        HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long), ctx->stream));
+        if ((rc = ensure(ctx, "cond", (size_t)kCondWords, &w.cond))) return rc;
+        HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, ctx->stream));
     }
     if ((rc = ensure(ctx, "dots", (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots))) return rc;
     if ((rc = ensure(ctx, "dots1", (size_t)2 * Mp, &w.dots1))) return rc;
@@ -2560,11 +2600,9 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     ctx->last_rebuilds = K + ntop + kstart;
     ctx->last_steps = K;
     if (w.cond && w.sstep >= 2) {
-        unsigned long long bits = 0;
-        HIP_TRY(hipMemcpy(&bits, w.cond, sizeof(bits), hipMemcpyDeviceToHost));
-        double inv_p = 0.0;
-        std::memcpy(&inv_p, &bits, sizeof(inv_p));
-        if (inv_p * kBlockPivotLimit > 1.0 || flags[0] || flags_b[0]) return kRetryColumns;
+        unsigned long long bits[kCondWords] = {};
+        HIP_TRY(hipMemcpy(bits, w.cond, sizeof(bits), hipMemcpyDeviceToHost));
+        if (cond_poor(bits) || flags[0] || flags_b[0]) return kRetryColumns;
     }
     if (flags[0] || flags_b[0]) return fail(ctx, SPC_ERR_BREAKDOWN, "Krylov breakdown in the adjoint sweep");
 

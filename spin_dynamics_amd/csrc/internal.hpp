@@ -76,6 +76,13 @@ hipError_t kron_masks(const KronDev *d_kron, int64_t Z, uint64_t *out, hipStream
 hipError_t h_field_values(const double2 *v0, const double2 *zx, const double2 *zy, const double2 *zz,
                           const double *fields, int64_t count, int n_fields, double2 *out, hipStream_t s);
 
+// ---------------------------------------------------------------- newton.cpp
+// count Newton shifts of H (a host copy of its CSR) in Leja order into out:
+// the Ritz values of a 40-step Lanczos run on (H + H^H)/2, repeated
+// cyclically when there are fewer; 0 on success.
+int newton_shifts_csr(int64_t N, const int32_t *rowptr, const int32_t *col, const double2 *val, int count,
+                      double *out);
+
 // ------------------------------------------------------------- sampling.hip
 struct VecInit {
     int kind;            // 0 SU(Z), 1 phase, 2 injected, 3 basis
@@ -213,21 +220,30 @@ struct KrylovWork {
     Prof *prof;            // optional kernel timing
     int sstep = 0;            // block size of the s-step Arnoldi (0 or 1: one column per pass)
     bool no_dots = false;     // products may skip their u^H v, u^H u partials (tiled product only)
+    double sigma = 0.0;       // products store H u - sigma u (the Newton chain's shift)
     double2 *sbuf = nullptr;  // its dots and forming coefficients, kSStepBuf x Mp
-    // largest 1/p over the lanes and blocks since the last reset, p the
-    // smallest pivot of a block's unit-diagonal Cholesky factor (bits of a
-    // positive double; 0 = none yet): the block basis loses orthogonality as
-    // p falls
+    // the conditioning monitor, two words since the last reset (bits of
+    // positive doubles; 0 = none yet): [0] the largest 1/p, p the smallest
+    // pivot of a monomial block's unit-diagonal Cholesky factor, [1] the
+    // largest condition estimate of a Newton block's scaled Gram matrix; the
+    // block basis loses orthogonality as they grow
     unsigned long long *cond = nullptr;
+    // Newton shifts of H (host, at least m values in Leja order; null: none):
+    // with them a basis that serves one step is built as one block
+    const double *shifts = nullptr;
     // Set by a caller that reads nothing of the basis after psi_update (one
     // step per rebuild): the block Arnoldi then leaves the last block's chain
-    // vectors unformed (fold_K, fold_nf > 0 after arnoldi) and psi_update
-    // folds the forming coefficients into c instead.
+    // vectors unformed (fold_K, fold_nf > 0 after arnoldi: psi's coordinates
+    // over them follow from fold_coef, and slot 0's vector is fold_q0) and
+    // psi_update folds the forming coefficients into c instead.
     bool fold_psi = false;
     int fold_K = 0, fold_nf = 0;
+    const double2 *fold_coef = nullptr, *fold_q0 = nullptr;
 };
+// Words of KrylovWork::cond.
+constexpr int kCondWords = 2;
 // Elements per lane of KrylovWork::sbuf.
-constexpr int kSStepBuf = 720;
+constexpr int kSStepBuf = 1600;
 // Row blocks of the s-step dots pass for dimension N (8 .. 256, N/128 rounded
 // up to a multiple of 8), and the partials it needs for order m, block size S.
 int sstep_row_blocks(int64_t N);

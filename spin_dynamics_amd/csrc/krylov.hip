@@ -30,6 +30,7 @@
 // Hessenberg matrix A (qm_dynamics.f90:432-467) to rounding.
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <array>
 #include <utility>
 
@@ -107,6 +108,14 @@ __device__ __forceinline__ void block_reduce(double2 (&acc)[NACC > 0 ? NACC : 1]
 }
 
 // ------------------------------------------------------- sparse product
+// The stored product row: (H u)_i - sig u_i (sig: the shift of a Newton chain
+// product; every product kernel applies it last, in this form, so they stay
+// bit-identical; sig = 0 stores H u unchanged).
+__device__ __forceinline__ double2 shift_sub(double2 y, double sig, double2 x)
+{
+    return sig == 0.0 ? y : cz(fma(-sig, x.x, y.x), fma(-sig, x.y, y.y));
+}
+
 // v = H u with H in 4x4 blocks over nuclear indices (block row n = rows
 // 4n..4n+3, block column nc = rows 4nc..4nc+3): a thread computes the 4 rows of
 // its nuclear index, loading each neighbouring nuclear index's 4 rows once.
@@ -114,7 +123,8 @@ __device__ __forceinline__ void block_reduce(double2 (&acc)[NACC > 0 ? NACC : 1]
 __global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ browptr, const int32_t *__restrict__ bcol,
                                                   const double2 *__restrict__ bvals, const double2 *__restrict__ u,
                                                   double2 *__restrict__ v, double2 *__restrict__ part, int64_t Z,
-                                                  int Mp, int rpw, int nrb, int64_t bval_stride, int chunks_per_bval)
+                                                  int Mp, int rpw, int nrb, int64_t bval_stride, int chunks_per_bval,
+                                                  double sig)
 {
     __shared__ double2 red[(kWaves - 1) * 8 * kWave];
     int rb, ch;
@@ -163,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const int32_t *__restrict__ bro
         for (int e = 0; e < 4; ++e) {
             const int64_t off = (4 * n + e) * Mp + r;
             const double2 uu = u[off];
+            y[e] = shift_sub(y[e], sig, uu);
             v[off] = y[e];
             acc[0] = cfma_conj(uu, y[e], acc[0]);
             acc[1].x += cabs2(uu);
@@ -277,7 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
                                                        const double2 *__restrict__ gens, int64_t dstride,
                                                        int64_t gstride, int chunks_per_set,
                                                        const double2 *__restrict__ u, double2 *__restrict__ v,
-                                                       double2 *__restrict__ part, int64_t Z, int Mp, int rpw, int nrb)
+                                                       double2 *__restrict__ part, int64_t Z, int Mp, int rpw, int nrb,
+                                                       double sig)
 {
     __shared__ double2 red[(kWaves - 1) * 8 * kWave];
     int rb, ch;
@@ -313,6 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron(const KronDev *__restrict_
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t off = (4 * n + e) * Mp + r;
+            y[e] = shift_sub(y[e], sig, x0[e]);
             v[off] = y[e];
             acc[0] = cfma_conj(x0[e], y[e], acc[0]);
             acc[1].x += cabs2(x0[e]);
@@ -339,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
                                                               int64_t gstride, int chunks_per_set,
                                                               const double2 *__restrict__ u, double2 *__restrict__ v,
                                                               double2 *__restrict__ part, int64_t Z, int Mp, int rpw,
-                                                              int nrb)
+                                                              int nrb, double sig)
 {
     constexpr int W = kWave >> Q;
     constexpr int NG = 1 << Q;
@@ -428,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
         kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+            y[e] = shift_sub(y[e], sig, x0[e]);
             ntstore(&v[rowb + toff + e * Mp], y[e]); // streamed past L2, which keeps u's rows
             acc[0] = cfma_conj(x0[e], y[e], acc[0]);
             acc[1].x += cabs2(x0[e]);
@@ -476,7 +490,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
                                                             int64_t gstride, int chunks_per_set,
                                                             const double2 *__restrict__ u, double2 *__restrict__ v,
                                                             double2 *__restrict__ part, int64_t Z, int Mp, int rpg,
-                                                            int nrb, int lslow)
+                                                            int nrb, int lslow, double sig)
 {
     constexpr int W = kWave >> Q;
     constexpr int NG = 1 << Q;
@@ -586,6 +600,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
+            y[e] = shift_sub(y[e], sig, x0[e]);
             ntstore(&v[rowb + toff + e * Mp], y[e]); // streamed past L2, which keeps u's rows
             if constexpr (DOTS) {
                 acc[0] = cfma_conj(x0[e], y[e], acc[0]);
@@ -953,10 +968,14 @@ __global__ __launch_bounds__(kBlock) void k_update(const double2 *__restrict__ Q
 // populations of the new state: P_S = |psi_ab - psi_ba|^2/2, P_T0 =
 // |psi_ab + psi_ba|^2/2, P_Tp = |psi_aa|^2, P_Tm = |psi_bb|^2 (equal to
 // Re(c^H P_X c) of convert_to_krylov, qm_dynamics.f90:483-496 and 356-359).
+// Vector 0 is read from q0: Q itself, or psi, updated in place (an element of
+// q0 is read only by the thread that writes the same element of psi, before
+// it writes it).
 template <int MK, bool OBS>
-__global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, const double2 *__restrict__ cvec,
-                                                 double2 *__restrict__ psi, double2 *__restrict__ part, int64_t Z,
-                                                 int Mp, int rpw, int nrb, int64_t qs)
+__global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, const double2 *q0,
+                                                 const double2 *__restrict__ cvec, double2 *__restrict__ psi,
+                                                 double2 *__restrict__ part, int64_t Z, int Mp, int rpw, int nrb,
+                                                 int64_t qs)
 {
     __shared__ double2 red[(kWaves - 1) * 8 * kWave];
     int rb, ch;
@@ -976,9 +995,9 @@ __global__ __launch_bounds__(kBlock) void k_psi(const double2 *__restrict__ Q, c
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t off = (4 * n + e) * Mp + r;
-            double2 s = cz(0, 0);
+            double2 s = cfma(q0[off], cc[0], cz(0, 0));
 #pragma unroll
-            for (int k = 0; k < MK; ++k) s = cfma(Q[k * NMp + off], cc[k], s);
+            for (int k = 1; k < MK; ++k) s = cfma(Q[k * NMp + off], cc[k], s);
             p[e] = s;
             psi[off] = s;
         }
@@ -1327,7 +1346,8 @@ __global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, d
 // ---------------------------------------------------------- dispatch tables
 using UpdateFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, const double2 *, const double2 *,
                           double2 *, int64_t, int, int, int, int64_t);
-using PsiFn = void (*)(const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int, int, int64_t);
+using PsiFn = void (*)(const double2 *, const double2 *, const double2 *, double2 *, double2 *, int64_t, int, int,
+                       int, int64_t);
 
 template <size_t... I>
 constexpr auto make_update(std::index_sequence<I...>, std::false_type)
@@ -1367,7 +1387,7 @@ hipError_t reduce(const KrylovWork &w, int nacc, int nrb, double2 *out, hipStrea
 }
 
 using KronFn = void (*)(const KronDev *, const uint64_t *, const double2 *, const double2 *, int64_t, int64_t, int,
-                        const double2 *, double2 *, double2 *, int64_t, int, int, int);
+                        const double2 *, double2 *, double2 *, int64_t, int, int, int, double);
 
 // Lane-slice depth of the Kronecker product: 2^Q nuclear indices per wave on
 // 64 >> Q lanes.  Default 2 where the system allows it (N = 36864, M = 256 on
@@ -1452,7 +1472,8 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
         if (const int nw = kron_tile(w)) {
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave, rpg = tile_rpg(w, nw);
             using TileFn = void (*)(const KronDev *, const uint64_t *, const double2 *, const double2 *, int64_t,
-                                    int64_t, int, const double2 *, double2 *, double2 *, int64_t, int, int, int, int);
+                                    int64_t, int, const double2 *, double2 *, double2 *, int64_t, int, int, int, int,
+                                    double);
             static const TileFn tfn[2][2][3][2] = {
                 {{{k_spmm_kron_tiles<2, 2, 4>, k_spmm_kron_tiles<3, 2, 4>},
                   {k_spmm_kron_tiles<2, 2, 8>, k_spmm_kron_tiles<3, 2, 8>},
@@ -1471,7 +1492,8 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
                                0, s, w.kron,
                                w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
-                               w.Mp, rpg, nrb, tile_order(w, q + (nw == 16 ? 4 : nw == 8 ? 3 : 2), kWave >> q));
+                               w.Mp, rpg, nrb, tile_order(w, q + (nw == 16 ? 4 : nw == 8 ? 3 : 2), kWave >> q),
+                               w.sigma);
             return hipGetLastError();
         }
         if (q > 0) {
@@ -1482,17 +1504,17 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
             const int nrb = spmm_nrb(w), nsl = (w.Mp << q) / kWave;
             hipLaunchKernelGGL(fns[q - 1][group - 2], dim3(nrb * nsl), dim3(kBlock), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
-                               w.Mp, w.rpw1, nrb);
+                               w.Mp, w.rpw1, nrb, w.sigma);
             return hipGetLastError();
         }
         auto fn = group == 2 ? k_spmm_kron<2> : group == 3 ? k_spmm_kron<3> : k_spmm_kron<4>;
         hipLaunchKernelGGL(fn, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.kron, w.kmask, w.kdiag, w.kgen,
                            w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
-                           w.Mp, w.rpw1, nrb1);
+                           w.Mp, w.rpw1, nrb1, w.sigma);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_spmm, dim3(nrb1 * (w.Mp / kWave)), dim3(kBlock), 0, s, w.browptr, w.bcol, w.bval, w.u, w.v,
-                       w.part, w.Z, w.Mp, w.rpw1, nrb1, w.bval_stride, std::max(w.chunks_per_bval, 1));
+                       w.part, w.Z, w.Mp, w.rpw1, nrb1, w.bval_stride, std::max(w.chunks_per_bval, 1), w.sigma);
     return hipGetLastError();
 }
 
@@ -1897,6 +1919,8 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
         if (extra && w.fold_psi) { // psi_update forms psi from [Q_K, Y] directly
             w.fold_K = K;
             w.fold_nf = n;
+            w.fold_coef = scoef;
+            w.fold_q0 = nullptr;
             break;
         }
         {
@@ -1912,6 +1936,230 @@ hipError_t arnoldi_sstep(KrylovWork &w, int S, int nrb, hipStream_t s)
     return hipSuccess;
 }
 
+// ------------------------------------------------- Newton-basis Arnoldi
+// A basis that serves one step, built as one block in the Newton basis (Bai,
+// Hu and Reichel, IMA J. Numer. Anal. 14 (1994) 563; Hoemmen 2010, ch. 7):
+// y_0 = psi, y_{j+1} = (H - sig_j) y_j for j < m with Leja-ordered Ritz
+// values sig_j of H's Hermitian part as shifts, one pass for the Gram matrix
+// of [y_0 .. y_m] (upper triangle, packed), and per lane its Cholesky factor
+// Y = Q R (columns scaled to unit diagonal first), R^-1 and the Hessenberg
+// matrix.  Since span(y_0 .. y_j) is the Krylov space of order j + 1 for any
+// shifts, Q and the Hessenberg matrix are those of the column form up to
+// rounding.  No basis vector is written: psi_update takes psi's coordinates
+// over y_0 .. y_{m-1} from R^-1.  Per step m products, one pass over m + 1
+// vectors and the psi pass over m, against 2m, 49 and 25 + 40 passes of
+// the blocks of 8 at m = 24.
+constexpr int kNTile = 5;        // Gram tile edge of the dots pass (25 complex accumulators per thread)
+constexpr int kNRowBlocks = 64;  // row blocks of the dots pass (at most)
+constexpr int kNPacked = (kMaxKrylov + 1) * (kMaxKrylov + 2) / 2; // packed Gram entries at m = kMaxKrylov
+struct NShifts {
+    double s[kMaxKrylov];
+};
+// Position of G(x, y), x <= y < nd, in the packed upper triangle (row by row).
+__host__ __device__ __forceinline__ int packed_index(int x, int y, int nd) { return x * nd - x * (x - 1) / 2 + y - x; }
+
+// part[rb][packed_index(x, y)][r] = sum over the block's rows of conj(x_x) x_y
+// for the tile pair (a, b), a <= b, of T-column tiles (x in tile a, y in tile
+// b, x <= y).  Workgroup -> (pair, row block, lane chunk), the pairs of one
+// row block consecutive on one XCD (they read the same rows).
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_ngram(ColPtrs x, int nd, double2 *__restrict__ part, int64_t N, int Mp,
+                                                  int64_t rpb, int nrb, int nt)
+{
+    __shared__ double2 red[kWaves - 1][T][kWave];
+    const int npairs = nt * (nt + 1) / 2, npk = nd * (nd + 1) / 2;
+    const int nb = nrb * (Mp / kWave) * npairs;
+    int bid = blockIdx.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int pr = bid % npairs, t = bid / npairs;
+    const int rb = t % nrb, ch = t / nrb;
+    int a = 0, rem = pr;
+    while (rem >= nt - a) {
+        rem -= nt - a;
+        ++a;
+    }
+    const int b = a + rem;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int r = ch * kWave + lane;
+    const int x0 = a * T, y0 = b * T;
+    const double2 *xp[T], *yp[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        xp[i] = x.p[min(x0 + i, nd - 1)];
+        yp[i] = x.p[min(y0 + i, nd - 1)];
+    }
+    double2 acc[T][T];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) acc[i][j] = cz(0, 0);
+    const int64_t row0 = (int64_t)rb * rpb;
+    const int64_t row1 = row0 + rpb < N ? row0 + rpb : N;
+    if (a == b) { // diagonal tile: the upper triangle of one tile's vectors
+        for (int64_t row = row0 + wave; row < row1; row += kWaves) {
+            const int64_t off = row * Mp + r;
+            double2 xv[T];
+#pragma unroll
+            for (int i = 0; i < T; ++i) xv[i] = xp[i][off];
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = i; j < T; ++j) acc[i][j] = cfma_conj(xv[i], xv[j], acc[i][j]);
+        }
+    } else {
+        for (int64_t row = row0 + wave; row < row1; row += kWaves) {
+            const int64_t off = row * Mp + r;
+            double2 xv[T], yv[T];
+#pragma unroll
+            for (int j = 0; j < T; ++j) yv[j] = yp[j][off];
+#pragma unroll
+            for (int i = 0; i < T; ++i) xv[i] = xp[i][off];
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = 0; j < T; ++j) acc[i][j] = cfma_conj(xv[i], yv[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        if (wave > 0)
+#pragma unroll
+            for (int i = 0; i < T; ++i) red[wave - 1][i][lane] = acc[i][j];
+        __syncthreads();
+        if (wave == 0)
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const int xi = x0 + i, yj = y0 + j;
+                if (xi < nd && yj < nd && xi <= yj) {
+                    double2 s = acc[i][j];
+#pragma unroll
+                    for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][i][lane]);
+                    part[((int64_t)rb * npk + packed_index(xi, yj, nd)) * Mp + r] = s;
+                }
+            }
+        __syncthreads();
+    }
+}
+
+// Per lane (one wave each): from the packed Gram matrix G of [y_0 .. y_m]
+// (nd = m + 1), R^H R = G[:m, :m] (Cholesky with the columns scaled to unit
+// diagonal first), coef[x*m + j] = (R^-1)[x][j] (q_j = sum_x coef[x*m + j] y_x),
+// the extra vector's coordinates p = R^-H G[:m, m] = Q^H y_m, the Hessenberg
+// columns h_j = (r_{j+1} + sig_j R_j - sum_{k<j} R[k][j] h_k) / R[j][j]
+// (from H y_j = y_{j+1} + sig_j y_j in the basis; r_{j+1} = R_{j+1}, r_m = p),
+// beta_0 = R_00 = |psi|, and cond[1] = max(cond[1], m ||L^-1||_F^2), L the
+// scaled factor (a bound on the scaled Gram matrix's condition number).
+__global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts sh, const double2 *__restrict__ dots,
+                                                 double2 *__restrict__ coef, double2 *__restrict__ hess,
+                                                 double *__restrict__ beta0, int32_t *__restrict__ flags,
+                                                 unsigned long long *__restrict__ cond)
+{
+    constexpr int MX = kMaxKrylov;
+    __shared__ double2 R[MX * MX], Ri[MX * MX], Hc[MX * MX];
+    __shared__ double2 W[MX], P[MX];
+    __shared__ double sd[MX], colk[MX], sinv;
+    __shared__ int sfail;
+    const int r = blockIdx.x, q = threadIdx.x;
+    const bool active = r < M;
+    const int nd = m + 1;
+    if (q == 0) sfail = active ? 0 : 1;
+    for (int e = q; e < m * nd; e += kWave) { // G rows 0 .. m-1, upper triangle
+        const int a = e / nd, b = e % nd;
+        if (b < a) continue;
+        const double2 g = dots[(int64_t)packed_index(a, b, nd) * Mp + r];
+        if (b < m)
+            R[a * MX + b] = g;
+        else
+            W[a] = g;
+    }
+    __syncthreads();
+    if (q < m) {
+        const double x = R[q * MX + q].x;
+        const bool good = x > 0.0 && isfinite(x);
+        sd[q] = good ? sqrt(x) : 1.0;
+        if (!good) sfail = 1;
+    }
+    __syncthreads();
+    for (int idx = q; idx < m * m; idx += kWave) {
+        const int a = idx / m, b = idx % m;
+        if (b >= a) R[a * MX + b] = cscale(1.0 / (sd[a] * sd[b]), R[a * MX + b]);
+    }
+    __syncthreads();
+    for (int i = 0; i < m; ++i) { // right-looking Cholesky of the unit-diagonal matrix
+        if (q == 0) {
+            double x = R[i * MX + i].x;
+            if (!(x > 0.0) || !isfinite(x)) {
+                sfail = 1;
+                x = 1.0;
+            }
+            const double rii = sqrt(x);
+            R[i * MX + i] = cz(rii, 0.0);
+            sinv = 1.0 / rii;
+        }
+        __syncthreads();
+        if (q > i && q < m) R[i * MX + q] = cscale(sinv, R[i * MX + q]);
+        __syncthreads();
+        for (int idx = q; idx < m * m; idx += kWave) {
+            const int a = idx / m, b = idx % m;
+            if (a > i && b >= a) R[a * MX + b] = csub(R[a * MX + b], cmul(cconj(R[i * MX + a]), R[i * MX + b]));
+        }
+        __syncthreads();
+    }
+    const bool ok = sfail == 0;
+    for (int idx = q; idx < m * m; idx += kWave) { // R = L^H diag(sd)
+        const int a = idx / m, b = idx % m;
+        R[a * MX + b] = (b < a || !ok) ? cz(0, 0) : cscale(sd[b], R[a * MX + b]);
+    }
+    __syncthreads();
+    if (q < m) { // column j of R^-1, the extra vector's coordinate p_j and column j's share of ||L^-1||_F^2
+        const int j = q;
+        if (ok) {
+            Ri[j * MX + j] = cz(1.0 / R[j * MX + j].x, 0.0);
+            for (int i = j - 1; i >= 0; --i) {
+                double2 acc = cz(0, 0);
+                for (int k = i + 1; k <= j; ++k) acc = cfma(R[i * MX + k], Ri[k * MX + j], acc);
+                Ri[i * MX + j] = cscale(-1.0 / R[i * MX + i].x, acc);
+            }
+        }
+        for (int i = ok ? j + 1 : 0; i < m; ++i) Ri[i * MX + j] = cz(0, 0);
+        double2 pj = cz(0, 0);
+        double kc = 0.0;
+        if (ok)
+            for (int a = 0; a <= j; ++a) {
+                pj = cfma_conj(Ri[a * MX + j], W[a], pj);
+                kc += cabs2(cscale(sd[a], Ri[a * MX + j]));
+            }
+        P[j] = pj;
+        colk[j] = kc;
+    }
+    __syncthreads();
+    if (q == 0 && active) {
+        if (!ok) {
+            flags[0] = 1; // Krylov breakdown (the chain vectors are dependent)
+        } else if (cond) {
+            double kap = 0.0;
+            for (int j = 0; j < m; ++j) kap += colk[j];
+            kap *= m;
+            atomicMax(cond + 1, (unsigned long long)__double_as_longlong(kap));
+        }
+    }
+    for (int idx = q; idx < m * m; idx += kWave) coef[(int64_t)idx * Mp + r] = Ri[(idx / m) * MX + idx % m];
+    if (q == 0) beta0[r] = ok ? R[0].x : 0.0;
+    if (q >= m) return;
+    for (int j = 0; j < m; ++j) { // row q of the Hessenberg columns (thread q's own entries of Hc)
+        double2 a = cz(0, 0);
+        if (ok) {
+            a = j + 1 < m ? R[q * MX + j + 1] : P[q];
+            a = cadd(a, cscale(sh.s[j], R[q * MX + j]));
+            for (int k = 0; k < j; ++k) a = csub(a, cmul(Hc[k * MX + q], R[k * MX + j]));
+            a = cscale(1.0 / R[j * MX + j].x, a);
+        }
+        Hc[j * MX + q] = a;
+        hess[((int64_t)j * m + q) * Mp + r] = a;
+    }
+}
+
 // psi's coordinates when the last block (basis columns K .. K+nf-1) was left
 // as its chain vectors y_0 .. y_{nf-1} in those slots: q_{K+j} = sum_x
 // coef[x*nf + j] x_x over x = [q_0 .. q_{K-1}, y_0 .. y_{nf-1}] (k_sform's
@@ -1922,16 +2170,75 @@ __global__ void k_fold(int K, int nf, int Mp, const double2 *__restrict__ c, con
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= Mp) return;
-    double2 cn[kSMaxNY];
-#pragma unroll
-    for (int j = 0; j < kSMaxNY; ++j) cn[j] = j < nf ? c[(int64_t)(K + j) * Mp + r] : cz(0, 0);
     for (int x = 0; x < K + nf; ++x) {
         double2 acc = x < K ? c[(int64_t)x * Mp + r] : cz(0, 0);
-#pragma unroll
-        for (int j = 0; j < kSMaxNY; ++j)
-            if (j < nf) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], cn[j], acc);
+        for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], c[(int64_t)(K + j) * Mp + r], acc);
         d[(int64_t)x * Mp + r] = acc;
     }
+}
+
+// The Newton block (SPC_NEWTON=0 keeps the monomial blocks; read at every
+// rebuild).
+bool newton_enabled()
+{
+    const char *e = std::getenv("SPC_NEWTON");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+// Row blocks of the Newton dots pass: a multiple of 8, at most kNRowBlocks and
+// sstep_row_blocks(N), as many as the partials buffer holds; 0: none fit.
+int newton_nrb(const KrylovWork &w)
+{
+    if (w.m > kMaxKrylov) return 0;
+    const size_t npk = (size_t)(w.m + 1) * (size_t)(w.m + 2) / 2;
+    const size_t fit = w.part_capacity / (npk * (size_t)w.Mp);
+    return (int)std::min<size_t>((size_t)std::min(sstep_row_blocks(w.N), kNRowBlocks), fit & ~(size_t)7);
+}
+
+hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
+{
+    const int m = w.m, nd = m + 1, npk = nd * (nd + 1) / 2;
+    const int64_t qs = qstride(w);
+    double2 *const u0 = w.u, *const v0 = w.v;
+    ColPtrs x{};
+    x.p[0] = u0;
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < m; ++j) { // y_1 .. y_{m-1} in slots 1 .. m-1, y_m in v
+        double2 *const out = j == m - 1 ? v0 : w.Q + (int64_t)(j + 1) * qs;
+        w.u = j == 0 ? u0 : w.Q + (int64_t)j * qs;
+        w.v = out;
+        w.no_dots = true;
+        w.sigma = w.shifts[j];
+        e = spmm_launch(w, s);
+        w.u = u0;
+        w.v = v0;
+        w.no_dots = false;
+        w.sigma = 0.0;
+        if (e != hipSuccess) return e;
+        x.p[j + 1] = out;
+    }
+    const int nt = (nd + kNTile - 1) / kNTile, npairs = nt * (nt + 1) / 2;
+    {
+        ProfScope ps(w, PF_SDOTS, nd * vec_bytes(w), s);
+        hipLaunchKernelGGL(k_ngram<kNTile>, dim3(nrb * (w.Mp / kWave) * npairs), dim3(kBlock), 0, s, x, nd, w.part,
+                           w.N, w.Mp, ceil_div(w.N, nrb), nrb, nt);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = reduce(w, npk, nrb, w.sbuf, s)) != hipSuccess) return e;
+    NShifts sh{};
+    for (int j = 0; j < m; ++j) sh.s[j] = w.shifts[j];
+    double2 *const coef = w.sbuf + (size_t)kNPacked * w.Mp;
+    {
+        ProfScope ps(w, PF_COEFFS, 0.0, s);
+        hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, coef, w.hess, w.beta0,
+                           w.flags, w.cond);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    w.fold_K = 0;
+    w.fold_nf = m;
+    w.fold_coef = coef;
+    w.fold_q0 = u0;
+    return hipSuccess;
 }
 
 } // namespace
@@ -1940,7 +2247,9 @@ int sstep_row_blocks(int64_t N) { return (int)std::clamp<int64_t>((N / 128 + 7) 
 
 size_t sstep_partials(int64_t N, int m, int S, int Mp)
 {
-    return (size_t)sstep_row_blocks(N) * (size_t)(m + 1) * (size_t)std::min(S + 1, kSMaxNY) * (size_t)Mp;
+    const size_t mono = (size_t)sstep_row_blocks(N) * (size_t)(m + 1) * (size_t)std::min(S + 1, kSMaxNY);
+    const size_t newton = (size_t)std::min(sstep_row_blocks(N), kNRowBlocks) * (size_t)((m + 1) * (m + 2) / 2);
+    return std::max(mono, newton) * (size_t)Mp;
 }
 
 int64_t basis_stride(int64_t N, int Mp)
@@ -1964,8 +2273,14 @@ size_t partials_needed(const KrylovWork &w)
 
 hipError_t arnoldi(KrylovWork &w, hipStream_t s)
 {
-    static_assert(2 * kSMaxCols * kSMaxNY == kSStepBuf, "s-step buffer size");
+    static_assert(2 * kSMaxCols * kSMaxNY <= kSStepBuf, "s-step buffer size");
+    static_assert(kNPacked + kMaxKrylov * kMaxKrylov <= kSStepBuf, "Newton buffer size");
     w.fold_K = w.fold_nf = 0;
+    w.fold_coef = w.fold_q0 = nullptr;
+    if (w.fold_psi && w.shifts && w.sstep >= 2 && w.sbuf && w.m >= 2 && newton_enabled()) {
+        const int nrb = newton_nrb(w);
+        if (nrb >= 8) return arnoldi_newton(w, nrb, s);
+    }
     if (w.sstep >= 2 && w.sbuf && w.m >= 8) {
         const int S = std::min(w.sstep, kSMaxNY - 1);
         const int nrb = sstep_nrb(w, S);
@@ -2060,18 +2375,21 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s)
     const PsiFn fn = observe ? kPsiObs[w.m - 1] : kPsiPlain[w.m - 1];
     const double2 *cvec = w.c;
     hipError_t e = hipSuccess;
+    const double2 *q0 = w.Q;
     if (w.fold_nf > 0) { // the last block unformed: psi's coordinates over [Q_K, Y] (the dots area is free)
-        if (w.fold_K + w.fold_nf != w.m || w.fold_nf >= kSMaxNY) return hipErrorInvalidValue;
-        double2 *const d = w.sbuf, *const coef = w.sbuf + (size_t)kSMaxCols * kSMaxNY * w.Mp;
+        if (w.fold_K + w.fold_nf != w.m || !w.fold_coef) return hipErrorInvalidValue;
+        double2 *const d = w.sbuf;
         hipLaunchKernelGGL(k_fold, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.fold_K, w.fold_nf, w.Mp,
-                           w.c, coef, d);
+                           w.c, w.fold_coef, d);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         cvec = d;
+        if (w.fold_q0) q0 = w.fold_q0;
         w.fold_K = w.fold_nf = 0;
+        w.fold_coef = w.fold_q0 = nullptr;
     }
     {
         ProfScope ps(w, PF_PSI, (w.m + 1) * vec_bytes(w), s);
-        hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, cvec, w.psi, w.part, w.Z, w.Mp, w.rpwp,
+        hipLaunchKernelGGL(fn, dim3(nrbp * nch), dim3(kBlock), 0, s, w.Q, q0, cvec, w.psi, w.part, w.Z, w.Mp, w.rpwp,
                            nrbp, qstride(w));
     }
     e = hipGetLastError();

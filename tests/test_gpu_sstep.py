@@ -1,4 +1,5 @@
-"""GPU: the s-step Arnoldi (SPC_SSTEP = block size) against the oracle and the
+"""GPU: the s-step Arnoldi (SPC_SSTEP = block size; SPC_NEWTON=0 keeps the
+monomial blocks where a basis serves one step) against the oracle and the
 one-column-per-pass Arnoldi.
 
 The block form builds the same Krylov space from a chain of products
@@ -56,9 +57,10 @@ def test_block_basis_arnoldi_relation(ctx, monkeypatch, S):
         assert np.all(A[r][np.tril_indices(m, -2)] == 0)  # Hessenberg: zero below the subdiagonal
 
 
-@pytest.mark.parametrize("S", ["0", "6", "8"])  # 0: the one-column-per-pass form
-def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch, S):
+@pytest.mark.parametrize("S,newton", [("0", "1"), ("6", "0"), ("8", "0"), ("8", "1")])  # 0: the one-column-per-pass form
+def test_sstep_first_steps_match_the_oracle(ctx, monkeypatch, S, newton):
     monkeypatch.setenv("SPC_SSTEP", S)
+    monkeypatch.setenv("SPC_NEWTON", newton)
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=1.5)  # two steps
     res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
     s = oracle_system(w.sys, w.B)
@@ -77,6 +79,7 @@ def test_every_block_size_matches_the_column_path(ctx, monkeypatch, S):
     monkeypatch.setenv("SPC_SSTEP", "0")
     col = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     monkeypatch.setenv("SPC_SSTEP", str(S))
+    monkeypatch.setenv("SPC_NEWTON", "0")
     blk = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     for k in KEYS:
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (S, k)
@@ -94,14 +97,16 @@ def test_block_schedules_match_the_column_path(ctx, monkeypatch, m, S):
     monkeypatch.setenv("SPC_SSTEP", "0")
     col = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     monkeypatch.setenv("SPC_SSTEP", str(S))
+    monkeypatch.setenv("SPC_NEWTON", "0")
     blk = dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
     for k in KEYS:
         assert max_rel(getattr(blk, k), getattr(col, k)) < 1e-11, (m, S, k)
 
 
-@pytest.mark.parametrize("S", ["6", "8"])
-def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch, S):
+@pytest.mark.parametrize("S,newton", [("6", "0"), ("8", "0"), ("8", "1")])
+def test_sstep_equal_rates_decay_exactly(ctx, monkeypatch, S, newton):
     monkeypatch.setenv("SPC_SSTEP", S)
+    monkeypatch.setenv("SPC_NEWTON", newton)
     w = workloads.anisotropic_36864(M=256, m=24, T_ns=39.5)  # 40 steps
     k = 1.5 / GAMMA_E
     w.sys.kS = w.sys.kT = k
@@ -142,9 +147,11 @@ def _high_field_case(B, T_ns=300.0, M=8):
     return w, s, vecs
 
 
+@pytest.mark.parametrize("newton", ["0", "1"])
 @pytest.mark.parametrize("B", [0.05, 25.0])
-def test_high_field_runs_column_by_column(ctx, monkeypatch, B):
+def test_high_field_runs_column_by_column(ctx, monkeypatch, B, newton):
     w, s, vecs = _high_field_case(B)
+    monkeypatch.setenv("SPC_NEWTON", newton)
     res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
     monkeypatch.setenv("SPC_SSTEP", "0")
     col = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, trace_vectors=vecs, ctx=ctx)
@@ -152,9 +159,10 @@ def test_high_field_runs_column_by_column(ctx, monkeypatch, B):
     ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=len(vecs), m=w.sim.N_krylov,
                                   t_end=w.sim.t_end, dt=w.sim.dt, vec_kind=2, injected=vecs)
     same = all(np.array_equal(getattr(res, k), getattr(col, k)) for k in KEYS)
-    # 25 mT: the first rebuild's pivot is below the limit, so the chunk starts
-    # again column by column and gives the column form's results bit for bit;
-    # 0.05 mT keeps the block form (within rounding of the column form)
+    # 25 mT: the first rebuild's Newton block is past the condition limit, so
+    # the chunk starts again column by column and gives the column form's
+    # results bit for bit; 0.05 mT keeps the block form (within rounding of
+    # the column form)
     assert same == (B > 1.0)
     for k in KEYS:
         assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
