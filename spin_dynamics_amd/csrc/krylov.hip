@@ -2042,6 +2042,135 @@ __global__ __launch_bounds__(kBlock) void k_ngram(ColPtrs x, int nd, double2 *__
     }
 }
 
+// The same partials with each row of the chain read once per lane: a wave
+// takes L lanes and its 64 / L threads per lane the tile pairs (a, b) of that
+// lane's Gram matrix (T-column tiles, npairs <= 64 / L), so the chain's rows
+// for its lanes are loaded once (the 64 threads load nd x L elements
+// together, coalesced in runs of L lanes) and shared through LDS, with D rows
+// in flight while the previous D are used.  The kWaves waves of a workgroup
+// take every kWaves-th row of one row block for the same L lanes and add
+// their sums through LDS at the end.  Vectors: y_0 = u0, y_v = Q + v qs for
+// 0 < v < nd - 1, y_{nd-1} = ylast.  Tiles on the diagonal compute all T x T
+// products (the lower ones are not stored).
+template <int T, int L, int D>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_ngram_rows(
+    const double2 *__restrict__ u0, const double2 *__restrict__ Q, int64_t qs, const double2 *__restrict__ ylast,
+    int nd, double2 *__restrict__ part, int64_t N, int Mp, int64_t rpb, int nrb, int nt)
+{
+    // largest nd: npairs <= 64 / L tile pairs (25 at L = 4, 33 at L = 2)
+    constexpr int NDX = L == 4 ? 5 * T : kMaxKrylov + 1;
+    constexpr int EPT = (NDX * L + kWave - 1) / kWave; // elements per thread per row
+    __shared__ double2 buf[kWaves][2][D][NDX * L];
+    static_assert(kWaves * 2 * D * NDX * L >= (kWaves - 1) * T * kWave, "the sums reuse the row buffers");
+    double2(*red)[T][kWave] = reinterpret_cast<double2(*)[T][kWave]>(&buf[0][0][0][0]);
+    const int t = threadIdx.x & (kWave - 1);
+    const int wave = wave_uniform(threadIdx.x / kWave);
+    const int ngr = Mp / L;
+    const int nb = nrb * ngr;
+    int bid = blockIdx.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3); // neighbouring lane groups on one XCD
+    const int lg = bid % ngr, rb = bid / ngr;
+    const int r0 = lg * L; // the workgroup's first lane
+    const int ne = nd * L, npk = nd * (nd + 1) / 2;
+    // loading: element e = t + 64k is lane e % L of vector e / L
+    const double2 *lp[EPT];
+    int lc[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = min(t + kWave * k, ne - 1), v = e / L;
+        lp[k] = v == 0 ? u0 : v == nd - 1 ? ylast : Q + (int64_t)v * qs;
+        lc[k] = r0 + e % L;
+    }
+    // computing: thread t takes lane l = t % L and tile pair p = t / L
+    const int l = t % L, p = t / L;
+    int a = 0, rem = min(p, nt * (nt + 1) / 2 - 1);
+    while (rem >= nt - a) {
+        rem -= nt - a;
+        ++a;
+    }
+    const int x0 = a * T, y0 = (a + rem) * T;
+    double2 acc[T][T];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) acc[i][j] = cz(0, 0);
+    const int64_t rbeg = (int64_t)rb * rpb;
+    const int64_t rend = rbeg + rpb < N ? rbeg + rpb : N;
+    // this wave's rows: rbeg + wave + kWaves i, i < nr
+    const int64_t nr = rend - rbeg > wave ? (rend - rbeg - wave + kWaves - 1) / kWaves : 0;
+    double2 pf[D][EPT];
+    auto fetch = [&](int64_t i0) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t row = rbeg + wave + kWaves * min(i0 + d, nr - 1);
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) pf[d][k] = lp[k][row * Mp + lc[k]];
+        }
+    };
+    auto stash = [&](int bb) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < EPT; ++k)
+                if (t + kWave * k < ne) buf[wave][bb][d][t + kWave * k] = pf[d][k];
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    if (nr > 0) {
+        fetch(0);
+        stash(0);
+    }
+    wave_sync();
+    int bb = 0;
+    for (int64_t i0 = 0; i0 < nr; i0 += D, bb ^= 1) {
+        const bool more = i0 + D < nr;
+        if (more) fetch(i0 + D);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (i0 + d < nr) { // wave-uniform
+                const double2 *B = buf[wave][bb][d];
+                double2 yv[T];
+#pragma unroll
+                for (int j = 0; j < T; ++j) yv[j] = B[min(y0 + j, nd - 1) * L + l];
+#pragma unroll
+                for (int i = 0; i < T; ++i) {
+                    const double2 xi = B[min(x0 + i, nd - 1) * L + l];
+#pragma unroll
+                    for (int j = 0; j < T; ++j) acc[i][j] = cfma_conj(xi, yv[j], acc[i][j]);
+                }
+            }
+        }
+        wave_sync(); // every lane has read buffer bb ^ 1's rows of the previous round
+        if (more) stash(bb ^ 1);
+        wave_sync();
+    }
+    const bool writer = p < nt * (nt + 1) / 2;
+    const int r = r0 + l;
+    __syncthreads(); // every wave is done with the row buffers
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        if (wave > 0)
+#pragma unroll
+            for (int i = 0; i < T; ++i) red[wave - 1][i][t] = acc[i][j];
+        __syncthreads();
+        if (wave == 0 && writer)
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const int xi = x0 + i, yj = y0 + j;
+                if (xi < nd && yj < nd && xi <= yj) {
+                    double2 s = acc[i][j];
+#pragma unroll
+                    for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][i][t]);
+                    part[((int64_t)rb * npk + packed_index(xi, yj, nd)) * Mp + r] = s;
+                }
+            }
+        __syncthreads();
+    }
+}
+
 // Per lane (one wave each): from the packed Gram matrix G of [y_0 .. y_m]
 // (nd = m + 1), R^H R = G[:m, :m] (Cholesky with the columns scaled to unit
 // diagonal first), coef[x*m + j] = (R^-1)[x][j] (q_j = sum_x coef[x*m + j] y_x),
@@ -2185,6 +2314,19 @@ bool newton_enabled()
     return !(e && std::strcmp(e, "0") == 0);
 }
 
+// The Newton dots pass: SPC_NGRAM = tiles (a workgroup per tile pair,
+// k_ngram), rows2 or rows3 (default: each row read once per lane, 2 or 3 rows
+// in flight, k_ngram_rows); read at every rebuild.  N = 36 864, 64-lane
+// chunk, m = 24: 483, 313 and 296 us; 5.47, 4.75 and 4.68 ms per step at
+// M = 256.
+int ngram_form()
+{
+    const char *e = std::getenv("SPC_NGRAM");
+    if (e && std::strcmp(e, "tiles") == 0) return 0;
+    if (e && std::strcmp(e, "rows2") == 0) return 2;
+    return 3;
+}
+
 // Row blocks of the Newton dots pass: a multiple of 8, at most kNRowBlocks and
 // sstep_row_blocks(N), as many as the partials buffer holds; 0: none fit.
 int newton_nrb(const KrylovWork &w)
@@ -2220,8 +2362,21 @@ hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
     const int nt = (nd + kNTile - 1) / kNTile, npairs = nt * (nt + 1) / 2;
     {
         ProfScope ps(w, PF_SDOTS, nd * vec_bytes(w), s);
-        hipLaunchKernelGGL(k_ngram<kNTile>, dim3(nrb * (w.Mp / kWave) * npairs), dim3(kBlock), 0, s, x, nd, w.part,
-                           w.N, w.Mp, ceil_div(w.N, nrb), nrb, nt);
+        const int64_t rpb = ceil_div(w.N, nrb);
+        const int form = ngram_form();
+        if (form == 0) {
+            hipLaunchKernelGGL(k_ngram<kNTile>, dim3(nrb * (w.Mp / kWave) * npairs), dim3(kBlock), 0, s, x, nd, w.part,
+                               w.N, w.Mp, rpb, nrb, nt);
+        } else {
+            const int L = npairs <= 16 ? 4 : 2; // tile pairs per lane: 64 / L threads
+            if (npairs > 32 || w.Mp % L) return hipErrorInvalidValue;
+            using RowsFn = void (*)(const double2 *, const double2 *, int64_t, const double2 *, int, double2 *, int64_t,
+                                    int, int64_t, int, int);
+            const RowsFn fn = L == 4 ? (form == 2 ? k_ngram_rows<kNTile, 4, 2> : k_ngram_rows<kNTile, 4, 3>)
+                                     : (form == 2 ? k_ngram_rows<kNTile, 2, 2> : k_ngram_rows<kNTile, 2, 3>);
+            hipLaunchKernelGGL(fn, dim3(nrb * (w.Mp / L)), dim3(kBlock), 0, s, u0, w.Q, qs, v0, nd, w.part, w.N, w.Mp,
+                               rpb, nrb, nt);
+        }
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = reduce(w, npk, nrb, w.sbuf, s)) != hipSuccess) return e;
