@@ -2107,7 +2107,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
     // this wave's rows: rbeg + wave + kWaves i, i < nr
     const int64_t nr = rend - rbeg > wave ? (rend - rbeg - wave + kWaves - 1) / kWaves : 0;
     double2 pf[D][EPT];
+    int64_t fi0 = 0; // first row index of the rows in pf
     auto fetch = [&](int64_t i0) {
+        fi0 = i0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int64_t row = rbeg + wave + kWaves * min(i0 + d, nr - 1);
@@ -2115,12 +2117,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
             for (int k = 0; k < EPT; ++k) pf[d][k] = lp[k][row * Mp + lc[k]];
         }
     };
-    auto stash = [&](int bb) {
+    auto stash = [&](int bb) { // rows past the wave's last are stored as zeros (they add nothing)
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int k = 0; k < EPT; ++k)
-                if (t + kWave * k < ne) buf[wave][bb][d][t + kWave * k] = pf[d][k];
+                if (t + kWave * k < ne) buf[wave][bb][d][t + kWave * k] = fi0 + d < nr ? pf[d][k] : cz(0, 0);
     };
     auto wave_sync = [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2138,17 +2140,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
         if (more) fetch(i0 + D);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            if (i0 + d < nr) { // wave-uniform
-                const double2 *B = buf[wave][bb][d];
-                double2 yv[T];
+            const double2 *B = buf[wave][bb][d];
+            double2 yv[T];
 #pragma unroll
-                for (int j = 0; j < T; ++j) yv[j] = B[min(y0 + j, nd - 1) * L + l];
+            for (int j = 0; j < T; ++j) yv[j] = B[min(y0 + j, nd - 1) * L + l];
 #pragma unroll
-                for (int i = 0; i < T; ++i) {
-                    const double2 xi = B[min(x0 + i, nd - 1) * L + l];
+            for (int i = 0; i < T; ++i) {
+                const double2 xi = B[min(x0 + i, nd - 1) * L + l];
 #pragma unroll
-                    for (int j = 0; j < T; ++j) acc[i][j] = cfma_conj(xi, yv[j], acc[i][j]);
-                }
+                for (int j = 0; j < T; ++j) acc[i][j] = cfma_conj(xi, yv[j], acc[i][j]);
             }
         }
         wave_sync(); // every lane has read buffer bb ^ 1's rows of the previous round
@@ -2323,16 +2323,16 @@ bool newton_enabled()
 }
 
 // The Newton dots pass: SPC_NGRAM = tiles (a workgroup per tile pair,
-// k_ngram), rows2 or rows3 (default: each row read once per lane, 2 or 3 rows
-// in flight, k_ngram_rows); read at every rebuild.  N = 36 864, 64-lane
-// chunk, m = 24: 483, 313 and 296 us; 5.47, 4.75 and 4.68 ms per step at
-// M = 256.
+// k_ngram), rows2 (default) or rows3 (each row read once per lane, 2 or 3
+// rows in flight, k_ngram_rows); read at every rebuild.  N = 36 864, 64-lane
+// chunk, m = 24: 483, 291 and 333 us (rows3 holds 256 VGPRs: one wave per
+// SIMD); 5.47, 4.58 and 4.61 ms per step at M = 256.
 int ngram_form()
 {
     const char *e = std::getenv("SPC_NGRAM");
     if (e && std::strcmp(e, "tiles") == 0) return 0;
-    if (e && std::strcmp(e, "rows2") == 0) return 2;
-    return 3;
+    if (e && std::strcmp(e, "rows3") == 0) return 3;
+    return 2;
 }
 
 // Row blocks of the Newton dots pass: a multiple of 8, at most kNRowBlocks and
