@@ -31,8 +31,10 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // A wave covers 4 device rows (one nuclear index) x 16 lanes; the block's 4
 // waves take nuclear indices base + wave, base + wave + 4, ...  The per-lane
 // coefficients of the 16 lanes sit in LDS ([o][k][lane]), read as broadcasts.
+// Vector 0 is read from q0 (Q itself, or an output updated in place: an
+// element of q0 is read only by the thread that writes it, before it does).
 template <int MK>
-__global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ Q, int64_t qs,
+__global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ Q, const double2 *q0, int64_t qs,
                                                      const double2 *__restrict__ C, CombineOut out,
                                                      const double2 *__restrict__ src, double omega, int64_t Z, int M,
                                                      int Mp, int64_t rows_per_block)
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ 
         for (int o = 0; o < kCombineOut; ++o) acc[o] = cz(0, 0);
 #pragma unroll
         for (int k = 0; k < MK; ++k) {
-            const double2 q = Q[k * NMp + off];
+            const double2 q = k == 0 ? q0[off] : Q[k * NMp + off];
 #pragma unroll
             for (int o = 0; o < kCombineOut; ++o)
                 if (o < nout) acc[o] = cfma(cs[o][k][lr], q, acc[o]);
@@ -166,8 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
     }
 }
 
-using CombineFn = void (*)(const double2 *, int64_t, const double2 *, CombineOut, const double2 *, double, int64_t,
-                           int, int, int64_t);
+using CombineFn = void (*)(const double2 *, const double2 *, int64_t, const double2 *, CombineOut, const double2 *,
+                           double, int64_t, int, int, int64_t);
 
 template <size_t... I>
 constexpr auto make_combine(std::index_sequence<I...>)
@@ -179,15 +181,15 @@ const auto kCombine = make_combine(std::make_index_sequence<kMaxKrylov>{});
 } // namespace
 
 hipError_t combine(const double2 *Q, int64_t qs, int m, const double2 *C, const CombineOut &out, const double2 *src,
-                   double omega, int64_t Z, int M, int Mp, hipStream_t s)
+                   double omega, int64_t Z, int M, int Mp, hipStream_t s, const double2 *q0)
 {
     if (m < 1 || m > kMaxKrylov || out.n < 1 || out.n > kCombineOut || Mp % kGroup) return hipErrorInvalidValue;
     const int ngroups = Mp / kGroup;
     // about 2048 blocks: rows per block a multiple of the 4 waves
     int64_t rpb = std::max<int64_t>(4, ceil_div(Z * ngroups, 2048) / 4 * 4);
     const int64_t nrb = ceil_div(Z, rpb);
-    hipLaunchKernelGGL(kCombine[m - 1], dim3((unsigned)(nrb * ngroups)), dim3(kBlock), 0, s, Q, qs, C, out, src,
-                       omega, Z, M, Mp, rpb);
+    hipLaunchKernelGGL(kCombine[m - 1], dim3((unsigned)(nrb * ngroups)), dim3(kBlock), 0, s, Q, q0 ? q0 : Q, qs, C,
+                       out, src, omega, Z, M, Mp, rpb);
     return hipGetLastError();
 }
 

@@ -2408,7 +2408,9 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     if ((rc = ensure(ctx, "acc", (size_t)4 * nout, &d_acc))) return rc;
     KrylovWork wb = w; // the adjoint Krylov space: H', input lam
     set_ham(wb, ctx->adj);
+    wb.shifts = ham_shifts(ctx, ctx->adj, N);
     wb.psi = wb.u = lam;
+    wb.fold_psi = true; // its bases serve one step each (through combine)
     if (overlap) { // its own basis and scratch
         if ((rc = ensure(ctx, "adjb_Q", (size_t)m * wb.qs, &wb.Q))) return rc;
         if ((rc = ensure(ctx, "adjb_v", (size_t)N * Mp, &wb.v))) return rc;
@@ -2500,7 +2502,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
             HIP_TRY(hipMemcpyAsync(ckpt + (n / L) * NMp, w.psi, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, s));
         if (n == ntop) HIP_TRY(singlet_source(w.psi, cw[ntop] * wscale, lam, Z, Mp, s));
         const bool keep = backward && n >= kstart && n < ntop; // the step's vectors are stored from the basis
-        w.fold_psi = !keep;
+        w.fold_psi = true; // one step per basis
         const hipError_t ea = arnoldi(w, s);
         w.fold_psi = false;
         HIP_TRY(ea);
@@ -2509,7 +2511,9 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
             CombineOut out{};
             out.n = kGL + 1;
             for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
-            HIP_TRY(combine(w.Q, qstride(w), m, Cb, out, nullptr, 0.0, Z, (int)M, Mp, s));
+            const double2 *Cu = nullptr, *q0 = nullptr;
+            HIP_TRY(fold_sets(w, Cb, out.n, &Cu, &q0, s));
+            HIP_TRY(combine(w.Q, qstride(w), m, Cu, out, nullptr, 0.0, Z, (int)M, Mp, s, q0));
         }
         HIP_TRY(expm_lanes(w, dt, s));
         HIP_TRY(step_lanes(w, dt, sim->t_end, 0.0, false, s));
@@ -2546,7 +2550,9 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
         out.n = kGL + 1;
         out.y[0] = lam;
         for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
-        HIP_TRY(combine(wb.Q, qstride(wb), m, Cb, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s));
+        const double2 *Cu = nullptr, *q0 = nullptr;
+        HIP_TRY(fold_sets(wb, Cb, out.n, &Cu, &q0, s));
+        HIP_TRY(combine(wb.Q, qstride(wb), m, Cu, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s, q0));
         HIP_TRY(grad_density(stored(n, 1), bvec, kGL, NMp, ctx->ham.dsys, hs, (int)M, Mp, Sg, s));
         return SPC_OK;
     };
@@ -2556,13 +2562,18 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
         const int64_t n0 = seg * L, n1 = std::min<int64_t>(n0 + L, kstart);
         HIP_TRY(hipMemcpyAsync(w.psi, ckpt + seg * NMp, sizeof(double2) * NMp, hipMemcpyDeviceToDevice, sr));
         for (int64_t n = n0; n < n1; ++n) {
-            HIP_TRY(arnoldi(w, sr));
+            w.fold_psi = true; // one step per basis (psi through combine)
+            const hipError_t ea = arnoldi(w, sr);
+            w.fold_psi = false;
+            HIP_TRY(ea);
             HIP_TRY(expm_coeffs(w, tf, Cf, sr));
             CombineOut out{};
             out.n = kGL + 2;
             for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
             out.y[kGL + 1] = w.psi;
-            HIP_TRY(combine(w.Q, qstride(w), m, Cf, out, nullptr, 0.0, Z, (int)M, Mp, sr));
+            const double2 *Cu = nullptr, *q0 = nullptr;
+            HIP_TRY(fold_sets(w, Cf, out.n, &Cu, &q0, sr));
+            HIP_TRY(combine(w.Q, qstride(w), m, Cu, out, nullptr, 0.0, Z, (int)M, Mp, sr, q0));
         }
         HIP_TRY(hipEventRecord(ev_rec(seg), sr));
         return SPC_OK;

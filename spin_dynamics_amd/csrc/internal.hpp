@@ -274,18 +274,25 @@ struct TauSet {
     double scale[kMaxTaus];
 };
 hipError_t expm_coeffs(KrylovWork &w, const TauSet &ts, double2 *C, hipStream_t s); // hess, beta0 -> C[n][m][Mp]
+// The coefficient sets C[nout][m][Mp] over the basis as coefficients over what
+// arnoldi left in w (the last block's chain vectors, vector 0 in fold_q0):
+// *Cuse (C itself, or their fold in w.sbuf) and *q0 for combine().  Keeps the
+// fold for psi_update.
+hipError_t fold_sets(KrylovWork &w, const double2 *C, int nout, const double2 **Cuse, const double2 **q0,
+                     hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint.hip
 // Up to kCombineOut outputs y_o = sum_k C[o][k] q_k (+ omega P_S src for o = 0)
-// from one read of the basis Q ([m][N][Mp]); outputs with y[o] == nullptr
-// skipped; lanes r >= M written as zeros.
+// from one read of the basis Q ([m][N][Mp]; q_0 from q0 when given, which may
+// be one of the outputs); outputs with y[o] == nullptr skipped; lanes r >= M
+// written as zeros.
 constexpr int kCombineOut = 6;
 struct CombineOut {
     double2 *y[kCombineOut];
     int n;
 };
 hipError_t combine(const double2 *Q, int64_t qs, int m, const double2 *C, const CombineOut &out, const double2 *src,
-                   double omega, int64_t Z, int M, int Mp, hipStream_t s);
+                   double omega, int64_t Z, int M, int Mp, hipStream_t s, const double2 *q0 = nullptr);
 // lam = omega P_S psi (P_S the singlet projector on each electron block).
 hipError_t singlet_source(const double2 *psi, double omega, double2 *lam, int64_t Z, int Mp, hipStream_t s);
 // S[n][slot][e'][e] += sum_j sum_lanes conj(b_j[4n+e']) a_j[4n_s+e] over the

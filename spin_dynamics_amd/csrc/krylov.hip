@@ -2297,6 +2297,22 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
     }
 }
 
+// k_fold for the coefficient sets C[o][m][Mp], o = blockIdx.y: D[o][x] =
+// [x < K] C[o][x] + sum_j coef[x*nf + j] C[o][K+j].
+__global__ void k_fold_sets(int K, int nf, int m, int Mp, const double2 *__restrict__ C,
+                            const double2 *__restrict__ coef, double2 *__restrict__ D)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= Mp) return;
+    const double2 *Co = C + (int64_t)blockIdx.y * m * Mp;
+    double2 *Do = D + (int64_t)blockIdx.y * m * Mp;
+    for (int x = 0; x < K + nf; ++x) {
+        double2 acc = x < K ? Co[(int64_t)x * Mp + r] : cz(0, 0);
+        for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], Co[(int64_t)(K + j) * Mp + r], acc);
+        Do[(int64_t)x * Mp + r] = acc;
+    }
+}
+
 // psi's coordinates when the last block (basis columns K .. K+nf-1) was left
 // as its chain vectors y_0 .. y_{nf-1} in those slots: q_{K+j} = sum_x
 // coef[x*nf + j] x_x over x = [q_0 .. q_{K-1}, y_0 .. y_{nf-1}] (k_sform's
@@ -2480,6 +2496,22 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
 }
 
 hipError_t spmm_only(KrylovWork &w, hipStream_t s) { return spmm_launch(w, s); }
+
+hipError_t fold_sets(KrylovWork &w, const double2 *C, int nout, const double2 **Cuse, const double2 **q0,
+                     hipStream_t s)
+{
+    *Cuse = C;
+    *q0 = w.Q;
+    if (w.fold_nf == 0) return hipSuccess;
+    if (w.fold_K + w.fold_nf != w.m || !w.fold_coef || nout < 1 || (size_t)nout * w.m > (size_t)kNPacked)
+        return hipErrorInvalidValue;
+    double2 *const D = w.sbuf; // the dots area (psi_update's own fold comes after the last use)
+    hipLaunchKernelGGL(k_fold_sets, dim3((unsigned)ceil_div(w.Mp, 64), nout), dim3(64), 0, s, w.fold_K, w.fold_nf, w.m,
+                       w.Mp, C, w.fold_coef, D);
+    *Cuse = D;
+    if (w.fold_q0) *q0 = w.fold_q0;
+    return hipGetLastError();
+}
 
 hipError_t expm_lanes(KrylovWork &w, double dt, hipStream_t s)
 {
