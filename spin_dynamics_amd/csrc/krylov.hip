@@ -2297,20 +2297,18 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
     }
 }
 
-// k_fold for the coefficient sets C[o][m][Mp], o = blockIdx.y: D[o][x] =
-// [x < K] C[o][x] + sum_j coef[x*nf + j] C[o][K+j].
+// k_fold for the coefficient sets C[o][m][Mp], o = blockIdx.z: D[o][x] =
+// [x < K] C[o][x] + sum_j coef[x*nf + j] C[o][K+j].  Grid (lanes / 64, K + nf, sets).
 __global__ void k_fold_sets(int K, int nf, int m, int Mp, const double2 *__restrict__ C,
                             const double2 *__restrict__ coef, double2 *__restrict__ D)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
     if (r >= Mp) return;
-    const double2 *Co = C + (int64_t)blockIdx.y * m * Mp;
-    double2 *Do = D + (int64_t)blockIdx.y * m * Mp;
-    for (int x = 0; x < K + nf; ++x) {
-        double2 acc = x < K ? Co[(int64_t)x * Mp + r] : cz(0, 0);
-        for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], Co[(int64_t)(K + j) * Mp + r], acc);
-        Do[(int64_t)x * Mp + r] = acc;
-    }
+    const double2 *Co = C + (int64_t)blockIdx.z * m * Mp;
+    double2 *Do = D + (int64_t)blockIdx.z * m * Mp;
+    double2 acc = x < K ? Co[(int64_t)x * Mp + r] : cz(0, 0);
+    for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], Co[(int64_t)(K + j) * Mp + r], acc);
+    Do[(int64_t)x * Mp + r] = acc;
 }
 
 // psi's coordinates when the last block (basis columns K .. K+nf-1) was left
@@ -2318,16 +2316,15 @@ __global__ void k_fold_sets(int K, int nf, int m, int Mp, const double2 *__restr
 // coef[x*nf + j] x_x over x = [q_0 .. q_{K-1}, y_0 .. y_{nf-1}] (k_sform's
 // coefficients), so sum_k c_k q_k = sum_x d_x x_x with d_x = [x < K] c_x +
 // sum_j coef[x*nf + j] c_{K+j}.
+// Grid (lanes / 64, K + nf): one thread per lane and coordinate x.
 __global__ void k_fold(int K, int nf, int Mp, const double2 *__restrict__ c, const double2 *__restrict__ coef,
                        double2 *__restrict__ d)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
     if (r >= Mp) return;
-    for (int x = 0; x < K + nf; ++x) {
-        double2 acc = x < K ? c[(int64_t)x * Mp + r] : cz(0, 0);
-        for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], c[(int64_t)(K + j) * Mp + r], acc);
-        d[(int64_t)x * Mp + r] = acc;
-    }
+    double2 acc = x < K ? c[(int64_t)x * Mp + r] : cz(0, 0);
+    for (int j = 0; j < nf; ++j) acc = cfma(coef[((int64_t)x * nf + j) * Mp + r], c[(int64_t)(K + j) * Mp + r], acc);
+    d[(int64_t)x * Mp + r] = acc;
 }
 
 // The Newton block (SPC_NEWTON=0 keeps the monomial blocks; read at every
@@ -2506,8 +2503,8 @@ hipError_t fold_sets(KrylovWork &w, const double2 *C, int nout, const double2 **
     if (w.fold_K + w.fold_nf != w.m || !w.fold_coef || nout < 1 || (size_t)nout * w.m > (size_t)kNPacked)
         return hipErrorInvalidValue;
     double2 *const D = w.sbuf; // the dots area (psi_update's own fold comes after the last use)
-    hipLaunchKernelGGL(k_fold_sets, dim3((unsigned)ceil_div(w.Mp, 64), nout), dim3(64), 0, s, w.fold_K, w.fold_nf, w.m,
-                       w.Mp, C, w.fold_coef, D);
+    hipLaunchKernelGGL(k_fold_sets, dim3((unsigned)ceil_div(w.Mp, 64), w.m, nout), dim3(64), 0, s, w.fold_K, w.fold_nf,
+                       w.m, w.Mp, C, w.fold_coef, D);
     *Cuse = D;
     if (w.fold_q0) *q0 = w.fold_q0;
     return hipGetLastError();
@@ -2574,7 +2571,7 @@ hipError_t psi_update(KrylovWork &w, bool observe, hipStream_t s)
     if (w.fold_nf > 0) { // the last block unformed: psi's coordinates over [Q_K, Y] (the dots area is free)
         if (w.fold_K + w.fold_nf != w.m || !w.fold_coef) return hipErrorInvalidValue;
         double2 *const d = w.sbuf;
-        hipLaunchKernelGGL(k_fold, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.fold_K, w.fold_nf, w.Mp,
+        hipLaunchKernelGGL(k_fold, dim3((unsigned)ceil_div(w.Mp, 64), w.m), dim3(64), 0, s, w.fold_K, w.fold_nf, w.Mp,
                            w.c, w.fold_coef, d);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         cvec = d;
