@@ -107,13 +107,24 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
     const int nn = sys->nn;
     const int nslot = 1 + 2 * nn;
     const int nch = Mp / kWave;
+    // b's rows at n for the first lane chunk, kept over the slots
+    constexpr int JB = 4;
+    double2 b0[JB][4];
+    const bool b0_ok = nch == 1 && J <= JB;
+    if (b0_ok && lane < M) {
+#pragma unroll
+        for (int j = 0; j < JB; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                b0[j][e] = j < J ? b[j * NMp + (4 * n + e) * (int64_t)Mp + lane] : cz(0, 0);
+    }
     for (int slot = 0; slot < nslot; ++slot) {
         int64_t ns = n;
         if (slot > 0) {
             const int k = (slot - 1) >> 1;
             const int64_t st = sys->stride[k];
             const int g = sys->mult[k];
-            const int d = (int)((n / st) % g);
+            const int d = (int)((uint32_t)n / (uint32_t)st) % g; // n < 2^31 (wave-uniform)
             if (slot & 1) {
                 if (d >= g - 1) continue;
                 ns = n + st;
@@ -125,20 +136,36 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
         double2 P[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) P[q] = cz(0, 0);
-        for (int ch = 0; ch < nch; ++ch) {
-            const int r = ch * kWave + lane;
-            if (r >= M) continue;
-            for (int j = 0; j < J; ++j) {
-                double2 bb[4], aa[4];
+        if (b0_ok) {
+            if (lane < M) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    bb[e] = b[j * NMp + (4 * n + e) * (int64_t)Mp + r];
-                    aa[e] = a[j * NMp + (4 * ns + e) * (int64_t)Mp + r];
+                for (int j = 0; j < JB; ++j) {
+                    if (j >= J) break;
+                    double2 aa[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) aa[e] = a[j * NMp + (4 * ns + e) * (int64_t)Mp + lane];
+#pragma unroll
+                    for (int ep = 0; ep < 4; ++ep)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) P[4 * ep + e] = cfma_conj(b0[j][ep], aa[e], P[4 * ep + e]);
                 }
+            }
+        } else {
+            for (int ch = 0; ch < nch; ++ch) {
+                const int r = ch * kWave + lane;
+                if (r >= M) continue;
+                for (int j = 0; j < J; ++j) {
+                    double2 bb[4], aa[4];
 #pragma unroll
-                for (int ep = 0; ep < 4; ++ep)
+                    for (int e = 0; e < 4; ++e) {
+                        bb[e] = b[j * NMp + (4 * n + e) * (int64_t)Mp + r];
+                        aa[e] = a[j * NMp + (4 * ns + e) * (int64_t)Mp + r];
+                    }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) P[4 * ep + e] = cfma_conj(bb[ep], aa[e], P[4 * ep + e]);
+                    for (int ep = 0; ep < 4; ++ep)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) P[4 * ep + e] = cfma_conj(bb[ep], aa[e], P[4 * ep + e]);
+                }
             }
         }
         double *Sd = reinterpret_cast<double *>(S + ((int64_t)n * nslot + slot) * 16);
