@@ -351,6 +351,7 @@ KronDev kron_of(const DevSystem &S)
         k.stride[i] = (int32_t)S.stride[i];
         k.mult[i] = S.mult[i];
         k.elec[i] = i < S.n1 ? 0 : 1;
+        if (i >= S.n1) k.e2mask |= 1ull << i;
     }
     return k;
 }

@@ -33,6 +33,7 @@ struct KronDev {
     int32_t stride[64];
     int32_t mult[64];
     int32_t elec[64]; // 0: electron 1 (spin s1 of e = 2 s1 + s2), 1: electron 2
+    uint64_t e2mask;  // bit k: nucleus k on electron 2 (they follow electron 1's)
 };
 
 // How many of the last nuclei (at most 8) are spin-1/2 with strides 1, 2, 4,

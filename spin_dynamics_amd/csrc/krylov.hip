@@ -245,10 +245,12 @@ __device__ __forceinline__ NbrMasks masks_at(const uint64_t *__restrict__ kmask,
 // branches made the compiler wait for each before issuing the next.  base is
 // the offset of this thread's element of row 4n; rows of index n + s are at
 // base + s * M4.
-template <int G, bool MINUS>
-__device__ __forceinline__ void kron_neighbours(const KronDev *__restrict__ kd, const double2 *__restrict__ gen,
-                                                const double2 *__restrict__ u, int64_t base, int64_t M4, int Mp,
-                                                uint64_t mask, uint64_t one, double2 (&y)[4])
+// EL: the electron of every nucleus in `mask` (a compile-time choice of the
+// 2x2 layout: no branch per flip).
+template <int G, bool MINUS, int EL>
+__device__ __forceinline__ void kron_walk(const KronDev *__restrict__ kd, const double2 *__restrict__ gen,
+                                          const double2 *__restrict__ u, int64_t base, int64_t M4, int Mp,
+                                          uint64_t mask, uint64_t one, double2 (&y)[4])
 {
     while (mask) {
         double2 xn[G][4];
@@ -276,8 +278,26 @@ __device__ __forceinline__ void kron_neighbours(const KronDev *__restrict__ kd, 
             if (ks[j] >= 0) {
                 const int k = ks[j];
                 const int d = ((one >> k) & 1) ? 1 : (MINUS ? 2 : 0);
-                kron_apply(gen + ((int64_t)k * 6 + (MINUS ? 3 : 0) + d) * 4, kd->elec[k], xn[j], y);
+                kron_apply(gen + ((int64_t)k * 6 + (MINUS ? 3 : 0) + d) * 4, EL, xn[j], y);
             }
+    }
+}
+
+// The nuclei of electron 1 precede those of electron 2 (lower k), so walking
+// electron 1's minus neighbours before electron 2's (and electron 2's plus
+// neighbours before electron 1's) keeps the ascending column order.
+template <int G, bool MINUS>
+__device__ __forceinline__ void kron_neighbours(const KronDev *__restrict__ kd, const double2 *__restrict__ gen,
+                                                const double2 *__restrict__ u, int64_t base, int64_t M4, int Mp,
+                                                uint64_t mask, uint64_t one, double2 (&y)[4])
+{
+    const uint64_t m2 = mask & kd->e2mask, m1 = mask & ~kd->e2mask;
+    if (MINUS) {
+        kron_walk<G, true, 0>(kd, gen, u, base, M4, Mp, m1, one, y);
+        kron_walk<G, true, 1>(kd, gen, u, base, M4, Mp, m2, one, y);
+    } else {
+        kron_walk<G, false, 1>(kd, gen, u, base, M4, Mp, m2, one, y);
+        kron_walk<G, false, 0>(kd, gen, u, base, M4, Mp, m1, one, y);
     }
 }
 
