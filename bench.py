@@ -119,6 +119,7 @@ def side_workloads(ctx, a):
                                  f"steps (segments beyond the resident ones recomputed)"}
     # batched orientations: one pair per dimension class, 96 Fibonacci orientations
     steps_b = 8
+    steps_r = 128  # batched rounds: several steps per basis, so a run of many steps
     dirs = batch.fibonacci_sphere(96)
     pairs, seen = [], set()
     for p in batch.radical_pairs():
@@ -126,47 +127,62 @@ def side_workloads(ctx, a):
             seen.add(p.N)
             pairs.append(p)
     classes = {}
-    t_plain = t_batch = 0.0
+    t_plain = t_batch = t_same = 0.0
     counts = {}
     for p in batch.radical_pairs():
         counts[p.N] = counts.get(p.N, 0) + 1
+
+    def plain_per_step(sysp, m, tol, n1, n2):
+        # per orientation-step from the difference of runs of n1 and n2 steps (the
+        # once-per-run vector set-up cancels, as the batched time counts only the
+        # chunks' steps), over the first two orientations
+        tot = 0.0
+        for d in dirs[:2]:
+            s = copy.deepcopy(sysp)
+            s.field_dir = tuple(d)
+            sims = []
+            for n in (n1, n2):
+                sm = batch.orientation_sim(m=m, T_ns=float(n) - 0.5)
+                sm.tol = tol
+                sims.append(sm)
+            dynamics.trace_sampling_para(s, sims[0], B=0.05, ctx=ctx)
+            ms1, _, st1 = dynamics.last_run_stats(ctx)
+            dynamics.trace_sampling_para(s, sims[1], B=0.05, ctx=ctx)
+            ms2, _, st2 = dynamics.last_run_stats(ctx)
+            tot += (ms2 - ms1) / (2 * (st2 - st1))
+        return tot
+
     for p in sorted(pairs, key=lambda q: q.N):
-        sim = batch.orientation_sim(T_ns=float(steps_b) - 0.5)
-        sim2 = batch.orientation_sim(T_ns=float(3 * steps_b) - 0.5)
         warm = copy.deepcopy(p.sys)
         warm.field_dir = tuple(dirs[0])
         dynamics.trace_sampling_para(warm, batch.orientation_sim(T_ns=1.5), B=0.05, ctx=ctx)  # first launches
-        plain = 0.0
-        for d in dirs[:2]:
-            # per-step cost: the difference of runs of 8 and 24 steps (the once-per-run
-            # vector set-up cancels, as the batched time counts only the chunks' steps)
-            s = copy.deepcopy(p.sys)
-            s.field_dir = tuple(d)
-            dynamics.trace_sampling_para(s, sim, B=0.05, ctx=ctx)
-            ms1, _, st1 = dynamics.last_run_stats(ctx)
-            dynamics.trace_sampling_para(s, sim2, B=0.05, ctx=ctx)
-            ms2, _, st2 = dynamics.last_run_stats(ctx)
-            plain += (ms2 - ms1) / (2 * (st2 - st1))
-        # the order is tuned for the full run length (10 us: the pilot's tolerance is
-        # scaled to it), the timing uses a few steps
-        opc, m, _ = batch.tune(p.sys, batch.orientation_sim(), dirs, B=0.05, ctx=ctx)
-        s2 = copy.deepcopy(sim)
-        s2.N_krylov = m
-        batch.orientation_yields(p.sys, s2, dirs, B=0.05, orient_per_chunk=opc, ctx=ctx)
-        ms, _, st = dynamics.last_run_stats(ctx)
+        plain = plain_per_step(p.sys, 24, 0.0, steps_b, 3 * steps_b)
+        # order, rebuild tolerance and width tuned for the 10 us run at 1e-10 yield tolerance
+        opc, m, tol, _ = batch.tune_adaptive(p.sys, batch.orientation_sim(), dirs, B=0.05, ctx=ctx)
+        sb = batch.orientation_sim(m=m, T_ns=float(steps_r) - 0.5)
+        sb.tol = tol
+        batch.orientation_yields(p.sys, sb, dirs, B=0.05, orient_per_chunk=opc, ctx=ctx)
+        ms, rounds, st = dynamics.last_run_stats(ctx)
         per = ms / (96 * st)
+        same = plain_per_step(p.sys, m, tol, steps_r // 2, 3 * steps_r // 2) if tol > 0 else plain
         classes[str(p.N)] = {"plain_ms_per_orientation_step": round(plain, 4),
                              "batched_ms_per_orientation_step": round(per, 4), "speedup": round(plain / per, 2),
-                             "orient_per_chunk": opc, "krylov": m, "pairs": counts[p.N]}
+                             "plain_same_settings_ms_per_orientation_step": round(same, 4),
+                             "speedup_same_settings": round(same / per, 2),
+                             "orient_per_chunk": opc, "krylov": m, "tol": tol, "rounds": rounds,
+                             "pairs": counts[p.N]}
         t_plain += counts[p.N] * plain
         t_batch += counts[p.N] * per
+        t_same += counts[p.N] * same
     out["batched_orientations"] = {
-        "speedup_weighted": round(t_plain / t_batch, 2), "classes": classes,
-        "sample": f"first pair of each dimension class, 96 orientations, M=64, {steps_b} steps; plain = one "
-                  f"spc_trace_sample per orientation (m=24), per step from runs of {steps_b} and {3 * steps_b} "
-                  f"steps; batched = spc_orientation_batch with the order and width spc_batch_tune picks for the "
-                  f"10 us run (1e-10 yield tolerance), device time of the chunks' steps; weighted by the 48 "
-                  f"pairs' classes"}
+        "speedup_weighted": round(t_plain / t_batch, 2), "speedup_same_settings_weighted": round(t_same / t_batch, 2),
+        "classes": classes,
+        "sample": f"first pair of each dimension class, 96 orientations, M=64; plain = one spc_trace_sample per "
+                  f"orientation at m=24, tol=0 (one step per basis), per step from runs of {steps_b} and "
+                  f"{3 * steps_b} steps; batched = spc_orientation_batch over {steps_r} steps with the order, rebuild "
+                  f"tolerance and width spc_batch_tune_adaptive picks for the 10 us run (1e-10 yield tolerance "
+                  f"against m=24, tol=0), device time of the chunks' rounds; plain_same_settings = the plain loop "
+                  f"at the batch's order and tolerance; weighted by the 48 pairs' classes"}
     # semiclassical
     w5 = workloads.semiclassical_48()
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **{**w5.kwargs(), "n_steps_": 10})  # warm
@@ -196,6 +212,8 @@ def main(argv=None):
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the gradient, batched-orientation and semiclassical measurements")
+    ap.add_argument("--adaptive-tol", type=float, default=1e-20,
+                    help="also time the workload at this integrator_tolerance (0: skip)")
     ap.add_argument("--grad-T-ns", type=float, default=5000.0,
                     help="simulated time of the gradient/forward comparison (5000 ns: 5001 steps)")
     a = ap.parse_args(argv)
@@ -211,8 +229,8 @@ def main(argv=None):
     lib = ctx.lib
     N, M, m, dt = H.N, a.M, a.krylov, w.sim.dt
 
-    def run(steps):
-        sim = _lib.SimParams(M, m, steps * dt, dt, 0.0, 0, _lib.VEC_PHASE)
+    def run(steps, tol=0.0):
+        sim = _lib.SimParams(M, m, steps * dt, dt, tol, 0, _lib.VEC_PHASE)
         n_out = int(lib.spc_n_steps(sim.t_end, dt)) + 1
         outs = [np.zeros(n_out) for _ in range(4)]
         seed = np.array(w.sim.seed, dtype=np.int64)
@@ -230,6 +248,24 @@ def main(argv=None):
     chunks = rebuilds // max(taken, 1)  # lane chunks run concurrently (one rebuild each per step)
     value = N * M * taken / wall
     ms_per_step = 1000.0 * wall / taken
+
+    # the same steps with several steps per basis (the reference's rebuild rule at
+    # tol = 1e-20), the same trace vectors: throughput and the populations' distance
+    # from the one-step-per-basis run above
+    adaptive = None
+    if a.adaptive_tol > 0:
+        run(max(a.warmup, 1), a.adaptive_tol)
+        t0 = time.perf_counter()
+        taken_a, outs_a = run(a.steps, a.adaptive_tol)
+        wall_a = time.perf_counter() - t0
+        _, reb_a, _ = dynamics.last_run_stats(ctx)
+        d = max(float(np.max(np.abs(x - y))) / float(np.max(np.abs(y))) for x, y in zip(outs_a, outs))
+        adaptive = {"value": N * M * taken_a / wall_a, "unit": "state·steps/s", "tol": a.adaptive_tol,
+                    "ms_per_step": 1000.0 * wall_a / taken_a, "rebuilds_per_chunk": reb_a / max(chunks, 1),
+                    "steps": int(taken_a), "max_rel_population_difference_vs_tol0": d,
+                    "note": "not the headline: integrator_tolerance > 0 (Newton bases with projector Grams, a "
+                            "basis reused while |c_m|^2 <= tol |c|^2); yield distance from tol = 0 pinned by "
+                            "tests over 100-10 000 steps"}
 
     roofline = None
     families = {}
@@ -289,7 +325,7 @@ def main(argv=None):
                                    f"m={m}, dt=1 ns, tol=0", "N": N, "M": M, "krylov": m, "nnz": H.nnz,
                        "lane_chunks": chunks,
                        "device_ms_per_step": dev_ms / max(taken, 1)},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "adaptive": adaptive}
     if families:
         line["kernel_families"] = families
     line.update(extras)

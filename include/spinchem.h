@@ -217,12 +217,16 @@ int spc_trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
  * reference, which runs one field per call (app/main.f90:48-79).  The system
  * of spc_system_set (its field magnitude B; its direction is replaced) at
  * n_orient directions dirs [n][3], with the same M trace vectors (sample_seeds
- * [M][2], as spc_trace_sample_seeds) for every orientation; tol must be 0 and
- * M a multiple of 64.  Orientations share one block pattern of H and go into
- * the lane dimension, orient_per_chunk at a time (0: as many as fit); up to
- * 4 chunks run at once on their own streams when memory allows
- * (SPC_LANE_STREAMS caps them); with use_graph each chunk's step is captured
- * once in a hipGraph and replayed.  Outputs:
+ * [M][2], as spc_trace_sample_seeds) for every orientation; M a multiple of
+ * 64.  Orientations share one block pattern of H and go into the lane
+ * dimension, orient_per_chunk at a time (0: as many as fit); up to 4 chunks
+ * run at once on their own streams when memory allows (SPC_LANE_STREAMS caps
+ * them); with use_graph each chunk's step is captured once in a hipGraph and
+ * replayed.  tol > 0 applies the reference's rebuild rule
+ * (qm_dynamics.f90:348-364) per lane: a chunk then runs in rounds (a basis
+ * per lane, then each lane's steps until |c_m|^2 > tol |c|^2, populations
+ * from projector Grams), one captured round replayed per round, with a host
+ * check of the lane clocks between rounds.  Outputs:
  * Phi_S [n_orient] final singlet yields; P_S [n_orient][n_out] or NULL. */
 int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
                           const double *dirs, int32_t orient_per_chunk, int32_t use_graph, double *Phi_S, double *P_S,
@@ -235,10 +239,24 @@ int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
  * requested order's, scaled to the pilot's share of the run, and the
  * orientations per chunk (power of two) with the least device time per
  * orientation-step over a short timed run, with as many chunks in flight as
- * spc_orientation_batch runs. */
+ * spc_orientation_batch runs.  Every run takes one step per basis (sim->tol
+ * is not used). */
 int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
                    const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
                    double *ms_per_orient_step);
+
+/* Autotuner for spc_orientation_batch with several steps per basis (no
+ * counterpart in the reference): over the orders 16, 20, .., 32, the loosest
+ * tolerance of 1e-12, 1e-16, .., 1e-28 (the reference's rebuild rule,
+ * qm_dynamics.f90:348-364) whose pilot yields (256 steps, up to 4
+ * orientations) stay within a tenth of yield_tol of the requested order at
+ * tol = 0, scaled to the pilot's share of the run; of those the (order, tol) with the
+ * least pilot time per orientation-step, or the requested order at tol = 0
+ * (*tol = 0) if that is faster or none passes; then the orientations per
+ * chunk as spc_batch_tune. */
+int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                            const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
+                            double *tol, double *ms_per_orient_step);
 
 /* ------------------------------------------------------- semiclassical */
 /* Semiclassical spin system (isotropic couplings, as src/sc uses a_iso). */

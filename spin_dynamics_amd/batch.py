@@ -76,6 +76,25 @@ def tune(sys: SysParam, sim: SimParam, dirs: np.ndarray, seed=None, *, B: Option
     return int(opc[0]), int(m[0]), float(ms[0])
 
 
+def tune_adaptive(sys: SysParam, sim: SimParam, dirs: np.ndarray, seed=None, *, B: Optional[float] = None,
+                  yield_tol: float = 1e-10, ctx: Optional[_lib.Context] = None):
+    """(orientations per chunk, Krylov order, tol, ms per orientation-step) chosen by
+    spc_batch_tune_adaptive: several steps per basis under the reference's
+    rebuild rule, at yields within yield_tol of sim.N_krylov with one step
+    per basis."""
+    ctx, simc, sd, keep = _prep(sys, sim, B, seed, ctx)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    opc = np.zeros(1, np.int32)
+    m = np.zeros(1, np.int32)
+    tol = np.zeros(1)
+    ms = np.zeros(1)
+    ctx.check(ctx.lib.spc_batch_tune_adaptive(ctx.h, simc, _lib.ptr(sd, _lib._i64p), dirs.shape[0], _lib.ptr(dirs),
+                                              float(yield_tol), _lib.ptr(opc, _lib._i32p), _lib.ptr(m, _lib._i32p),
+                                              _lib.ptr(tol), _lib.ptr(ms)))
+    del keep
+    return int(opc[0]), int(m[0]), float(tol[0]), float(ms[0])
+
+
 # ------------------------------------------------------------ pair sets
 DIMS = (768, 3072, 12288, 49152, 147456)
 _NUCLEI = {768: (1, 6), 3072: (1, 8), 12288: (1, 10), 49152: (1, 12), 147456: (2, 12)}  # (14N, 1H)

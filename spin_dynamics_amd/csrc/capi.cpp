@@ -454,6 +454,9 @@ const double *ham_shifts(spc_ctx *ctx, HamDev &h, int64_t N)
 }
 
 // Workspace, stream and host staging of one lane chunk of propagate().
+// Lane chunks of one propagate call that run at once, each on its own stream.
+constexpr int kMaxLaneSets = 4;
+
 struct LaneSet {
     KrylovWork w{};
     hipStream_t s = nullptr;
@@ -540,6 +543,9 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("E").c_str(), (size_t)m * m * Mp, &w.E))) return rc;
     if ((rc = ensure(ctx, nm("c").c_str(), (size_t)m * Mp, &w.c))) return rc;
     if (need_gram && (rc = ensure(ctx, nm("gram").c_str(), (size_t)4 * m * m * Mp, &w.gram))) return rc;
+    if (need_gram && w.sstep >= 2 &&
+        (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
+        return rc;
     if ((rc = ensure(ctx, nm("beta0").c_str(), (size_t)Mp, &w.beta0))) return rc;
     if ((rc = ensure(ctx, nm("lane_t").c_str(), (size_t)Mp, &w.lane_t))) return rc;
     if ((rc = ensure(ctx, nm("lane_k").c_str(), (size_t)Mp, &w.lane_k))) return rc;
@@ -652,11 +658,9 @@ int block_poor(spc_ctx *ctx, LaneSet &L, bool &poor)
     return SPC_OK;
 }
 
-// The projector-Gram pass of L's chunk (several steps per basis), run to the
-// end with a host check of the lane clocks after every rebuild.  Its Gram
-// matrices and their partial sums are allocated here when the chunk was set
-// up for direct mode (a tol = 0 lane that needed a second step).
-int gram_pass(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, const double pop[4], int64_t K, int64_t &rebuilds)
+// Projector Grams and their bins for L's chunk, allocated here when the
+// chunk was set up for direct mode (a tol = 0 lane that needed a second step).
+int gram_buffers(spc_ctx *ctx, LaneSet &L)
 {
     KrylovWork &w = L.w;
     const std::string sfx = L.idx ? "#" + std::to_string(L.idx) : "";
@@ -667,26 +671,61 @@ int gram_pass(spc_ctx *ctx, LaneSet &L, const spc_sim_params *sim, const double 
         w.part_capacity = part_need;
     }
     if (!w.gram && (rc = ensure(ctx, ("gram" + sfx).c_str(), (size_t)4 * w.m * w.m * w.Mp, &w.gram))) return rc;
-    if ((rc = start_pass(ctx, L, pop))) return rc;
-    for (int64_t round = 0;; ++round) {
-        if ((rc = rebuild(ctx, L, sim, true))) return rc;
-        ++rebuilds;
-        if (round == 0) { // a poorly conditioned block basis: the chunk again, column by column
-            bool poor = false;
-            if ((rc = block_poor(ctx, L, poor))) return rc;
-            if (poor) {
-                w.sstep = 0;
-                if ((rc = start_pass(ctx, L, pop))) return rc;
-                round = -1;
-                continue;
-            }
-        }
-        HIP_TRY(hipMemcpyAsync(L.lt.data(), L.w.lane_t, sizeof(double) * L.w.Mp, hipMemcpyDeviceToHost, L.s));
-        HIP_TRY(hipStreamSynchronize(L.s));
-        bool done = true;
-        for (int r = 0; r < L.w.M; ++r) done = done && !(L.lt[r] < sim->t_end);
-        if (done) break;
+    if (w.sstep < 2) {
+        w.gbins = nullptr; // column by column: the Grams from the formed basis
+    } else if (!w.gbins) {
+        if ((rc = ensure(ctx, ("gbins" + sfx).c_str(), (size_t)2 * (w.m + 1) * (w.m + 2) * w.Mp, &w.gbins)))
+            return rc;
+    }
+    return SPC_OK;
+}
+
+// The projector-Gram passes of the chunks sets[j] for j in js (several steps
+// per basis), their rounds interleaved over the chunks' streams.  A round
+// rebuilds every lane of a chunk from its current psi and steps each lane
+// until its tol rule asks for a new basis (qm_dynamics.f90:348-364); the host
+// reads the lane clocks after each round, and a chunk is done when every lane
+// has reached t_end.  A chunk whose first block basis is poorly conditioned
+// starts again column by column.
+int gram_rounds(spc_ctx *ctx, LaneSet *sets, const std::vector<int> &js, const spc_sim_params *sim,
+                const double pop[4], int64_t K, int64_t &rebuilds)
+{
+    int rc;
+    std::vector<int> active = js;
+    std::vector<char> first(kMaxLaneSets, 0);
+    for (int j : js) {
+        if ((rc = gram_buffers(ctx, sets[j]))) return rc;
+        if ((rc = start_pass(ctx, sets[j], pop))) return rc;
+        first[j] = 1;
+    }
+    for (int64_t round = 0; !active.empty(); ++round) {
         if (round > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
+        for (int j : active) { // every chunk's round in flight before the host waits for any
+            if ((rc = rebuild(ctx, sets[j], sim, true))) return rc;
+            ++rebuilds;
+        }
+        std::vector<int> next;
+        for (int j : active) {
+            LaneSet &L = sets[j];
+            if (first[j]) { // a poorly conditioned block basis: the chunk again, column by column
+                first[j] = 0;
+                bool poor = false;
+                if ((rc = block_poor(ctx, L, poor))) return rc;
+                if (poor) {
+                    L.w.sstep = 0;
+                    L.w.gbins = nullptr;
+                    if ((rc = start_pass(ctx, L, pop))) return rc;
+                    next.push_back(j);
+                    continue;
+                }
+            }
+            HIP_TRY(hipMemcpyAsync(L.lt.data(), L.w.lane_t, sizeof(double) * L.w.Mp, hipMemcpyDeviceToHost, L.s));
+            HIP_TRY(hipStreamSynchronize(L.s));
+            bool done = true;
+            for (int r = 0; r < L.w.M; ++r) done = done && !(L.lt[r] < sim->t_end);
+            if (!done) next.push_back(j);
+        }
+        active.swap(next);
     }
     return SPC_OK;
 }
@@ -704,7 +743,6 @@ int read_flags(spc_ctx *ctx, LaneSet &L, int32_t flags[4])
 // lanes): 4 by default, one per hardware queue of the process (N = 36864,
 // M = 256 on MI355X: 14.7 ms per step with 4 chunks, 15.5 with 2, 16.9 with
 // one); SPC_LANE_STREAMS sets 1 (one chunk at a time) to kMaxLaneSets.
-constexpr int kMaxLaneSets = 4;
 int lane_streams()
 {
     const char *e = std::getenv("SPC_LANE_STREAMS");
@@ -756,7 +794,7 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
     if (max_lanes < 64) return fail(ctx, SPC_ERR_OOM, "not enough device memory for one 64-vector chunk");
     int nsets = 1;
     const bool arena = ctx->arena != nullptr; // scratch call: one chunk, no streams or events created
-    if (!need_gram && !arena) nsets = (int)std::min<int64_t>({lane_streams(), (M_total + 63) / 64, max_lanes / 64});
+    if (!arena) nsets = (int)std::min<int64_t>({lane_streams(), (M_total + 63) / 64, max_lanes / 64});
     const int64_t Mchunk =
         std::min<int64_t>(max_lanes / (64 * nsets) * 64, (M_total + 64 * nsets - 1) / (64 * nsets) * 64);
     const int Mp = (int)((Mchunk + 63) / 64 * 64);
@@ -815,13 +853,15 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
                 return rc;
         }
         if (need_gram) {
+            std::vector<int> js;
+            for (int j = 0; j < used; ++j) js.push_back(j);
+            if ((rc = gram_rounds(ctx, sets, js, sim, pop, K, rebuilds))) return rc;
             for (int j = 0; j < used; ++j) {
-                if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
                 int32_t flags[4];
                 if ((rc = read_flags(ctx, sets[j], flags))) return rc;
                 if (flags[0] && sets[j].w.sstep >= 2) { // the block form broke down: column by column
                     sets[j].w.sstep = 0;
-                    if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
+                    if ((rc = gram_rounds(ctx, sets, {j}, sim, pop, K, rebuilds))) return rc;
                     if ((rc = read_flags(ctx, sets[j], flags))) return rc;
                 }
                 breakdown = breakdown || flags[0];
@@ -868,7 +908,7 @@ int propagate(spc_ctx *ctx, const spc_sim_params *sim, VecInit vi, int64_t M_tot
                 breakdown = breakdown || flags[0];
                 if (!flags[1]) continue;
                 // a lane needed two steps per rebuild: redo the chunk with projector Grams
-                if ((rc = gram_pass(ctx, sets[j], sim, pop, K, rebuilds))) return rc;
+                if ((rc = gram_rounds(ctx, sets, {j}, sim, pop, K, rebuilds))) return rc;
                 if ((rc = read_flags(ctx, sets[j], flags))) return rc;
                 breakdown = breakdown || flags[0];
             }
@@ -1103,6 +1143,7 @@ struct BatchPlan {
     int slots = 1;     // chunks in flight at once, each on its own stream
     bool kron = true;  // per-orientation Kronecker tables (else the block product)
     bool columns = false; // one column per pass (after the block form broke down or lost orthogonality)
+    double tol = 0.0;     // > 0: several steps per basis (rounds with projector Grams)
 };
 
 // One stream of the batched run: its workspace and the step graph of its
@@ -1173,7 +1214,8 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     w.u = w.psi;
     if ((rc = ensure(ctx, nm("v").c_str(), (size_t)N * Mp, &w.v))) return rc;
     w.sstep = plan.columns ? 0 : sstep_block(m);
-    const size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
+    size_t part_need = std::max(partials_needed(w), w.sstep >= 2 ? sstep_partials(N, m, w.sstep, Mp) : 0);
+    if (plan.tol > 0.0) part_need = std::max(part_need, (size_t)4 * m * m * Mp * 8);
     if ((rc = ensure(ctx, nm("part").c_str(), part_need, &w.part))) return rc;
     w.part_capacity = part_need;
     if ((rc = ensure(ctx, nm("dots").c_str(), (size_t)(4 * m * m + 2 * SPC_MAX_KRYLOV + 4) * Mp, &w.dots)))
@@ -1190,6 +1232,12 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
+    if (plan.tol > 0.0) { // projector Grams (and their bins on the Newton chain) for several steps per basis
+        if ((rc = ensure(ctx, nm("gram").c_str(), (size_t)4 * m * m * Mp, &w.gram))) return rc;
+        if (w.sstep >= 2 &&
+            (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
+            return rc;
+    }
     if (w.sstep >= 2) { // the pivot monitor, over every chunk of the slot
         if ((rc = ensure(ctx, nm("cond").c_str(), (size_t)kCondWords, &w.cond))) return rc;
         if (!sl.n) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, sl.s));
@@ -1214,17 +1262,16 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     return SPC_OK;
 }
 
-// Enqueues chunk c (orientations [o0, o0 + n)) on the slot's stream for K
-// steps: the values of each orientation's H on the block pattern and its
-// Kronecker tables, the trace vectors, the steps between the events a and b,
-// then each orientation's summed populations added to acc[o][4 * n_out]
-// and the chunk's flags saved.  Nothing here waits for the device.
-int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, const BatchPlan &plan,
-                             const BatchShared &sh, int M, int c, int o0, int n, int64_t K, int64_t nout,
-                             double *d_acc, hipEvent_t a, hipEvent_t b)
+// Enqueues the start of chunk c (orientations [o0, o0 + n)) on the slot's
+// stream: the values of each orientation's H on the block pattern and its
+// Kronecker tables, the trace vectors, lane clocks and populations, then the
+// event a.  Nothing here waits for the device (a new width drains the slot's
+// earlier chunks before its buffers and graph change).
+int begin_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, const BatchPlan &plan,
+                            const BatchShared &sh, int M, int c, int o0, int n, int64_t nout, hipEvent_t a)
 {
     int rc;
-    if (sl.n != n) { // a new width: the slot's earlier chunks drain before its buffers and graph change
+    if (sl.n != n) {
         if (sl.n) HIP_TRY(hipStreamSynchronize(sl.s));
         if ((rc = slot_setup(ctx, sl, plan, M, n, nout))) return rc;
     }
@@ -1245,7 +1292,8 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
             !holds("bvals", (size_t)n * nb * 16 * sizeof(double2)) ||
             !holds("obs", (size_t)nout * 4 * Mp * sizeof(double)) ||
             !holds("psi", (size_t)ctx->N * Mp * sizeof(double2)) ||
-            !holds("Q", (size_t)m * w.qs * sizeof(double2)) || !holds("part", w.part_capacity * sizeof(double2)))
+            !holds("Q", (size_t)m * w.qs * sizeof(double2)) || !holds("part", w.part_capacity * sizeof(double2)) ||
+            (plan.tol > 0.0 && !holds("gram", (size_t)4 * m * m * Mp * sizeof(double2))))
             return fail(ctx, SPC_ERR_INVALID, "batched orientations: a chunk does not fit its slot's workspace");
     }
     HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], sh.fields + 3 * (size_t)o0, nb * 16, n,
@@ -1266,22 +1314,35 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
     HIP_TRY(hipMemcpy2DAsync(w.obs, sizeof(double) * Mp, sh.obs0, sizeof(double) * plan.opc * M,
                              sizeof(double) * Mp, 4, hipMemcpyDeviceToDevice, s));
-    auto one_step = [&]() -> hipError_t {
+    HIP_TRY(hipEventRecord(a, s));
+    return SPC_OK;
+}
+
+// Enqueues `count` steps (tol = 0) or rounds (tol > 0: a basis from each
+// lane's psi, then each lane's steps until its tol rule asks for a new one)
+// of the slot's chunk, replayed from the slot's graph of one step or round
+// when plan.graph (captured at the first use of this width).
+int enqueue_steps(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, const BatchPlan &plan, int64_t count)
+{
+    KrylovWork &w = sl.w;
+    hipStream_t s = sl.s;
+    const bool rounds = plan.tol > 0.0;
+    auto one = [&]() -> hipError_t {
         hipError_t e;
-        w.fold_psi = true; // one step per basis
+        w.fold_psi = !rounds; // one step per basis: nothing reads the basis after the psi pass
         e = arnoldi(w, s);
         w.fold_psi = false;
         if (e != hipSuccess) return e;
         if ((e = expm_lanes(w, sim->dt, s)) != hipSuccess) return e;
-        if ((e = step_lanes(w, sim->dt, sim->t_end, 0.0, false, s)) != hipSuccess) return e;
-        return psi_update(w, true, s);
+        if (rounds && (e = gram_projectors(w, s)) != hipSuccess) return e;
+        if ((e = step_lanes(w, sim->dt, sim->t_end, rounds ? plan.tol : 0.0, rounds, s)) != hipSuccess) return e;
+        return psi_update(w, !rounds, s);
     };
-    HIP_TRY(hipEventRecord(a, s));
-    if (plan.graph && K > 1) {
+    if (plan.graph && (count > 1 || rounds)) {
         if (sl.capture) {
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipError_t e = one_step();
+            hipError_t e = one();
             hipError_t e2 = hipStreamEndCapture(s, &g);
             sl.graphs.emplace_back(g, nullptr);
             if (e != hipSuccess) return hip_fail(ctx, e, "step capture");
@@ -1290,32 +1351,125 @@ int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params 
             sl.capture = false;
         }
         hipGraphExec_t ge = sl.graphs.back().second;
-        for (int64_t k = 0; k < K; ++k) HIP_TRY(hipGraphLaunch(ge, s));
+        for (int64_t k = 0; k < count; ++k) HIP_TRY(hipGraphLaunch(ge, s));
     } else {
-        for (int64_t k = 0; k < K; ++k) HIP_TRY(one_step());
+        for (int64_t k = 0; k < count; ++k) HIP_TRY(one());
     }
+    return SPC_OK;
+}
+
+// Enqueues the end of chunk c: the event b, each orientation's summed
+// populations added to acc[o][4 * n_out], and the chunk's flags saved.
+int end_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const BatchShared &sh, int M, int c, int o0, int n,
+                          int64_t nout, double *d_acc, hipEvent_t b)
+{
+    KrylovWork &w = sl.w;
+    hipStream_t s = sl.s;
     HIP_TRY(hipEventRecord(b, s));
-    HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, Mp, d_acc + (size_t)o0 * 4 * nout, s));
+    HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, w.Mp, d_acc + (size_t)o0 * 4 * nout, s));
     HIP_TRY(hipMemcpyAsync(sh.flags + 4 * (size_t)c, w.flags, sizeof(int32_t) * 4, hipMemcpyDeviceToDevice, s));
+    return SPC_OK;
+}
+
+// Chunk c of a tol = 0 batch for K steps (one step per basis), enqueued
+// between the events a and b without waiting for the device.
+int launch_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *sim, const BatchPlan &plan,
+                             const BatchShared &sh, int M, int c, int o0, int n, int64_t K, int64_t nout,
+                             double *d_acc, hipEvent_t a, hipEvent_t b)
+{
+    int rc;
+    if ((rc = begin_orientation_chunk(ctx, sl, sim, plan, sh, M, c, o0, n, nout, a))) return rc;
+    if ((rc = enqueue_steps(ctx, sl, sim, plan, K))) return rc;
+    return end_orientation_chunk(ctx, sl, sh, M, c, o0, n, nout, d_acc, b);
+}
+
+// The chunks of a tol > 0 batch in rounds: each slot's chunk takes a round,
+// then the host reads its lane clocks (pinned buffer, one copy per round,
+// every slot's round in flight before the host waits for any) and starts the
+// slot's next chunk once every lane has reached t_end.  *rounds counts the
+// rounds of all chunks.
+int run_round_chunks(spc_ctx *ctx, OrientSlot *slots, int S, const spc_sim_params *sim, const BatchPlan &plan,
+                     const BatchShared &sh, int M, int n_orient, int64_t K, int64_t nout, double *d_acc,
+                     std::vector<hipEvent_t> &evs, int64_t *rounds)
+{
+    const int opc = plan.opc, nchunks = (n_orient + opc - 1) / opc;
+    struct Host {
+        double *lt = nullptr;
+        ~Host()
+        {
+            if (lt) (void)hipHostFree(lt);
+        }
+    } host[kMaxLaneSets];
+    int chunk[kMaxLaneSets], o0s[kMaxLaneSets], ns[kMaxLaneSets];
+    int64_t done_rounds[kMaxLaneSets] = {};
+    int rc, next = 0;
+    for (int j = 0; j < S; ++j) {
+        chunk[j] = -1;
+        HIP_TRY(hipHostMalloc((void **)&host[j].lt, sizeof(double) * (size_t)opc * M, hipHostMallocDefault));
+    }
+    auto start = [&](int j) -> int {
+        const int c = next++;
+        chunk[j] = c;
+        o0s[j] = c * opc;
+        ns[j] = std::min(opc, n_orient - c * opc);
+        done_rounds[j] = 0;
+        return begin_orientation_chunk(ctx, slots[j], sim, plan, sh, M, c, o0s[j], ns[j], nout, evs[1 + 2 * c]);
+    };
+    for (int j = 0; j < S && next < nchunks; ++j)
+        if ((rc = start(j))) return rc;
+    for (;;) {
+        bool any = false;
+        for (int j = 0; j < S; ++j) {
+            if (chunk[j] < 0) continue;
+            any = true;
+            OrientSlot &sl = slots[j];
+            if ((rc = enqueue_steps(ctx, sl, sim, plan, 1))) return rc;
+            HIP_TRY(hipMemcpyAsync(host[j].lt, sl.w.lane_t, sizeof(double) * sl.w.Mp, hipMemcpyDeviceToHost, sl.s));
+            ++*rounds;
+        }
+        if (!any) break;
+        for (int j = 0; j < S; ++j) {
+            if (chunk[j] < 0) continue;
+            OrientSlot &sl = slots[j];
+            HIP_TRY(hipStreamSynchronize(sl.s));
+            bool done = true;
+            for (int r = 0; r < sl.w.Mp; ++r) done = done && !(host[j].lt[r] < sim->t_end);
+            if (++done_rounds[j] > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
+            if (!done) continue;
+            const int c = chunk[j];
+            if ((rc = end_orientation_chunk(ctx, sl, sh, M, c, o0s[j], ns[j], nout, d_acc, evs[2 + 2 * c])))
+                return rc;
+            chunk[j] = -1;
+            if (next < nchunks && (rc = start(j))) return rc;
+        }
+    }
     return SPC_OK;
 }
 
 // Orientations per chunk that fit 70% of the free memory (power of two); the
 // workspace of earlier runs counts as free.
-int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout)
+int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout, bool grams = false)
 {
+    // the lane workspace of earlier runs (every slot's) is released first: a
+    // slot that does not reallocate a buffer would otherwise keep it beside
+    // the new ones
+    for (const std::string sfx : {"", "#1", "#2", "#3"})
+        for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part", "gram", "gbins", "bkdiag"})
+            release_buf(ctx, (nm + sfx).c_str());
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
-    for (const std::string sfx : {"", "#1", "#2", "#3"})
-        for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part"}) {
-            auto it = ctx->bufs.find(nm + sfx);
-            if (it != ctx->bufs.end()) free_b += it->second.bytes;
-        }
-    const size_t per_orient = ((size_t)m + 3) * ctx->N * M * sizeof(double2) + (size_t)nout * 4 * M * sizeof(double) +
-                              (size_t)ctx->fpat.nblocks * 16 * sizeof(double2);
+    size_t per_orient = ((size_t)m + 3) * ctx->N * M * sizeof(double2) + (size_t)nout * 4 * M * sizeof(double) +
+                        (size_t)ctx->fpat.nblocks * 16 * sizeof(double2);
+    if (grams) // projector Grams, their bins and the partials of eight row blocks (tol > 0)
+        per_orient += ((size_t)4 * m * m * 9 + (size_t)2 * (m + 1) * (m + 2)) * M * sizeof(double2);
+    // at most 64 GiB per chunk: one allocation of the basis near the whole
+    // device failed (147 456-dimensional pair, 32 orientations: 154 GB), and
+    // wider chunks of large pairs buy nothing (one orientation's product
+    // already fills the device)
+    const size_t budget = std::min<size_t>(free_b / 10 * 7, (size_t)64 << 30);
     int best = 1;
     for (int c = 1; c <= n_orient; c *= 2)
-        if ((size_t)c * per_orient <= free_b / 10 * 7) best = c;
+        if ((size_t)c * per_orient <= budget) best = c;
     return best;
 }
 
@@ -1356,8 +1510,12 @@ double union_ms(std::vector<std::pair<double, double>> iv)
 // so the result does not depend on the slots.  total_ms: device time during
 // which some chunk's steps run.
 int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
-                       const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms)
+                       const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms,
+                       int64_t *rounds_out = nullptr)
 {
+    int64_t rounds_local = 0;
+    int64_t *const rounds = rounds_out ? rounds_out : &rounds_local;
+    *rounds = 0;
     const int64_t n_steps = spc_n_steps(sim->t_end, sim->dt);
     const int64_t K = steps_taken(sim->t_end, sim->dt);
     const int64_t nout = n_steps + 1;
@@ -1435,13 +1593,22 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
     hipEvent_t ev0 = evs[0];
     HIP_TRY(hipEventRecord(ev0, s));
     for (int j = 1; j < S; ++j) HIP_TRY(hipStreamWaitEvent(slots[j].s, ev0, 0));
-    for (int c = 0; c < nchunks; ++c) {
-        const int o0 = c * opc;
-        if ((rc = launch_orientation_chunk(ctx, slots[c % S], sim, plan, sh, M, c, o0, std::min(opc, n_orient - o0),
-                                           K, nout, d_acc, evs[1 + 2 * c], evs[2 + 2 * c]))) {
+    if (plan.tol > 0.0) {
+        if ((rc = run_round_chunks(ctx, slots, S, sim, plan, sh, M, n_orient, K, nout, d_acc, evs, rounds))) {
             cleanup();
             return rc;
         }
+    } else {
+        for (int c = 0; c < nchunks; ++c) {
+            const int o0 = c * opc;
+            if ((rc = launch_orientation_chunk(ctx, slots[c % S], sim, plan, sh, M, c, o0,
+                                               std::min(opc, n_orient - o0), K, nout, d_acc, evs[1 + 2 * c],
+                                               evs[2 + 2 * c]))) {
+                cleanup();
+                return rc;
+            }
+        }
+        *rounds = K * nchunks;
     }
     hipError_t es = hipSuccess;
     for (int j = 0; j < S; ++j) {
@@ -1480,14 +1647,14 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         // an orientation's H did not match its Kronecker tables: the block product
         BatchPlan p2 = plan;
         p2.kron = false;
-        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms);
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out);
     }
     bool broke = false;
     for (int c = 0; c < nchunks; ++c) broke = broke || flags[4 * c] != 0;
     if ((poor || broke) && !plan.columns) { // the whole batch again, column by column
         BatchPlan p2 = plan;
         p2.columns = true;
-        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms);
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out);
     }
     for (int c = 0; c < nchunks; ++c) {
         if (flags[4 * c + 3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
@@ -1518,7 +1685,6 @@ int check_batch_args(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sam
     if (!ctx->have_sys) return fail(ctx, SPC_ERR_INVALID, "no spin system set (spc_system_set)");
     int rc = check_sim(ctx, sim);
     if (rc) return rc;
-    if (sim->tol != 0.0) return fail(ctx, SPC_ERR_UNSUPPORTED, "batched orientations need integrator_tolerance = 0");
     if (sim->vec_kind != SPC_VEC_SUZ && sim->vec_kind != SPC_VEC_PHASE)
         return fail(ctx, SPC_ERR_INVALID, "vec_kind must be SPC_VEC_SUZ or SPC_VEC_PHASE");
     if (sim->n_samples < 64 || sim->n_samples % 64)
@@ -2683,15 +2849,17 @@ int spc_orientation_batch(spc_ctx *ctx, const spc_sim_params *sim, const int64_t
     BatchPlan plan;
     plan.m = sim->n_krylov;
     plan.graph = use_graph != 0;
-    const int cap = max_orient_per_chunk(ctx, plan.m, sim->n_samples, n_orient, nout);
+    plan.tol = sim->tol;
+    const int cap = max_orient_per_chunk(ctx, plan.m, sim->n_samples, n_orient, nout, plan.tol > 0.0);
     plan.opc = orient_per_chunk > 0 ? std::min<int>(orient_per_chunk, n_orient) : cap;
     plan.slots = orient_slots(plan.opc, cap, n_orient);
     std::vector<double> ps;
     double ms = 0.0;
-    if ((rc = batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, plan, ps, &ms))) return rc;
+    int64_t rounds = 0;
+    if ((rc = batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, plan, ps, &ms, &rounds))) return rc;
     ctx->last_ms = ms;
     ctx->last_steps = steps_taken(sim->t_end, sim->dt);
-    ctx->last_rebuilds = ctx->last_steps * ((n_orient + plan.opc - 1) / plan.opc);
+    ctx->last_rebuilds = rounds;
     for (int o = 0; o < n_orient; ++o) {
         if (Phi_S) Phi_S[o] = final_yield(ps.data() + (size_t)o * nout, nout, sim->dt, ctx->kS);
         if (P_S) std::copy(ps.begin() + (size_t)o * nout, ps.begin() + (size_t)(o + 1) * nout, P_S + (size_t)o * nout);
@@ -2714,6 +2882,7 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
     const int64_t pilot = std::min<int64_t>(K, 64);
     spc_sim_params ps = *sim;
     ps.t_end = sim->dt * (double)pilot - 0.5 * sim->dt;
+    ps.tol = 0.0; // one step per basis (spc_batch_tune_adaptive also tunes the tolerance)
     const int npilot = std::min<int>(n_orient, 4);
     const int64_t pout = spc_n_steps(ps.t_end, ps.dt) + 1;
     BatchPlan plan;
@@ -2744,6 +2913,7 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
     // each timed with as many chunks in flight as spc_orientation_batch runs
     spc_sim_params ts = *sim;
     ts.n_krylov = m_best;
+    ts.tol = 0.0;
     const int64_t tsteps = std::min<int64_t>(K, 6);
     ts.t_end = sim->dt * (double)tsteps - 0.5 * sim->dt;
     const int64_t tout = spc_n_steps(ts.t_end, ts.dt) + 1;
@@ -2769,6 +2939,105 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
     }
     if (orient_per_chunk) *orient_per_chunk = best_c;
     if (n_krylov) *n_krylov = m_best;
+    if (ms_per_orient_step) *ms_per_orient_step = best_t;
+    return SPC_OK;
+}
+
+int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int32_t n_orient,
+                            const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
+                            double *tol, double *ms_per_orient_step)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_batch_args(ctx, sim, sample_seeds, n_orient, dirs);
+    if (rc) return rc;
+    const int64_t K = steps_taken(sim->t_end, sim->dt);
+    // pilot: up to 256 steps of up to 4 orientations, one chunk; the reference
+    // is the requested order with one step per basis (tol = 0)
+    const int64_t pilot = std::min<int64_t>(K, 256);
+    spc_sim_params ps = *sim;
+    ps.t_end = sim->dt * (double)pilot - 0.5 * sim->dt;
+    ps.tol = 0.0;
+    const int npilot = std::min<int>(n_orient, 4);
+    const int64_t pout = spc_n_steps(ps.t_end, ps.dt) + 1;
+    BatchPlan plan;
+    plan.opc = npilot;
+    plan.m = sim->n_krylov;
+    std::vector<double> ref, cur;
+    if ((rc = batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, ref, nullptr))) return rc;
+    double ref_ms = 0.0;
+    if ((rc = batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, &ref_ms))) return rc;
+    // the scaled tolerance with a margin of 10: the pilot's error grows faster
+    // than linearly in some runs (12 288-dimensional pair, m = 32, tol = 1e-16:
+    // 4e-11 after 1000 steps from a pilot within 2.6e-12 at 256)
+    const double ytol =
+        0.1 * yield_tol * (double)std::max<int64_t>(pilot, 1) / (double)std::max<int64_t>(K, 1);
+    auto worst_err = [&]() {
+        double worst = 0.0;
+        for (int o = 0; o < npilot; ++o) {
+            const double a = final_yield(ref.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
+            const double b = final_yield(cur.data() + (size_t)o * pout, pout, ps.dt, ctx->kS);
+            worst = std::max(worst, std::fabs(a - b) / std::max(std::fabs(a), 1e-300));
+        }
+        return worst;
+    };
+    // (order, tolerance): for each order the loosest tolerance of the ladder
+    // whose pilot yields stay within the scaled yield tolerance (a tighter one
+    // only rebuilds more often), then the pair with the least pilot time;
+    // one step per basis at the requested order if none passes
+    int m_best = sim->n_krylov;
+    double tol_best = 0.0, per_best = ref_ms / ((double)npilot * (double)std::max<int64_t>(pilot, 1));
+    const double ladder[] = {1e-12, 1e-16, 1e-20, 1e-24, 1e-28};
+    for (int m = 16; m <= SPC_MAX_KRYLOV; m += 4) {
+        for (double t : ladder) {
+            plan.m = m;
+            plan.tol = t;
+            ps.n_krylov = m;
+            ps.tol = t;
+            double ms = 0.0;
+            if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, &ms)) break;
+            if (worst_err() > ytol) continue;
+            const double per = ms / ((double)npilot * (double)std::max<int64_t>(pilot, 1));
+            if (per < per_best) {
+                per_best = per;
+                m_best = m;
+                tol_best = t;
+            }
+            break;
+        }
+    }
+    ctx->err.clear();
+    // lane width: as spc_batch_tune, each width timed over the pilot length
+    spc_sim_params ts = *sim;
+    ts.n_krylov = m_best;
+    ts.tol = tol_best;
+    const int64_t tsteps = tol_best > 0.0 ? pilot : std::min<int64_t>(K, 6);
+    ts.t_end = sim->dt * (double)tsteps - 0.5 * sim->dt;
+    const int64_t tout = spc_n_steps(ts.t_end, ts.dt) + 1;
+    const int cap = max_orient_per_chunk(ctx, m_best, sim->n_samples, n_orient, tout, tol_best > 0.0);
+    int best_c = 1;
+    double best_t = 1e300;
+    for (int c = 1; c <= cap; c *= 2) {
+        BatchPlan tp;
+        tp.m = m_best;
+        tp.tol = tol_best;
+        tp.opc = c;
+        tp.graph = true;
+        tp.slots = orient_slots(c, cap, n_orient);
+        const int no = std::min(n_orient, tp.slots * c);
+        std::vector<double> tmp;
+        double ms = 0.0;
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc; // warm
+        if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc;
+        const double per = ms / ((double)no * (double)std::max<int64_t>(tsteps, 1));
+        if (per < best_t * 0.97) {
+            best_t = per;
+            best_c = c;
+        }
+    }
+    if (orient_per_chunk) *orient_per_chunk = best_c;
+    if (n_krylov) *n_krylov = m_best;
+    if (tol) *tol = tol_best;
     if (ms_per_orient_step) *ms_per_orient_step = best_t;
     return SPC_OK;
 }

@@ -238,6 +238,11 @@ struct KrylovWork {
     // over them follow from fold_coef, and slot 0's vector is fold_q0) and
     // psi_update folds the forming coefficients into c instead.
     bool fold_psi = false;
+    // Packed projector bins [4][(m+1)(m+2)/2][Mp] (null: none).  With them
+    // (tol > 0) a Newton block also makes the projector Grams from its chain
+    // (gram_ready until gram_projectors consumes them).
+    double2 *gbins = nullptr;
+    bool gram_ready = false;
     int fold_K = 0, fold_nf = 0;
     const double2 *fold_coef = nullptr, *fold_q0 = nullptr;
 };

@@ -1244,93 +1244,126 @@ size_t expm_lds_bytes(int m) { return 4 * (size_t)m * m * sizeof(double2); }
 
 namespace {
 
+// Position of G(x, y), x <= y < nd, in the packed upper triangle (row by row).
+__host__ __device__ __forceinline__ int packed_index(int x, int y, int nd) { return x * nd - x * (x - 1) / 2 + y - x; }
+
 // --------------------------------------------------------------- step loop
-// qm_dynamics.f90:348-364 for each lane: c <- E c while |c_m|^2 <= tol |c|^2
-// and t < t_end, the step that exceeds tol kept.  With gram != 0 the
-// populations of every step are Re(c^H G_X c); otherwise they come from the
-// psi update and a lane must take exactly one step (flag 1 otherwise).
-__global__ void k_step(int m, int M, int Mp, double dt, double t_end, double tol, int gram,
-                       const double2 *__restrict__ E, const double *__restrict__ beta0, double2 *__restrict__ cout,
-                       double *__restrict__ lane_t, int32_t *__restrict__ lane_k, double *__restrict__ obs,
-                       int64_t n_out, const double2 *__restrict__ G, int32_t *__restrict__ flags)
+// qm_dynamics.f90:348-364 for each lane when a basis serves one step (the
+// populations then come from the psi update): one step from c = beta e_1,
+// i.e. c = beta E(:,0) (the general loop's sum adds only E(i,j) * 0 beyond
+// j = 0: the same bits), with the loop's bookkeeping; a lane whose tol rule
+// asks for a second step raises flag 1.
+__global__ void k_step(int m, int M, int Mp, double dt, double t_end, double tol, const double2 *__restrict__ E,
+                       const double *__restrict__ beta0, double2 *__restrict__ cout, double *__restrict__ lane_t,
+                       int32_t *__restrict__ lane_k, int64_t n_out, int32_t *__restrict__ flags)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= M) return;
     double t = lane_t[r];
     int64_t k = lane_k[r];
-    if (!gram) {
-        // direct mode: one step from c = beta e_1, i.e. c = beta E(:,0) (the
-        // general loop's sum adds only E(i,j) * 0 beyond j = 0: the same bits),
-        // with the loop's bookkeeping; a second step would flag the lane
-        const double b0 = beta0[r];
-        if (t < t_end) {
-            if (k + 1 >= n_out) {
-                for (int i = 0; i < m; ++i) cout[(int64_t)i * Mp + r] = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
-                flags[3] = 1;
-                lane_k[r] = (int32_t)(k + 1);
-                return;
-            }
-            double c2 = 0.0, cm2 = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double2 ci = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
-                cout[(int64_t)i * Mp + r] = ci;
-                c2 += cabs2(ci);
-                if (i == m - 1) cm2 = cabs2(ci);
-            }
-            k = k + 1;
-            t = t + dt;
-            if (cm2 <= tol * c2 && t < t_end) flags[1] = 1;
-        } else {
-            for (int j = 0; j < m; ++j) cout[(int64_t)j * Mp + r] = j == 0 ? cz(b0, 0.0) : cz(0, 0);
-        }
-        lane_t[r] = t;
-        lane_k[r] = (int32_t)k;
-        return;
-    }
-    double2 c[kMaxKrylov], cn[kMaxKrylov];
-    for (int j = 0; j < m; ++j) c[j] = cz(0, 0);
-    c[0] = cz(beta0[r], 0.0);
+    const double b0 = beta0[r];
     if (t < t_end) {
-        double c2 = 1.0, cm2 = 0.0;
-        int steps = 0;
-        while (cm2 <= tol * c2 && t < t_end) {
-            if (!gram && steps == 1) {
-                flags[1] = 1;
-                break;
-            }
-            for (int i = 0; i < m; ++i) {
-                double2 s = cz(0, 0);
-                for (int j = 0; j < m; ++j) s = cfma(E[((int64_t)j * m + i) * Mp + r], c[j], s);
-                cn[i] = s;
-            }
-            for (int i = 0; i < m; ++i) c[i] = cn[i];
-            k = k + 1;
-            if (k >= n_out) {
-                flags[3] = 1;
-                break;
-            }
-            if (gram) {
-                for (int x = 0; x < 4; ++x) {
-                    const double2 *Gx = G + (int64_t)x * m * m * Mp;
-                    double acc = 0.0;
-                    for (int i = 0; i < m; ++i) {
-                        double2 s = cz(0, 0);
-                        for (int j = 0; j < m; ++j) s = cfma(Gx[((int64_t)i * m + j) * Mp + r], c[j], s);
-                        acc += c[i].x * s.x + c[i].y * s.y; // Re(conj(c_i) s)
-                    }
-                    obs[(k * 4 + x) * Mp + r] = acc;
-                }
-            }
-            c2 = 0.0;
-            for (int i = 0; i < m; ++i) c2 += cabs2(c[i]);
-            cm2 = cabs2(c[m - 1]);
-            t = t + dt;
-            ++steps;
+        if (k + 1 >= n_out) {
+            for (int i = 0; i < m; ++i) cout[(int64_t)i * Mp + r] = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
+            flags[3] = 1;
+            lane_k[r] = (int32_t)(k + 1);
+            return;
         }
+        double c2 = 0.0, cm2 = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double2 ci = cfma(E[(int64_t)i * Mp + r], cz(b0, 0.0), cz(0, 0));
+            cout[(int64_t)i * Mp + r] = ci;
+            c2 += cabs2(ci);
+            if (i == m - 1) cm2 = cabs2(ci);
+        }
+        k = k + 1;
+        t = t + dt;
+        if (cm2 <= tol * c2 && t < t_end) flags[1] = 1;
+    } else {
+        for (int j = 0; j < m; ++j) cout[(int64_t)j * Mp + r] = j == 0 ? cz(b0, 0.0) : cz(0, 0);
     }
-    for (int j = 0; j < m; ++j) cout[(int64_t)j * Mp + r] = c[j];
     lane_t[r] = t;
     lane_k[r] = (int32_t)k;
+}
+
+// The same loop with projector Grams (tol > 0: several steps per basis), one
+// workgroup per lane: E (E(i,j) at E[(j*m + i)*Mp + r]) and the upper
+// triangles of the four Hermitian G_X (G_X(i,j) at G[((X*m + i)*m + j)*Mp + r])
+// staged in LDS; thread i < m takes row i of c <- E c and of G_X c, and one
+// thread sums over i in k_step's order (the populations, |c|^2).
+__global__ __launch_bounds__(kWave) void k_step_gram(int m, int M, int Mp, double dt, double t_end, double tol,
+                                                     const double2 *__restrict__ E, const double *__restrict__ beta0,
+                                                     double2 *__restrict__ cout, double *__restrict__ lane_t,
+                                                     int32_t *__restrict__ lane_k, double *__restrict__ obs,
+                                                     int64_t n_out, const double2 *__restrict__ G,
+                                                     int32_t *__restrict__ flags)
+{
+    constexpr int MX = kMaxKrylov, NP = MX * (MX + 1) / 2;
+    __shared__ double2 Es[MX * MX], Gp[4][NP], cs[2][MX];
+    __shared__ double pv[4][MX];
+    __shared__ double sc2, scm2;
+    const int r = blockIdx.x, q = threadIdx.x;
+    if (r >= M) return; // uniform over the workgroup
+    for (int idx = q; idx < m * m; idx += kWave) Es[idx] = E[(int64_t)idx * Mp + r];
+    for (int x = 0; x < 4; ++x)
+        for (int idx = q; idx < m * m; idx += kWave) {
+            const int i = idx / m, j = idx % m;
+            if (j >= i) Gp[x][packed_index(i, j, m)] = G[((int64_t)x * m * m + idx) * Mp + r];
+        }
+    const double b0 = beta0[r];
+    if (q < m) cs[0][q] = q == 0 ? cz(b0, 0.0) : cz(0, 0);
+    double t = lane_t[r];
+    int64_t k = lane_k[r];
+    if (q == 0) {
+        sc2 = 1.0;
+        scm2 = 0.0;
+    }
+    __syncthreads();
+    int cur = 0;
+    while (scm2 <= tol * sc2 && t < t_end) {
+        if (q < m) {
+            double2 a = cz(0, 0);
+            for (int j = 0; j < m; ++j) a = cfma(Es[j * m + q], cs[cur][j], a);
+            cs[cur ^ 1][q] = a;
+        }
+        cur ^= 1;
+        k = k + 1;
+        if (k >= n_out) {
+            if (q == 0) flags[3] = 1;
+            break;
+        }
+        __syncthreads();
+        if (q < m) {
+            const double2 ci = cs[cur][q];
+            for (int x = 0; x < 4; ++x) {
+                double2 a = cz(0, 0);
+                for (int j = 0; j < m; ++j) {
+                    const double2 g = j >= q ? Gp[x][packed_index(q, j, m)] : cconj(Gp[x][packed_index(j, q, m)]);
+                    a = cfma(g, cs[cur][j], a);
+                }
+                pv[x][q] = ci.x * a.x + ci.y * a.y; // Re(conj(c_i) (G_X c)_i)
+            }
+        }
+        __syncthreads();
+        if (q == 0) {
+            for (int x = 0; x < 4; ++x) {
+                double acc = 0.0;
+                for (int i = 0; i < m; ++i) acc += pv[x][i];
+                obs[(k * 4 + x) * Mp + r] = acc;
+            }
+            double c2 = 0.0;
+            for (int i = 0; i < m; ++i) c2 += cabs2(cs[cur][i]);
+            sc2 = c2;
+            scm2 = cabs2(cs[cur][m - 1]);
+        }
+        t = t + dt;
+        __syncthreads();
+    }
+    if (q < m) cout[(int64_t)q * Mp + r] = cs[cur][q];
+    if (q == 0) {
+        lane_t[r] = t;
+        lane_k[r] = (int32_t)k;
+    }
 }
 
 // ----------------------------------------------------- projector Grams
@@ -1983,8 +2016,6 @@ constexpr int kNPacked = (kMaxKrylov + 1) * (kMaxKrylov + 2) / 2; // packed Gram
 struct NShifts {
     double s[kMaxKrylov];
 };
-// Position of G(x, y), x <= y < nd, in the packed upper triangle (row by row).
-__host__ __device__ __forceinline__ int packed_index(int x, int y, int nd) { return x * nd - x * (x - 1) / 2 + y - x; }
 
 // part[rb][packed_index(x, y)][r] = sum over the block's rows of conj(x_x) x_y
 // for the tile pair (a, b), a <= b, of T-column tiles (x in tile a, y in tile
@@ -2080,10 +2111,17 @@ __global__ __launch_bounds__(kBlock) void k_ngram(ColPtrs x, int nd, double2 *__
 // their sums through LDS at the end.  Vectors: y_0 = u0, y_v = Q + v qs for
 // 0 < v < nd - 1, y_{nd-1} = ylast.  Tiles on the diagonal compute all T x T
 // products (the lower ones are not stored).
-template <int T, int L, int D>
+// BIN = kBinAll sums over all N rows.  BIN = 0 .. 3 sums over the Z nuclear
+// indices n the electron-state components of the projector P_X: 0 singlet
+// (y[4n+1] - y[4n+2])/sqrt2, 1 T0 (y[4n+1] + y[4n+2])/sqrt2, 2 T+ y[4n],
+// 3 T- y[4n+3] (N is then Z), so the four bins add up to the plain Gram
+// matrix and bin X is Y^H P_X Y.  Partial sums go to
+// part[(rb * pks + pko + packed) * Mp + r].
+constexpr int kBinAll = 4;
+template <int T, int L, int D, int BIN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_ngram_rows(
     const double2 *__restrict__ u0, const double2 *__restrict__ Q, int64_t qs, const double2 *__restrict__ ylast,
-    int nd, double2 *__restrict__ part, int64_t N, int Mp, int64_t rpb, int nrb, int nt)
+    int nd, double2 *__restrict__ part, int64_t N, int Mp, int64_t rpb, int nrb, int nt, int pks, int pko)
 {
     // largest nd: npairs <= 64 / L tile pairs (25 at L = 4, 33 at L = 2)
     constexpr int NDX = L == 4 ? 5 * T : kMaxKrylov + 1;
@@ -2099,7 +2137,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3); // neighbouring lane groups on one XCD
     const int lg = bid % ngr, rb = bid / ngr;
     const int r0 = lg * L; // the workgroup's first lane
-    const int ne = nd * L, npk = nd * (nd + 1) / 2;
+    const int ne = nd * L;
     // loading: element e = t + 64k is lane e % L of vector e / L
     const double2 *lp[EPT];
     int lc[EPT];
@@ -2126,7 +2164,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
     const int64_t rend = rbeg + rpb < N ? rbeg + rpb : N;
     // this wave's rows: rbeg + wave + kWaves i, i < nr
     const int64_t nr = rend - rbeg > wave ? (rend - rbeg - wave + kWaves - 1) / kWaves : 0;
-    double2 pf[D][EPT];
+    // two components per element for the singlet and T0 bins (combined when
+    // stashed, so the loads stay in flight over the products)
+    constexpr int D2 = BIN == 0 || BIN == 1 ? D : 1;
+    constexpr int ROW0 = BIN == 2 ? 0 : BIN == 3 ? 3 : 1;
+    double2 pf[D][EPT], pf2[D2][EPT];
     int64_t fi0 = 0; // first row index of the rows in pf
     auto fetch = [&](int64_t i0) {
         fi0 = i0;
@@ -2134,15 +2176,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
         for (int d = 0; d < D; ++d) {
             const int64_t row = rbeg + wave + kWaves * min(i0 + d, nr - 1);
 #pragma unroll
-            for (int k = 0; k < EPT; ++k) pf[d][k] = lp[k][row * Mp + lc[k]];
+            for (int k = 0; k < EPT; ++k) {
+                if constexpr (BIN == kBinAll) {
+                    pf[d][k] = lp[k][row * Mp + lc[k]];
+                } else {
+                    pf[d][k] = lp[k][(4 * row + ROW0) * Mp + lc[k]];
+                    if constexpr (BIN == 0 || BIN == 1) pf2[d][k] = lp[k][(4 * row + 2) * Mp + lc[k]];
+                }
+            }
         }
     };
     auto stash = [&](int bb) { // rows past the wave's last are stored as zeros (they add nothing)
+        constexpr double h = 0.70710678118654752440;
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int k = 0; k < EPT; ++k)
-                if (t + kWave * k < ne) buf[wave][bb][d][t + kWave * k] = fi0 + d < nr ? pf[d][k] : cz(0, 0);
+                if (t + kWave * k < ne) {
+                    double2 v = pf[d][k];
+                    if constexpr (BIN == 0) v = cscale(h, csub(pf[d][k], pf2[d][k]));
+                    if constexpr (BIN == 1) v = cscale(h, cadd(pf[d][k], pf2[d][k]));
+                    buf[wave][bb][d][t + kWave * k] = fi0 + d < nr ? v : cz(0, 0);
+                }
     };
     auto wave_sync = [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2192,7 +2247,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
                     double2 s = acc[i][j];
 #pragma unroll
                     for (int w = 0; w < kWaves - 1; ++w) s = cadd(s, red[w][i][t]);
-                    part[((int64_t)rb * npk + packed_index(xi, yj, nd)) * Mp + r] = s;
+                    part[((int64_t)rb * pks + pko + packed_index(xi, yj, nd)) * Mp + r] = s;
                 }
             }
         __syncthreads();
@@ -2207,10 +2262,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
 // (from H y_j = y_{j+1} + sig_j y_j in the basis; r_{j+1} = R_{j+1}, r_m = p),
 // beta_0 = R_00 = |psi|, and cond[1] = max(cond[1], m ||L^-1||_F^2), L the
 // scaled factor (a bound on the scaled Gram matrix's condition number).
+// G is the sum of nbins packed matrices bstride apart (the projector bins).
 __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts sh, const double2 *__restrict__ dots,
-                                                 double2 *__restrict__ coef, double2 *__restrict__ hess,
-                                                 double *__restrict__ beta0, int32_t *__restrict__ flags,
-                                                 unsigned long long *__restrict__ cond)
+                                                 int nbins, int64_t bstride, double2 *__restrict__ coef,
+                                                 double2 *__restrict__ hess, double *__restrict__ beta0,
+                                                 int32_t *__restrict__ flags, unsigned long long *__restrict__ cond)
 {
     constexpr int MX = kMaxKrylov;
     __shared__ double2 R[MX * MX], Ri[MX * MX], Hc[MX * MX];
@@ -2224,7 +2280,9 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
     for (int e = q; e < m * nd; e += kWave) { // G rows 0 .. m-1, upper triangle
         const int a = e / nd, b = e % nd;
         if (b < a) continue;
-        const double2 g = dots[(int64_t)packed_index(a, b, nd) * Mp + r];
+        const double2 *gp = dots + (int64_t)packed_index(a, b, nd) * Mp + r;
+        double2 g = gp[0];
+        for (int x = 1; x < nbins; ++x) g = cadd(g, gp[x * bstride]);
         if (b < m)
             R[a * MX + b] = g;
         else
@@ -2317,6 +2375,43 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
     }
 }
 
+// Per lane (one wave): the projector Gram matrices over the orthonormal
+// basis Q = Y R^-1 of a Newton block, G_X = R^-H (Y^H P_X Y) R^-1 for the
+// four bins of k_ngram_rows (leading m x m of each packed (m+1)-matrix,
+// bstride apart), written as gram[X][i][j][Mp] (row-major per lane).
+// coef[x*m + j] = (R^-1)[x][j] as k_ncoef leaves it.
+__global__ __launch_bounds__(kWave) void k_nproj(int m, int Mp, const double2 *__restrict__ bins, int64_t bstride,
+                                                 const double2 *__restrict__ coef, double2 *__restrict__ gram)
+{
+    constexpr int MX = kMaxKrylov;
+    __shared__ double2 Ri[MX * MX], G[MX * MX], Tm[MX * MX];
+    const int r = blockIdx.x, q = threadIdx.x;
+    const int nd = m + 1;
+    for (int idx = q; idx < m * m; idx += kWave) Ri[idx] = coef[(int64_t)idx * Mp + r];
+    for (int X = 0; X < 4; ++X) {
+        const double2 *b = bins + X * bstride + r;
+        for (int idx = q; idx < m * m; idx += kWave) {
+            const int a = idx / m, c = idx % m;
+            G[idx] = a <= c ? b[(int64_t)packed_index(a, c, nd) * Mp] : cconj(b[(int64_t)packed_index(c, a, nd) * Mp]);
+        }
+        __syncthreads();
+        for (int idx = q; idx < m * m; idx += kWave) { // T = G R^-1 (R^-1 upper triangular)
+            const int a = idx / m, j = idx % m;
+            double2 acc = cz(0, 0);
+            for (int k = 0; k <= j; ++k) acc = cfma(G[a * m + k], Ri[k * m + j], acc);
+            Tm[idx] = acc;
+        }
+        __syncthreads();
+        for (int idx = q; idx < m * m; idx += kWave) { // R^-H T
+            const int i = idx / m, j = idx % m;
+            double2 acc = cz(0, 0);
+            for (int a = 0; a <= i; ++a) acc = cfma_conj(Ri[a * m + i], Tm[a * m + j], acc);
+            gram[((int64_t)X * m * m + idx) * Mp + r] = acc;
+        }
+        __syncthreads();
+    }
+}
+
 // k_fold for the coefficient sets C[o][m][Mp], o = blockIdx.z: D[o][x] =
 // [x < K] C[o][x] + sum_j coef[x*nf + j] C[o][K+j].  Grid (lanes / 64, K + nf, sets).
 __global__ void k_fold_sets(int K, int nf, int m, int Mp, const double2 *__restrict__ C,
@@ -2373,7 +2468,7 @@ int ngram_form()
 int newton_nrb(const KrylovWork &w)
 {
     if (w.m > kMaxKrylov) return 0;
-    const size_t npk = (size_t)(w.m + 1) * (size_t)(w.m + 2) / 2;
+    const size_t npk = (size_t)(w.m + 1) * (size_t)(w.m + 2) / 2 * (w.gbins ? 4 : 1);
     const size_t fit = w.part_capacity / (npk * (size_t)w.Mp);
     return (int)std::min<size_t>((size_t)std::min(sstep_row_blocks(w.N), kNRowBlocks), fit & ~(size_t)7);
 }
@@ -2401,6 +2496,43 @@ hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
         x.p[j + 1] = out;
     }
     const int nt = (nd + kNTile - 1) / kNTile, npairs = nt * (nt + 1) / 2;
+    using RowsFn = void (*)(const double2 *, const double2 *, int64_t, const double2 *, int, double2 *, int64_t, int,
+                            int64_t, int, int, int, int);
+    NShifts sh{};
+    for (int j = 0; j < m; ++j) sh.s[j] = w.shifts[j];
+    double2 *const coef = w.sbuf + (size_t)kNPacked * w.Mp;
+    if (w.gbins) { // several steps per basis: the four projector bins, their sum the Gram matrix
+        const int L = npairs <= 16 ? 4 : 2;
+        if (npairs > 32 || w.Mp % L) return hipErrorInvalidValue;
+        const RowsFn fns[2][4] = {{k_ngram_rows<kNTile, 4, 2, 0>, k_ngram_rows<kNTile, 4, 2, 1>,
+                                   k_ngram_rows<kNTile, 4, 2, 2>, k_ngram_rows<kNTile, 4, 2, 3>},
+                                  {k_ngram_rows<kNTile, 2, 2, 0>, k_ngram_rows<kNTile, 2, 2, 1>,
+                                   k_ngram_rows<kNTile, 2, 2, 2>, k_ngram_rows<kNTile, 2, 2, 3>}};
+        {
+            // bins 0, 1 read rows 4n+1, 4n+2 of each vector, bins 2, 3 one row each: 1.5 passes
+            ProfScope ps(w, PF_GRAM, 1.5 * nd * vec_bytes(w), s);
+            const int64_t rpb = ceil_div(w.Z, nrb);
+            for (int x = 0; x < 4; ++x)
+                hipLaunchKernelGGL(fns[L == 4 ? 0 : 1][x], dim3(nrb * (w.Mp / L)), dim3(kBlock), 0, s, u0, w.Q, qs, v0,
+                                   nd, w.part, w.Z, w.Mp, rpb, nrb, nt, 4 * npk, x * npk);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = reduce(w, 4 * npk, nrb, w.gbins, s)) != hipSuccess) return e;
+        {
+            ProfScope ps(w, PF_COEFFS, 0.0, s);
+            hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.gbins, 4,
+                               (int64_t)npk * w.Mp, coef, w.hess, w.beta0, w.flags, w.cond);
+            hipLaunchKernelGGL(k_nproj, dim3(w.Mp), dim3(kWave), 0, s, m, w.Mp, w.gbins, (int64_t)npk * w.Mp, coef,
+                               w.gram);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        w.gram_ready = true;
+        w.fold_K = 0;
+        w.fold_nf = m;
+        w.fold_coef = coef;
+        w.fold_q0 = u0;
+        return hipSuccess;
+    }
     {
         ProfScope ps(w, PF_SDOTS, nd * vec_bytes(w), s);
         const int64_t rpb = ceil_div(w.N, nrb);
@@ -2411,23 +2543,18 @@ hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
         } else {
             const int L = npairs <= 16 ? 4 : 2; // tile pairs per lane: 64 / L threads
             if (npairs > 32 || w.Mp % L) return hipErrorInvalidValue;
-            using RowsFn = void (*)(const double2 *, const double2 *, int64_t, const double2 *, int, double2 *, int64_t,
-                                    int, int64_t, int, int);
-            const RowsFn fn = L == 4 ? (form == 2 ? k_ngram_rows<kNTile, 4, 2> : k_ngram_rows<kNTile, 4, 3>)
-                                     : (form == 2 ? k_ngram_rows<kNTile, 2, 2> : k_ngram_rows<kNTile, 2, 3>);
+            const RowsFn fn = L == 4 ? (form == 2 ? k_ngram_rows<kNTile, 4, 2, kBinAll> : k_ngram_rows<kNTile, 4, 3, kBinAll>)
+                                     : (form == 2 ? k_ngram_rows<kNTile, 2, 2, kBinAll> : k_ngram_rows<kNTile, 2, 3, kBinAll>);
             hipLaunchKernelGGL(fn, dim3(nrb * (w.Mp / L)), dim3(kBlock), 0, s, u0, w.Q, qs, v0, nd, w.part, w.N, w.Mp,
-                               rpb, nrb, nt);
+                               rpb, nrb, nt, npk, 0);
         }
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = reduce(w, npk, nrb, w.sbuf, s)) != hipSuccess) return e;
-    NShifts sh{};
-    for (int j = 0; j < m; ++j) sh.s[j] = w.shifts[j];
-    double2 *const coef = w.sbuf + (size_t)kNPacked * w.Mp;
     {
         ProfScope ps(w, PF_COEFFS, 0.0, s);
-        hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, coef, w.hess, w.beta0,
-                           w.flags, w.cond);
+        hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, 1, (int64_t)0, coef,
+                           w.hess, w.beta0, w.flags, w.cond);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     w.fold_K = 0;
@@ -2473,7 +2600,10 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
     static_assert(kNPacked + kMaxKrylov * kMaxKrylov <= kSStepBuf, "Newton buffer size");
     w.fold_K = w.fold_nf = 0;
     w.fold_coef = w.fold_q0 = nullptr;
-    if (w.fold_psi && w.shifts && w.sstep >= 2 && w.sbuf && w.m >= 2 && newton_enabled()) {
+    w.gram_ready = false;
+    // a Newton block where a basis serves one step, or several with the
+    // projector bins (the Grams then come from the chain)
+    if ((w.fold_psi || (w.gbins && w.gram)) && w.shifts && w.sstep >= 2 && w.sbuf && w.m >= 2 && newton_enabled()) {
         const int nrb = newton_nrb(w);
         if (nrb >= 8) return arnoldi_newton(w, nrb, s);
     }
@@ -2556,6 +2686,10 @@ hipError_t expm_coeffs(KrylovWork &w, const TauSet &ts, double2 *C, hipStream_t 
 
 hipError_t gram_projectors(KrylovWork &w, hipStream_t s)
 {
+    if (w.gram_ready) { // the Newton block made them from its chain
+        w.gram_ready = false;
+        return hipSuccess;
+    }
     const int nch = w.Mp / kWave;
     // row blocks sized so the partials fit the shared buffer
     int64_t per = (int64_t)4 * w.m * w.m * w.Mp;
@@ -2575,8 +2709,12 @@ hipError_t gram_projectors(KrylovWork &w, hipStream_t s)
 hipError_t step_lanes(KrylovWork &w, double dt, double t_end, double tol, bool gram, hipStream_t s)
 {
     ProfScope ps(w, PF_STEP, 0.0, s);
-    hipLaunchKernelGGL(k_step, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.m, w.M, w.Mp, dt, t_end, tol,
-                       gram ? 1 : 0, w.E, w.beta0, w.c, w.lane_t, w.lane_k, w.obs, w.n_out, w.gram, w.flags);
+    if (gram)
+        hipLaunchKernelGGL(k_step_gram, dim3(w.Mp), dim3(kWave), 0, s, w.m, w.M, w.Mp, dt, t_end, tol, w.E, w.beta0,
+                           w.c, w.lane_t, w.lane_k, w.obs, w.n_out, w.gram, w.flags);
+    else
+        hipLaunchKernelGGL(k_step, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.m, w.M, w.Mp, dt, t_end, tol,
+                           w.E, w.beta0, w.c, w.lane_t, w.lane_k, w.n_out, w.flags);
     return hipGetLastError();
 }
 
