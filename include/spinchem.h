@@ -246,7 +246,7 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
                    double *ms_per_orient_step);
 
 /* Autotuner for spc_orientation_batch with several steps per basis (no
- * counterpart in the reference): over the orders 16, 20, .., 32, the loosest
+ * counterpart in the reference): over the orders 16, 20, 24, 28, the loosest
  * tolerance of 1e-12, 1e-16, .., 1e-28 (the reference's rebuild rule,
  * qm_dynamics.f90:348-364) whose pilot yields (256 steps, up to 4
  * orientations) stay within a tenth of yield_tol of the requested order at

@@ -542,9 +542,7 @@ int alloc_lane_set(spc_ctx *ctx, LaneSet &L, int idx, int nsets, int Mp, int m, 
     if ((rc = ensure(ctx, nm("hess").c_str(), (size_t)m * m * Mp, &w.hess))) return rc;
     if ((rc = ensure(ctx, nm("E").c_str(), (size_t)m * m * Mp, &w.E))) return rc;
     if ((rc = ensure(ctx, nm("c").c_str(), (size_t)m * Mp, &w.c))) return rc;
-    if (need_gram && (rc = ensure(ctx, nm("gram").c_str(), (size_t)4 * m * m * Mp, &w.gram))) return rc;
-    if (need_gram && w.sstep >= 2 &&
-        (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
+    if (need_gram && (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
         return rc;
     if ((rc = ensure(ctx, nm("beta0").c_str(), (size_t)Mp, &w.beta0))) return rc;
     if ((rc = ensure(ctx, nm("lane_t").c_str(), (size_t)Mp, &w.lane_t))) return rc;
@@ -670,13 +668,8 @@ int gram_buffers(spc_ctx *ctx, LaneSet &L)
         if ((rc = ensure(ctx, ("part" + sfx).c_str(), part_need, &w.part))) return rc;
         w.part_capacity = part_need;
     }
-    if (!w.gram && (rc = ensure(ctx, ("gram" + sfx).c_str(), (size_t)4 * w.m * w.m * w.Mp, &w.gram))) return rc;
-    if (w.sstep < 2) {
-        w.gbins = nullptr; // column by column: the Grams from the formed basis
-    } else if (!w.gbins) {
-        if ((rc = ensure(ctx, ("gbins" + sfx).c_str(), (size_t)2 * (w.m + 1) * (w.m + 2) * w.Mp, &w.gbins)))
-            return rc;
-    }
+    if (!w.gbins && (rc = ensure(ctx, ("gbins" + sfx).c_str(), (size_t)2 * (w.m + 1) * (w.m + 2) * w.Mp, &w.gbins)))
+        return rc;
     return SPC_OK;
 }
 
@@ -713,7 +706,6 @@ int gram_rounds(spc_ctx *ctx, LaneSet *sets, const std::vector<int> &js, const s
                 if ((rc = block_poor(ctx, L, poor))) return rc;
                 if (poor) {
                     L.w.sstep = 0;
-                    L.w.gbins = nullptr;
                     if ((rc = start_pass(ctx, L, pop))) return rc;
                     next.push_back(j);
                     continue;
@@ -1232,12 +1224,9 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
     if ((rc = ensure(ctx, nm("flags").c_str(), (size_t)4, &w.flags))) return rc;
     if ((rc = ensure(ctx, nm("obs").c_str(), (size_t)nout * 4 * Mp, &w.obs))) return rc;
     if (w.sstep >= 2 && (rc = ensure(ctx, nm("sbuf").c_str(), (size_t)kSStepBuf * Mp, &w.sbuf))) return rc;
-    if (plan.tol > 0.0) { // projector Grams (and their bins on the Newton chain) for several steps per basis
-        if ((rc = ensure(ctx, nm("gram").c_str(), (size_t)4 * m * m * Mp, &w.gram))) return rc;
-        if (w.sstep >= 2 &&
-            (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
-            return rc;
-    }
+    if (plan.tol > 0.0 && // the projector bins for several steps per basis
+        (rc = ensure(ctx, nm("gbins").c_str(), (size_t)2 * (m + 1) * (m + 2) * Mp, &w.gbins)))
+        return rc;
     if (w.sstep >= 2) { // the pivot monitor, over every chunk of the slot
         if ((rc = ensure(ctx, nm("cond").c_str(), (size_t)kCondWords, &w.cond))) return rc;
         if (!sl.n) HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, sl.s));
@@ -1293,7 +1282,7 @@ int begin_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *
             !holds("obs", (size_t)nout * 4 * Mp * sizeof(double)) ||
             !holds("psi", (size_t)ctx->N * Mp * sizeof(double2)) ||
             !holds("Q", (size_t)m * w.qs * sizeof(double2)) || !holds("part", w.part_capacity * sizeof(double2)) ||
-            (plan.tol > 0.0 && !holds("gram", (size_t)4 * m * m * Mp * sizeof(double2))))
+            (plan.tol > 0.0 && !holds("gbins", (size_t)2 * (m + 1) * (m + 2) * Mp * sizeof(double2))))
             return fail(ctx, SPC_ERR_INVALID, "batched orientations: a chunk does not fit its slot's workspace");
     }
     HIP_TRY(h_field_values(ctx->fv0, ctx->fz[0], ctx->fz[1], ctx->fz[2], sh.fields + 3 * (size_t)o0, nb * 16, n,
@@ -1454,14 +1443,14 @@ int max_orient_per_chunk(spc_ctx *ctx, int m, int M, int n_orient, int64_t nout,
     // slot that does not reallocate a buffer would otherwise keep it beside
     // the new ones
     for (const std::string sfx : {"", "#1", "#2", "#3"})
-        for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part", "gram", "gbins", "bkdiag"})
+        for (const char *nm : {"Q", "psi", "v", "obs", "bvals", "part", "gbins", "bkdiag"})
             release_buf(ctx, (nm + sfx).c_str());
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
     size_t per_orient = ((size_t)m + 3) * ctx->N * M * sizeof(double2) + (size_t)nout * 4 * M * sizeof(double) +
                         (size_t)ctx->fpat.nblocks * 16 * sizeof(double2);
-    if (grams) // projector Grams, their bins and the partials of eight row blocks (tol > 0)
-        per_orient += ((size_t)4 * m * m * 9 + (size_t)2 * (m + 1) * (m + 2)) * M * sizeof(double2);
+    if (grams) // projector bins and the partials of eight row blocks (tol > 0)
+        per_orient += ((size_t)4 * m * m * 8 + (size_t)2 * (m + 1) * (m + 2)) * M * sizeof(double2);
     // at most 64 GiB per chunk: one allocation of the basis near the whole
     // device failed (147 456-dimensional pair, 32 orientations: 154 GB), and
     // wider chunks of large pairs buy nothing (one orientation's product
@@ -1511,8 +1500,9 @@ double union_ms(std::vector<std::pair<double, double>> iv)
 // which some chunk's steps run.
 int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sample_seeds, int n_orient,
                        const double *dirs, const BatchPlan &plan, std::vector<double> &ps_out, double *total_ms,
-                       int64_t *rounds_out = nullptr)
+                       int64_t *rounds_out = nullptr, bool *columns_out = nullptr)
 {
+    if (columns_out) *columns_out = plan.columns;
     int64_t rounds_local = 0;
     int64_t *const rounds = rounds_out ? rounds_out : &rounds_local;
     *rounds = 0;
@@ -1647,14 +1637,16 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
         // an orientation's H did not match its Kronecker tables: the block product
         BatchPlan p2 = plan;
         p2.kron = false;
-        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out);
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out,
+                                  columns_out);
     }
     bool broke = false;
     for (int c = 0; c < nchunks; ++c) broke = broke || flags[4 * c] != 0;
     if ((poor || broke) && !plan.columns) { // the whole batch again, column by column
         BatchPlan p2 = plan;
         p2.columns = true;
-        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out);
+        return batch_orientations(ctx, sim, sample_seeds, n_orient, dirs, p2, ps_out, total_ms, rounds_out,
+                                  columns_out);
     }
     for (int c = 0; c < nchunks; ++c) {
         if (flags[4 * c + 3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
@@ -2926,7 +2918,8 @@ int spc_batch_tune(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *sampl
         tp.opc = c;
         tp.graph = true;
         tp.slots = orient_slots(c, cap, n_orient);
-        const int no = std::min(n_orient, tp.slots * c);
+        // two chunks per slot: the time between a slot's chunks counts as in the full run
+        const int no = std::min(n_orient, 2 * tp.slots * c);
         std::vector<double> tmp;
         double ms = 0.0;
         if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc; // warm
@@ -2984,18 +2977,23 @@ int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64
     // (order, tolerance): for each order the loosest tolerance of the ladder
     // whose pilot yields stay within the scaled yield tolerance (a tighter one
     // only rebuilds more often), then the pair with the least pilot time;
-    // one step per basis at the requested order if none passes
+    // one step per basis at the requested order if none passes.  Orders up to
+    // 28: at 32 the Newton block's condition estimate passed its limit for
+    // some orientations of the 3 072-dimensional pair (the batch then runs
+    // column by column), and an order whose pilot did so is dropped too.
     int m_best = sim->n_krylov;
     double tol_best = 0.0, per_best = ref_ms / ((double)npilot * (double)std::max<int64_t>(pilot, 1));
     const double ladder[] = {1e-12, 1e-16, 1e-20, 1e-24, 1e-28};
-    for (int m = 16; m <= SPC_MAX_KRYLOV; m += 4) {
+    for (int m = 16; m <= std::min(28, SPC_MAX_KRYLOV); m += 4) {
         for (double t : ladder) {
             plan.m = m;
             plan.tol = t;
             ps.n_krylov = m;
             ps.tol = t;
             double ms = 0.0;
-            if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, &ms)) break;
+            bool cols = false;
+            if (batch_orientations(ctx, &ps, sample_seeds, npilot, dirs, plan, cur, &ms, nullptr, &cols) || cols)
+                break;
             if (worst_err() > ytol) continue;
             const double per = ms / ((double)npilot * (double)std::max<int64_t>(pilot, 1));
             if (per < per_best) {
@@ -3024,7 +3022,8 @@ int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64
         tp.opc = c;
         tp.graph = true;
         tp.slots = orient_slots(c, cap, n_orient);
-        const int no = std::min(n_orient, tp.slots * c);
+        // two chunks per slot: the time between a slot's chunks counts as in the full run
+        const int no = std::min(n_orient, 2 * tp.slots * c);
         std::vector<double> tmp;
         double ms = 0.0;
         if ((rc = batch_orientations(ctx, &ts, sample_seeds, no, dirs, tp, tmp, &ms))) return rc; // warm

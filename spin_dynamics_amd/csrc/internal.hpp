@@ -205,7 +205,6 @@ struct KrylovWork {
     double2 *hess;   // Hessenberg [m][m][Mp] (col-major per lane)
     double2 *E;      // expm [m][m][Mp]
     double2 *c;      // Krylov coefficients [m][Mp]
-    double2 *gram;   // projector Gram matrices [4][m][m][Mp] (tol > 0)
     double *beta0;   // [Mp]
     double *lane_t;  // [Mp]
     int32_t *lane_k; // [Mp] next output index
@@ -238,11 +237,19 @@ struct KrylovWork {
     // over them follow from fold_coef, and slot 0's vector is fold_q0) and
     // psi_update folds the forming coefficients into c instead.
     bool fold_psi = false;
-    // Packed projector bins [4][(m+1)(m+2)/2][Mp] (null: none).  With them
-    // (tol > 0) a Newton block also makes the projector Grams from its chain
-    // (gram_ready until gram_projectors consumes them).
+    // Packed projector bins [4][(m+1)(m+2)/2][Mp] (null: none; tol > 0):
+    // a Newton block leaves Y^H P_X Y of its chain there (gram_ready until
+    // gram_projectors consumes the flag), gram_projectors Q^H P_X Q of a
+    // formed basis.
     double2 *gbins = nullptr;
     bool gram_ready = false;
+    // what the step loop reads for the populations (set by gram_projectors or
+    // the Newton block): the four Hermitian G_X step_gxs apart, packed with
+    // row length step_nd (0: full m x m), over R^-1 c when step_Ri is set
+    const double2 *step_G = nullptr;
+    int64_t step_gxs = 0;
+    int step_nd = 0;
+    const double2 *step_Ri = nullptr;
     int fold_K = 0, fold_nf = 0;
     const double2 *fold_coef = nullptr, *fold_q0 = nullptr;
 };

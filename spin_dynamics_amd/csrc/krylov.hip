@@ -1131,17 +1131,19 @@ __device__ void expm_block(int m, double2 *X, double2 *P, double2 *E, double2 *D
     }
     // Solve D Y = E (the reference forms inv(D) then inv(D) E), result in E.
     for (int k = 0; k < m; ++k) {
-        if (tid == 0) {
-            int p = k;
-            double best = fabs(D[k + m * k].x) + fabs(D[k + m * k].y);
-            for (int i = k + 1; i < m; ++i) {
-                double v = fabs(D[i + m * k].x) + fabs(D[i + m * k].y);
-                if (v > best) {
-                    best = v;
-                    p = i;
+        if (tid < kWave) { // the first row of largest |re| + |im| in column k at or below k (m <= 64)
+            double v = tid >= k && tid < m ? fabs(D[tid + m * k].x) + fabs(D[tid + m * k].y) : -1.0;
+            int p = tid;
+#pragma unroll
+            for (int sft = kWave / 2; sft >= 1; sft >>= 1) {
+                const double ov = __shfl_xor(v, sft);
+                const int op = __shfl_xor(p, sft);
+                if (ov > v || (ov == v && op < p)) {
+                    v = ov;
+                    p = op;
                 }
             }
-            ipiv[k] = p;
+            if (tid == 0) ipiv[k] = p;
         }
         __syncthreads();
         const int p = ipiv[k];
@@ -1193,12 +1195,17 @@ __global__ __launch_bounds__(kBlock) void k_expm(int m, int Mp, int M, double dt
         for (int idx = threadIdx.x; idx < mm; idx += blockDim.x) out[(int64_t)idx * Mp + r] = cz(0, 0);
         return;
     }
-    for (int idx = threadIdx.x; idx < mm; idx += blockDim.x) {
-        if (mode == 0) {
-            const double2 h = in[(int64_t)idx * Mp + r];
-            X[idx] = cz(dt * h.y, -dt * h.x); // (0,-1)*dt*h
-        } else {
-            X[idx] = in[idx];
+    { // every load in flight before the first store (m*m <= 4 blockDim.x)
+        double2 hv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t idx = min((int)threadIdx.x + kBlock * u, mm - 1);
+            hv[u] = in[mode == 0 ? idx * Mp + r : idx];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = threadIdx.x + kBlock * u;
+            if (idx < mm) X[idx] = mode == 0 ? cz(dt * hv[u].y, -dt * hv[u].x) : hv[u]; // (0,-1)*dt*h
         }
     }
     __syncthreads();
@@ -1287,44 +1294,89 @@ __global__ void k_step(int m, int M, int Mp, double dt, double t_end, double tol
 }
 
 // The same loop with projector Grams (tol > 0: several steps per basis), one
-// workgroup per lane: E (E(i,j) at E[(j*m + i)*Mp + r]) and the upper
-// triangles of the four Hermitian G_X (G_X(i,j) at G[((X*m + i)*m + j)*Mp + r])
-// staged in LDS; thread i < m takes row i of c <- E c and of G_X c, and one
-// thread sums over i in k_step's order (the populations, |c|^2).
-__global__ __launch_bounds__(kWave) void k_step_gram(int m, int M, int Mp, double dt, double t_end, double tol,
+// wave per lane.  The populations are Re(d^H G_X d) for four Hermitian G_X
+// (X-th at G + X*gxs; entry (a, b), a <= b < m, at packed_index(a, b, nd)
+// when nd > 0, else at a*m + b of a full matrix), d = R^-1 c when Ri is given
+// (a Newton block: G_X = Y^H P_X Y over its chain vectors, coef layout
+// Ri[(x*m + j)*Mp + r] = (R^-1)[x][j]), else d = c (G_X over an orthonormal
+// basis).  E(i,j) at E[(j*m + i)*Mp + r].  E and R^-1 sit in LDS (column by
+// column: thread q reads row q conflict-free), each thread keeps its share of
+// the triangle entries of the four G_X in registers, and the sums over the
+// lane's entries are wave reductions.
+constexpr int kStepEntries = (kMaxKrylov * (kMaxKrylov + 1) / 2 + kWave - 1) / kWave;
+__global__ __launch_bounds__(kWave) void k_step_proj(int m, int M, int Mp, double dt, double t_end, double tol,
                                                      const double2 *__restrict__ E, const double *__restrict__ beta0,
                                                      double2 *__restrict__ cout, double *__restrict__ lane_t,
                                                      int32_t *__restrict__ lane_k, double *__restrict__ obs,
-                                                     int64_t n_out, const double2 *__restrict__ G,
-                                                     int32_t *__restrict__ flags)
+                                                     int64_t n_out, const double2 *__restrict__ G, int64_t gxs, int nd,
+                                                     const double2 *__restrict__ Ri, int32_t *__restrict__ flags)
 {
-    constexpr int MX = kMaxKrylov, NP = MX * (MX + 1) / 2;
-    __shared__ double2 Es[MX * MX], Gp[4][NP], cs[2][MX];
-    __shared__ double pv[4][MX];
-    __shared__ double sc2, scm2;
+    constexpr int MX = kMaxKrylov, NE = kStepEntries;
+    __shared__ double2 Es[MX * MX], Rs[MX * MX], cs[2][MX], ds[MX];
     const int r = blockIdx.x, q = threadIdx.x;
     if (r >= M) return; // uniform over the workgroup
-    for (int idx = q; idx < m * m; idx += kWave) Es[idx] = E[(int64_t)idx * Mp + r];
-    for (int x = 0; x < 4; ++x)
-        for (int idx = q; idx < m * m; idx += kWave) {
-            const int i = idx / m, j = idx % m;
-            if (j >= i) Gp[x][packed_index(i, j, m)] = G[((int64_t)x * m * m + idx) * Mp + r];
+    { // every load in flight before the first store (clamped indices, no branch around the loads)
+        constexpr int LPT = MX * MX / kWave;
+        const int mm = m * m;
+        double2 te[LPT], tr[LPT];
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int64_t idx = min(q + kWave * u, mm - 1);
+            te[u] = E[idx * Mp + r]; // (j*m + i)
+            tr[u] = Ri ? Ri[idx * Mp + r] : cz(0, 0);
         }
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int idx = q + kWave * u;
+            if (idx < mm) {
+                Es[idx] = te[u];
+                // (R^-1)[i][j] from coef[(i*m + j)], stored at j*m + i
+                const int i = idx / m, j = idx % m;
+                if (Ri) Rs[j * m + i] = j >= i ? tr[u] : cz(0, 0);
+            }
+        }
+    }
+    // this thread's triangle entries e = q + 64k (row-major over the leading m x m)
+    const int np = m * (m + 1) / 2;
+    int ea[NE], eb[NE];
+    double wt[NE];
+    double2 g[4][NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = q + kWave * k;
+        int a = 0, rem = e < np ? e : 0;
+        while (rem >= m - a) {
+            rem -= m - a;
+            ++a;
+        }
+        const int b = a + rem;
+        ea[k] = a;
+        eb[k] = b;
+        wt[k] = e < np ? (a == b ? 1.0 : 2.0) : 0.0;
+        const int64_t off = nd > 0 ? (int64_t)packed_index(a, b, nd) : (int64_t)a * m + b;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) g[x][k] = G[(x * gxs) + off * Mp + r]; // (0, 0) when e >= np: weight 0
+    }
     const double b0 = beta0[r];
     if (q < m) cs[0][q] = q == 0 ? cz(b0, 0.0) : cz(0, 0);
     double t = lane_t[r];
     int64_t k = lane_k[r];
-    if (q == 0) {
-        sc2 = 1.0;
-        scm2 = 0.0;
-    }
     __syncthreads();
+    double c2 = 1.0, cm2 = 0.0;
     int cur = 0;
-    while (scm2 <= tol * sc2 && t < t_end) {
+    while (cm2 <= tol * c2 && t < t_end) {
+        double2 cq = cz(0, 0);
         if (q < m) {
-            double2 a = cz(0, 0);
-            for (int j = 0; j < m; ++j) a = cfma(Es[j * m + q], cs[cur][j], a);
-            cs[cur ^ 1][q] = a;
+            double2 c1 = cz(0, 0); // two chains: half the FMA latency
+            int j = 0;
+#pragma unroll 4
+            for (; j + 1 < m; j += 2) {
+                cq = cfma(Es[j * m + q], cs[cur][j], cq);
+                c1 = cfma(Es[(j + 1) * m + q], cs[cur][j + 1], c1);
+            }
+            if (j < m) cq = cfma(Es[j * m + q], cs[cur][j], cq);
+            cq = cadd(cq, c1);
+            cs[cur ^ 1][q] = cq;
         }
         cur ^= 1;
         k = k + 1;
@@ -1333,73 +1385,47 @@ __global__ __launch_bounds__(kWave) void k_step_gram(int m, int M, int Mp, doubl
             break;
         }
         __syncthreads();
-        if (q < m) {
-            const double2 ci = cs[cur][q];
-            for (int x = 0; x < 4; ++x) {
-                double2 a = cz(0, 0);
-                for (int j = 0; j < m; ++j) {
-                    const double2 g = j >= q ? Gp[x][packed_index(q, j, m)] : cconj(Gp[x][packed_index(j, q, m)]);
-                    a = cfma(g, cs[cur][j], a);
+        if (Ri) {
+            if (q < m) {
+                double2 dq = cz(0, 0), d1 = cz(0, 0);
+                int j = q;
+#pragma unroll 4
+                for (; j + 1 < m; j += 2) {
+                    dq = cfma(Rs[j * m + q], cs[cur][j], dq);
+                    d1 = cfma(Rs[(j + 1) * m + q], cs[cur][j + 1], d1);
                 }
-                pv[x][q] = ci.x * a.x + ci.y * a.y; // Re(conj(c_i) (G_X c)_i)
+                if (j < m) dq = cfma(Rs[j * m + q], cs[cur][j], dq);
+                ds[q] = cadd(dq, d1);
             }
+            __syncthreads();
         }
-        __syncthreads();
-        if (q == 0) {
+        const double2 *dv = Ri ? ds : cs[cur];
+        double p[5] = {0.0, 0.0, 0.0, 0.0, q < m ? cabs2(cq) : 0.0};
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const double2 da = dv[ea[e]], db = dv[eb[e]];
+#pragma unroll
             for (int x = 0; x < 4; ++x) {
-                double acc = 0.0;
-                for (int i = 0; i < m; ++i) acc += pv[x][i];
-                obs[(k * 4 + x) * Mp + r] = acc;
+                const double2 gb = cmul(g[x][e], db);
+                p[x] += wt[e] * (da.x * gb.x + da.y * gb.y); // Re(conj(d_a) G_ab d_b)
             }
-            double c2 = 0.0;
-            for (int i = 0; i < m; ++i) c2 += cabs2(cs[cur][i]);
-            sc2 = c2;
-            scm2 = cabs2(cs[cur][m - 1]);
         }
+#pragma unroll
+        for (int sft = kWave / 2; sft >= 1; sft >>= 1)
+#pragma unroll
+            for (int x = 0; x < 5; ++x) p[x] += __shfl_xor(p[x], sft);
+        if (q == 0)
+            for (int x = 0; x < 4; ++x) obs[(k * 4 + x) * Mp + r] = p[x];
+        c2 = p[4];
+        cm2 = cabs2(cs[cur][m - 1]);
         t = t + dt;
-        __syncthreads();
+        __syncthreads(); // the next step rewrites ds and cs[cur ^ 1]
     }
     if (q < m) cout[(int64_t)q * Mp + r] = cs[cur][q];
     if (q == 0) {
         lane_t[r] = t;
         lane_k[r] = (int32_t)k;
     }
-}
-
-// ----------------------------------------------------- projector Grams
-// G_X(i,j) = <x_i, x_j> over the nuclear rows for x = singlet combination
-// (ab - ba)/sqrt2, T0 combination (ab + ba)/sqrt2, Tp block, Tm block.
-// Re(c^H G_X c) equals Re(c^H P_X c) of convert_to_krylov
-// (qm_dynamics.f90:469-498), whose P_S/P_T0 differ from G_X only by
-// anti-Hermitian parts.  Used when tol > 0 (several steps per rebuild).
-__global__ __launch_bounds__(kWave) void k_gram(const double2 *__restrict__ Q, double2 *__restrict__ part, int m,
-                                                 int64_t Z, int Mp, int64_t rows_per_block, int64_t qs)
-{
-    const int nch = Mp / kWave;
-    const int ch = blockIdx.x % nch;
-    const int rb = blockIdx.x / nch;
-    const int xi = blockIdx.y; // x*m + i
-    const int x = xi / m, i = xi % m;
-    const int r = ch * kWave + threadIdx.x;
-    const int64_t NMp = qs;
-    double2 acc[kMaxKrylov];
-    for (int j = 0; j < m; ++j) acc[j] = cz(0, 0);
-    const double h = 0.70710678118654752440;
-    auto load = [&](int j, int64_t n) -> double2 {
-        // x follows the output order: 0 P_S, 1 P_T0, 2 P_Tp, 3 P_Tm
-        const double2 *q = Q + (int64_t)j * NMp;
-        if (x == 2) return q[(4 * n) * Mp + r];
-        if (x == 3) return q[(4 * n + 3) * Mp + r];
-        const double2 ab = q[(4 * n + 1) * Mp + r], ba = q[(4 * n + 2) * Mp + r];
-        return x == 0 ? cscale(h, csub(ab, ba)) : cscale(h, cadd(ab, ba));
-    };
-    const int64_t n0 = rb * rows_per_block;
-    const int64_t n1 = n0 + rows_per_block < Z ? n0 + rows_per_block : Z;
-    for (int64_t n = n0; n < n1; ++n) {
-        const double2 xi_ = load(i, n);
-        for (int j = 0; j < m; ++j) acc[j] = cfma_conj(xi_, load(j, n), acc[j]);
-    }
-    for (int j = 0; j < m; ++j) part[(((int64_t)rb * 4 * m + xi) * m + j) * Mp + r] = acc[j];
 }
 
 // ---------------------------------------------------------- dispatch tables
@@ -2254,6 +2280,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
     }
 }
 
+// The Gram matrix as the sum of its four projector bins: G[e][r] =
+// sum_x bins[x][e][r] over the npk packed entries (one thread per entry and lane).
+__global__ void k_sum_bins(int64_t n, int64_t bstride, const double2 *__restrict__ bins, double2 *__restrict__ G)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 a = bins[i], b = bins[i + bstride], c = bins[i + 2 * bstride], d = bins[i + 3 * bstride];
+    G[i] = cadd(cadd(a, b), cadd(c, d));
+}
+
 // Per lane (one wave each): from the packed Gram matrix G of [y_0 .. y_m]
 // (nd = m + 1), R^H R = G[:m, :m] (Cholesky with the columns scaled to unit
 // diagonal first), coef[x*m + j] = (R^-1)[x][j] (q_j = sum_x coef[x*m + j] y_x),
@@ -2262,11 +2298,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1, 2))) 
 // (from H y_j = y_{j+1} + sig_j y_j in the basis; r_{j+1} = R_{j+1}, r_m = p),
 // beta_0 = R_00 = |psi|, and cond[1] = max(cond[1], m ||L^-1||_F^2), L the
 // scaled factor (a bound on the scaled Gram matrix's condition number).
-// G is the sum of nbins packed matrices bstride apart (the projector bins).
 __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts sh, const double2 *__restrict__ dots,
-                                                 int nbins, int64_t bstride, double2 *__restrict__ coef,
-                                                 double2 *__restrict__ hess, double *__restrict__ beta0,
-                                                 int32_t *__restrict__ flags, unsigned long long *__restrict__ cond)
+                                                 double2 *__restrict__ coef, double2 *__restrict__ hess,
+                                                 double *__restrict__ beta0, int32_t *__restrict__ flags,
+                                                 unsigned long long *__restrict__ cond)
 {
     constexpr int MX = kMaxKrylov;
     __shared__ double2 R[MX * MX], Ri[MX * MX], Hc[MX * MX];
@@ -2277,16 +2312,24 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
     const bool active = r < M;
     const int nd = m + 1;
     if (q == 0) sfail = active ? 0 : 1;
-    for (int e = q; e < m * nd; e += kWave) { // G rows 0 .. m-1, upper triangle
-        const int a = e / nd, b = e % nd;
-        if (b < a) continue;
-        const double2 *gp = dots + (int64_t)packed_index(a, b, nd) * Mp + r;
-        double2 g = gp[0];
-        for (int x = 1; x < nbins; ++x) g = cadd(g, gp[x * bstride]);
-        if (b < m)
-            R[a * MX + b] = g;
-        else
-            W[a] = g;
+    { // G rows 0 .. m-1, upper triangle: every load in flight before the first store
+        constexpr int EPT = (MX * (MX + 1) + kWave - 1) / kWave;
+        const int ne = m * nd;
+        double2 gv[EPT];
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int e = min(q + kWave * u, ne - 1), a = e / nd, b = max(e % nd, a);
+            gv[u] = dots[(int64_t)packed_index(a, b, nd) * Mp + r];
+        }
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+            const int e = q + kWave * u, a = e / nd, b = e % nd;
+            if (e >= ne || b < a) continue;
+            if (b < m)
+                R[a * MX + b] = gv[u];
+            else
+                W[a] = gv[u];
+        }
     }
     __syncthreads();
     if (q < m) {
@@ -2372,43 +2415,6 @@ __global__ __launch_bounds__(kWave) void k_ncoef(int m, int M, int Mp, NShifts s
         }
         Hc[j * MX + q] = a;
         hess[((int64_t)j * m + q) * Mp + r] = a;
-    }
-}
-
-// Per lane (one wave): the projector Gram matrices over the orthonormal
-// basis Q = Y R^-1 of a Newton block, G_X = R^-H (Y^H P_X Y) R^-1 for the
-// four bins of k_ngram_rows (leading m x m of each packed (m+1)-matrix,
-// bstride apart), written as gram[X][i][j][Mp] (row-major per lane).
-// coef[x*m + j] = (R^-1)[x][j] as k_ncoef leaves it.
-__global__ __launch_bounds__(kWave) void k_nproj(int m, int Mp, const double2 *__restrict__ bins, int64_t bstride,
-                                                 const double2 *__restrict__ coef, double2 *__restrict__ gram)
-{
-    constexpr int MX = kMaxKrylov;
-    __shared__ double2 Ri[MX * MX], G[MX * MX], Tm[MX * MX];
-    const int r = blockIdx.x, q = threadIdx.x;
-    const int nd = m + 1;
-    for (int idx = q; idx < m * m; idx += kWave) Ri[idx] = coef[(int64_t)idx * Mp + r];
-    for (int X = 0; X < 4; ++X) {
-        const double2 *b = bins + X * bstride + r;
-        for (int idx = q; idx < m * m; idx += kWave) {
-            const int a = idx / m, c = idx % m;
-            G[idx] = a <= c ? b[(int64_t)packed_index(a, c, nd) * Mp] : cconj(b[(int64_t)packed_index(c, a, nd) * Mp]);
-        }
-        __syncthreads();
-        for (int idx = q; idx < m * m; idx += kWave) { // T = G R^-1 (R^-1 upper triangular)
-            const int a = idx / m, j = idx % m;
-            double2 acc = cz(0, 0);
-            for (int k = 0; k <= j; ++k) acc = cfma(G[a * m + k], Ri[k * m + j], acc);
-            Tm[idx] = acc;
-        }
-        __syncthreads();
-        for (int idx = q; idx < m * m; idx += kWave) { // R^-H T
-            const int i = idx / m, j = idx % m;
-            double2 acc = cz(0, 0);
-            for (int a = 0; a <= i; ++a) acc = cfma_conj(Ri[a * m + i], Tm[a * m + j], acc);
-            gram[((int64_t)X * m * m + idx) * Mp + r] = acc;
-        }
-        __syncthreads();
     }
 }
 
@@ -2520,13 +2526,18 @@ hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
         if ((e = reduce(w, 4 * npk, nrb, w.gbins, s)) != hipSuccess) return e;
         {
             ProfScope ps(w, PF_COEFFS, 0.0, s);
-            hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.gbins, 4,
-                               (int64_t)npk * w.Mp, coef, w.hess, w.beta0, w.flags, w.cond);
-            hipLaunchKernelGGL(k_nproj, dim3(w.Mp), dim3(kWave), 0, s, m, w.Mp, w.gbins, (int64_t)npk * w.Mp, coef,
-                               w.gram);
+            const int64_t n = (int64_t)npk * w.Mp;
+            hipLaunchKernelGGL(k_sum_bins, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, n, w.gbins, w.sbuf);
+            hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, coef, w.hess,
+                               w.beta0, w.flags, w.cond);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        // the step loop evaluates Re(d^H (Y^H P_X Y) d), d = R^-1 c, on the bins
         w.gram_ready = true;
+        w.step_G = w.gbins;
+        w.step_gxs = (int64_t)npk * w.Mp;
+        w.step_nd = nd;
+        w.step_Ri = coef;
         w.fold_K = 0;
         w.fold_nf = m;
         w.fold_coef = coef;
@@ -2553,8 +2564,8 @@ hipError_t arnoldi_newton(KrylovWork &w, int nrb, hipStream_t s)
     if ((e = reduce(w, npk, nrb, w.sbuf, s)) != hipSuccess) return e;
     {
         ProfScope ps(w, PF_COEFFS, 0.0, s);
-        hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, 1, (int64_t)0, coef,
-                           w.hess, w.beta0, w.flags, w.cond);
+        hipLaunchKernelGGL(k_ncoef, dim3(w.Mp), dim3(kWave), 0, s, m, w.M, w.Mp, sh, w.sbuf, coef, w.hess,
+                           w.beta0, w.flags, w.cond);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     w.fold_K = 0;
@@ -2603,7 +2614,7 @@ hipError_t arnoldi(KrylovWork &w, hipStream_t s)
     w.gram_ready = false;
     // a Newton block where a basis serves one step, or several with the
     // projector bins (the Grams then come from the chain)
-    if ((w.fold_psi || (w.gbins && w.gram)) && w.shifts && w.sstep >= 2 && w.sbuf && w.m >= 2 && newton_enabled()) {
+    if ((w.fold_psi || w.gbins) && w.shifts && w.sstep >= 2 && w.sbuf && w.m >= 2 && newton_enabled()) {
         const int nrb = newton_nrb(w);
         if (nrb >= 8) return arnoldi_newton(w, nrb, s);
     }
@@ -2684,37 +2695,60 @@ hipError_t expm_coeffs(KrylovWork &w, const TauSet &ts, double2 *C, hipStream_t 
     return hipGetLastError();
 }
 
+// Projector Grams of a formed basis q_0 .. q_{m-1} (column and monomial
+// forms): the four bins of k_ngram_rows over the m basis vectors, G_X =
+// Q^H P_X Q packed (convert_to_krylov, qm_dynamics.f90:469-498; Re(c^H G_X c)
+// equals Re(c^H P_X c) there, whose P_S, P_T0 differ from G_X only by
+// anti-Hermitian parts).  After a Newton block its own bins are used.
 hipError_t gram_projectors(KrylovWork &w, hipStream_t s)
 {
-    if (w.gram_ready) { // the Newton block made them from its chain
+    if (w.gram_ready) { // the Newton block left its bins and R^-1 for the step loop
         w.gram_ready = false;
         return hipSuccess;
     }
-    const int nch = w.Mp / kWave;
-    // row blocks sized so the partials fit the shared buffer
-    int64_t per = (int64_t)4 * w.m * w.m * w.Mp;
-    int nrb = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)(w.part_capacity / per)));
-    int64_t rows = ceil_div(w.Z, nrb);
-    nrb = (int)ceil_div(w.Z, rows);
+    const int m = w.m, nd = m, npk = nd * (nd + 1) / 2;
+    const int nt = (nd + kNTile - 1) / kNTile, npairs = nt * (nt + 1) / 2;
+    const int L = npairs <= 16 ? 4 : 2;
+    if (!w.gbins || npairs > 32 || w.Mp % L || m < 1) return hipErrorInvalidValue;
+    const size_t per = (size_t)4 * npk * w.Mp;
+    const int nrb = (int)std::min<size_t>({(size_t)std::min(sstep_row_blocks(w.N), kNRowBlocks), w.part_capacity / per});
+    if (nrb < 1) return hipErrorInvalidValue;
+    const int64_t qs = qstride(w);
+    using RowsFn = void (*)(const double2 *, const double2 *, int64_t, const double2 *, int, double2 *, int64_t, int,
+                            int64_t, int, int, int, int);
+    const RowsFn fns[2][4] = {{k_ngram_rows<kNTile, 4, 2, 0>, k_ngram_rows<kNTile, 4, 2, 1>,
+                               k_ngram_rows<kNTile, 4, 2, 2>, k_ngram_rows<kNTile, 4, 2, 3>},
+                              {k_ngram_rows<kNTile, 2, 2, 0>, k_ngram_rows<kNTile, 2, 2, 1>,
+                               k_ngram_rows<kNTile, 2, 2, 2>, k_ngram_rows<kNTile, 2, 2, 3>}};
     {
-        ProfScope ps(w, PF_GRAM, 0.0, s);
-        hipLaunchKernelGGL(k_gram, dim3(nrb * nch, 4 * w.m), dim3(kWave), 0, s, w.Q, w.part, w.m, w.Z, w.Mp, rows,
-                           qstride(w));
+        ProfScope ps(w, PF_GRAM, 1.5 * nd * vec_bytes(w), s);
+        const int64_t rpb = ceil_div(w.Z, nrb);
+        for (int x = 0; x < 4; ++x)
+            hipLaunchKernelGGL(fns[L == 4 ? 0 : 1][x], dim3(nrb * (w.Mp / L)), dim3(kBlock), 0, s, w.Q, w.Q, qs,
+                               w.Q + (int64_t)(m - 1) * qs, nd, w.part, w.Z, w.Mp, rpb, nrb, nt, 4 * npk, x * npk);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return reduce(w, 4 * w.m * w.m, nrb, w.gram, s);
+    if ((e = reduce(w, 4 * npk, nrb, w.gbins, s)) != hipSuccess) return e;
+    w.step_G = w.gbins;
+    w.step_gxs = (int64_t)npk * w.Mp;
+    w.step_nd = nd;
+    w.step_Ri = nullptr;
+    return hipSuccess;
 }
 
 hipError_t step_lanes(KrylovWork &w, double dt, double t_end, double tol, bool gram, hipStream_t s)
 {
     ProfScope ps(w, PF_STEP, 0.0, s);
-    if (gram)
-        hipLaunchKernelGGL(k_step_gram, dim3(w.Mp), dim3(kWave), 0, s, w.m, w.M, w.Mp, dt, t_end, tol, w.E, w.beta0,
-                           w.c, w.lane_t, w.lane_k, w.obs, w.n_out, w.gram, w.flags);
-    else
+    if (gram) {
+        if (!w.step_G) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_step_proj, dim3(w.Mp), dim3(kWave), 0, s, w.m, w.M, w.Mp, dt, t_end, tol, w.E, w.beta0,
+                           w.c, w.lane_t, w.lane_k, w.obs, w.n_out, w.step_G, w.step_gxs, w.step_nd, w.step_Ri,
+                           w.flags);
+    } else {
         hipLaunchKernelGGL(k_step, dim3((unsigned)ceil_div(w.Mp, 64)), dim3(64), 0, s, w.m, w.M, w.Mp, dt, t_end, tol,
                            w.E, w.beta0, w.c, w.lane_t, w.lane_k, w.n_out, w.flags);
+    }
     return hipGetLastError();
 }
 
