@@ -208,6 +208,12 @@ def test_torch_function_chain_rule(ctx):
         fd = (value(h6, **{k: vals[k] + step}) - value(h6, **{k: vals[k] - step})) / (2 * step)
         assert abs(scal[k].grad.item() - fd) <= 1e-6 * max(abs(fd), 1e-3), k
     assert math.isclose(F.item(), value(h6), rel_tol=1e-13)
+    # no input needs a gradient: forward alone (the forward run of the gradient
+    # call: the same value), and its device time is a forward run's
+    F0 = f.apply(th.detach(), *[scal[k].detach() for k in ("B", "theta", "phi", "J", "kS", "kT")], prob)
+    assert math.isclose(F0.item(), F.item(), rel_tol=1e-12)
+    _, rebuilds, steps = dynamics.last_run_stats(ctx)
+    assert rebuilds == steps  # one trace-sampling pass, no adjoint sweep
 
 
 def test_gradient_independent_of_the_product_kernel(ctx, monkeypatch):
