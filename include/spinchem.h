@@ -258,6 +258,25 @@ int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64
                             const double *dirs, double yield_tol, int32_t *orient_per_chunk, int32_t *n_krylov,
                             double *tol, double *ms_per_orient_step);
 
+/* Many radical pairs of uneven Hilbert-space dimension, each at the same
+ * n_orient field directions dirs [n][3], in one call: no counterpart in the
+ * reference, which runs one system and one field per call
+ * (app/main.f90:48-79).  Pair p is systems[p] (its field magnitude B; its
+ * direction is replaced) with the same M trace vectors (sample_seeds [M][2])
+ * for every pair and orientation; n_krylov[p], tol[p] and
+ * orient_per_chunk[p] per pair (each array may be NULL: sim's order and tol,
+ * as many orientations per chunk as fit; from spc_batch_tune_adaptive for
+ * the 1e-10 yield tolerance).  Every pair's Hamiltonian, field basis and
+ * Newton shifts are resident at once, and the chunks of all pairs (largest
+ * dimension first) share up to 4 streams: each stream takes the next chunk of
+ * any pair when its current one ends, its step or round replayed from the
+ * stream's hipGraph for that pair and width.  Results equal
+ * spc_orientation_batch per pair.  The context's own system (spc_system_set)
+ * is kept.  Output: Phi_S [n_pairs][n_orient]. */
+int spc_pairs_batch(spc_ctx *ctx, int32_t n_pairs, const spc_spin_system *systems, const spc_sim_params *sim,
+                    const int64_t *sample_seeds, int32_t n_orient, const double *dirs, const int32_t *n_krylov,
+                    const double *tol, const int32_t *orient_per_chunk, double *Phi_S);
+
 /* ------------------------------------------------------- semiclassical */
 /* Semiclassical spin system (isotropic couplings, as src/sc uses a_iso). */
 typedef struct spc_sc_system {

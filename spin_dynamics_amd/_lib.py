@@ -32,7 +32,7 @@ EXPORTS = [
     "spc_ctx_create", "spc_ctx_destroy", "spc_last_error", "spc_version", "spc_system_set", "spc_system_set_block",
     "spc_shrink", "spc_symmetry_blocks", "spc_system_info",
     "spc_hamiltonian_copy", "spc_n_steps", "spc_trace_sample", "spc_trace_sample_noalloc", "spc_trace_sample_seeds", "spc_exact_trace", "spc_expm_small",
-    "spc_krylov_basis", "spc_spmm", "spc_trace_sample_grad", "spc_orientation_batch", "spc_batch_tune", "spc_batch_tune_adaptive", "spc_semiclassical_run", "spc_semiclassical_tangent", "spc_last_run_stats", "spc_profile_enable", "spc_profile_read", "spc_ini_open", "spc_ini_close",
+    "spc_krylov_basis", "spc_spmm", "spc_trace_sample_grad", "spc_orientation_batch", "spc_batch_tune", "spc_batch_tune_adaptive", "spc_pairs_batch", "spc_semiclassical_run", "spc_semiclassical_tangent", "spc_last_run_stats", "spc_profile_enable", "spc_profile_read", "spc_ini_open", "spc_ini_close",
     "spc_ini_num_sections", "spc_ini_has_section", "spc_ini_count_values", "spc_ini_get_string",
     "spc_ini_get_doubles", "spc_ini_get_int64s",
 ]
@@ -110,6 +110,8 @@ def load() -> ctypes.CDLL:
                                    _i32p, _i32p, _dp]
     lib.spc_batch_tune_adaptive.argtypes = [vp, ctypes.POINTER(SimParams), _i64p, ctypes.c_int32, _dp,
                                             ctypes.c_double, _i32p, _i32p, _dp, _dp]
+    lib.spc_pairs_batch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(SpinSystem), ctypes.POINTER(SimParams), _i64p,
+                                    ctypes.c_int32, _dp, _i32p, _dp, _i32p, _dp]
     lib.spc_semiclassical_run.argtypes = [vp, ctypes.POINTER(ScSystem), ctypes.c_int64, _i64p, ctypes.c_double,
                                           ctypes.c_int64, _dp, _dp, _dp, _dp]
     lib.spc_semiclassical_tangent.argtypes = [vp, ctypes.POINTER(ScSystem), ctypes.c_int64, _i64p, ctypes.c_double,

@@ -95,6 +95,41 @@ def tune_adaptive(sys: SysParam, sim: SimParam, dirs: np.ndarray, seed=None, *, 
     return int(opc[0]), int(m[0]), float(tol[0]), float(ms[0])
 
 
+def pairs_yields(systems: Sequence[SysParam], sim: SimParam, dirs: np.ndarray, seed=None, *,
+                 B: Optional[Sequence[float]] = None, n_krylov: Optional[Sequence[int]] = None,
+                 tol: Optional[Sequence[float]] = None, orient_per_chunk: Optional[Sequence[int]] = None,
+                 ctx: Optional[_lib.Context] = None) -> np.ndarray:
+    """Final singlet yields [n_pairs][n_orient] of several radical pairs (of any
+    dimensions) at the directions dirs, in one spc_pairs_batch call: every
+    pair resident at once, the chunks of all pairs sharing the streams.  B[p]
+    defaults to -systems[p].e1.w; per-pair order, tolerance and width default
+    to sim's and to as many orientations as fit."""
+    import ctypes
+    ctx = ctx or context()
+    n = len(systems)
+    Bs = [(-s.e1.w if B is None else B[i]) for i, s in enumerate(systems)]
+    arr = (_lib.SpinSystem * n)()
+    keep = []
+    for i, s in enumerate(systems):
+        c, k = to_c_system(s, Bs[i])
+        arr[i] = c
+        keep.append(k)
+    sd = sample_seeds(sim.seed if seed is None else seed, sim.N_samples)
+    kind = _lib.VEC_PHASE if sim.vec_kind == "phase" else _lib.VEC_SUZ
+    simc = _sim_c(sim, sim.N_samples, kind)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    mk = None if n_krylov is None else np.ascontiguousarray(n_krylov, dtype=np.int32)
+    tl = None if tol is None else np.ascontiguousarray(tol, dtype=np.float64)
+    oc = None if orient_per_chunk is None else np.ascontiguousarray(orient_per_chunk, dtype=np.int32)
+    phi = np.zeros((n, dirs.shape[0]))
+    ctx.check(ctx.lib.spc_pairs_batch(ctx.h, n, ctypes.cast(arr, ctypes.POINTER(_lib.SpinSystem)), simc,
+                                      _lib.ptr(sd, _lib._i64p), dirs.shape[0], _lib.ptr(dirs),
+                                      _lib.ptr(mk, _lib._i32p), _lib.ptr(tl), _lib.ptr(oc, _lib._i32p),
+                                      _lib.ptr(phi)))
+    del keep
+    return phi
+
+
 # ------------------------------------------------------------ pair sets
 DIMS = (768, 3072, 12288, 49152, 147456)
 _NUCLEI = {768: (1, 6), 3072: (1, 8), 12288: (1, 10), 49152: (1, 12), 147456: (2, 12)}  # (14N, 1H)

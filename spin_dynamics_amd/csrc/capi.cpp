@@ -109,6 +109,7 @@ struct spc_ctx {
     HamDev fpat;
     double2 *fv0 = nullptr, *fz[3] = {nullptr, nullptr, nullptr};
     bool fbasis_valid = false;
+    std::string ftag; // name suffix of the field-basis buffers (a resident system's own)
     // workspace
     std::map<std::string, DevBuf> bufs;
     unsigned char *scratch = nullptr; // this device's scratch area (hipGetSymbolAddress)
@@ -120,6 +121,44 @@ struct spc_ctx {
 };
 
 namespace {
+
+// What the context holds of one spin system: its parameters, its Hamiltonian
+// (with the Newton shifts), the adjoint operator and the field basis of the
+// batched orientations.  A multi-pair batch keeps one per pair, resident
+// together, and swaps each into the context while it enqueues that pair's
+// work (swap_state; the device buffers stay where they are).
+struct SysState {
+    bool have_sys = false;
+    int64_t N = 0, Z = 0, nnz = 0, nnz_export = 0;
+    DevSystem hsys{};
+    double kS = 0.0, kT = 0.0;
+    bool iso_applied = false, ref_form = false, adj_valid = false, fbasis_valid = false;
+    HamDev ham, adj, fpat;
+    double2 *fv0 = nullptr, *fz[3] = {nullptr, nullptr, nullptr};
+    std::string ftag;
+};
+
+void swap_state(spc_ctx *ctx, SysState &st)
+{
+    std::swap(ctx->have_sys, st.have_sys);
+    std::swap(ctx->N, st.N);
+    std::swap(ctx->Z, st.Z);
+    std::swap(ctx->nnz, st.nnz);
+    std::swap(ctx->nnz_export, st.nnz_export);
+    std::swap(ctx->hsys, st.hsys);
+    std::swap(ctx->kS, st.kS);
+    std::swap(ctx->kT, st.kT);
+    std::swap(ctx->iso_applied, st.iso_applied);
+    std::swap(ctx->ref_form, st.ref_form);
+    std::swap(ctx->adj_valid, st.adj_valid);
+    std::swap(ctx->fbasis_valid, st.fbasis_valid);
+    std::swap(ctx->ham, st.ham);
+    std::swap(ctx->adj, st.adj);
+    std::swap(ctx->fpat, st.fpat);
+    std::swap(ctx->fv0, st.fv0);
+    for (int a = 0; a < 3; ++a) std::swap(ctx->fz[a], st.fz[a]);
+    std::swap(ctx->ftag, st.ftag);
+}
 
 // This device's scratch address, looked up once per device with the device
 // current (each GPU holds its own copy of the array at its own address).
@@ -1092,10 +1131,10 @@ int build_field_basis(spc_ctx *ctx)
     int rc = build_ham(ctx, with_field(gen), ctx->fpat);
     if (rc) return rc;
     const int64_t nb = ctx->fpat.nblocks, Z = ctx->Z;
-    if ((rc = ensure(ctx, "fv0", (size_t)nb * 16, &ctx->fv0))) return rc;
+    if ((rc = ensure(ctx, ("fv0" + ctx->ftag).c_str(), (size_t)nb * 16, &ctx->fv0))) return rc;
     const char *names[3] = {"fzx", "fzy", "fzz"};
     for (int a = 0; a < 3; ++a)
-        if ((rc = ensure(ctx, names[a], (size_t)nb * 16, &ctx->fz[a]))) return rc;
+        if ((rc = ensure(ctx, (names[a] + ctx->ftag).c_str(), (size_t)nb * 16, &ctx->fz[a]))) return rc;
     double2 *tmp = nullptr;
     int32_t *missing = nullptr;
     double *neg = nullptr;
@@ -1136,6 +1175,7 @@ struct BatchPlan {
     bool kron = true;  // per-orientation Kronecker tables (else the block product)
     bool columns = false; // one column per pass (after the block form broke down or lost orthogonality)
     double tol = 0.0;     // > 0: several steps per basis (rounds with projector Grams)
+    int sys_id = 0;       // the resident system of a multi-pair batch (slots re-set up when it changes)
 };
 
 // One stream of the batched run: its workspace and the step graph of its
@@ -1146,6 +1186,8 @@ struct OrientSlot {
     hipStream_t s = nullptr;
     int idx = 0;
     int n = 0;            // orientations per chunk the workspace is set up for
+    int sys_id = -1, m = 0; // ... and the system, order and tolerance
+    double tol = 0.0;
     bool capture = true;  // the step graph of this width is still to be captured
     std::vector<std::pair<hipGraph_t, hipGraphExec_t>> graphs; // released after the final synchronize
     void release()
@@ -1164,10 +1206,12 @@ struct OrientSlot {
 // slots, checked once the batch has drained.
 struct BatchShared {
     double *fields = nullptr; // [n_orient][3]
-    int64_t *seeds = nullptr; // [opc * M][2]
-    double *obs0 = nullptr;   // [4][opc * M]
+    int64_t *seeds = nullptr; // [lanes][2]
+    double *obs0 = nullptr;   // [4][lanes]
+    int lanes = 0;            // lanes of the widest chunk (the pitch of obs0)
     int32_t *flags = nullptr; // [chunks][4]
     int32_t *bad = nullptr;   // [chunks]
+    unsigned long long *cond = nullptr; // [chunks][kCondWords] (multi-pair batches; null: per slot)
 };
 
 // Sets up the slot's workspace for chunks of n orientations (M lanes each).
@@ -1247,6 +1291,9 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
         w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;
     }
     sl.n = n;
+    sl.sys_id = plan.sys_id;
+    sl.m = plan.m;
+    sl.tol = plan.tol;
     sl.capture = true;
     return SPC_OK;
 }
@@ -1260,7 +1307,7 @@ int begin_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *
                             const BatchShared &sh, int M, int c, int o0, int n, int64_t nout, hipEvent_t a)
 {
     int rc;
-    if (sl.n != n) {
+    if (sl.n != n || sl.sys_id != plan.sys_id || sl.m != plan.m || sl.tol != plan.tol) {
         if (sl.n) HIP_TRY(hipStreamSynchronize(sl.s));
         if ((rc = slot_setup(ctx, sl, plan, M, n, nout))) return rc;
     }
@@ -1301,8 +1348,8 @@ int begin_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const spc_sim_params *
     HIP_TRY(hipMemsetAsync(w.lane_k, 0, sizeof(int32_t) * Mp, s));
     HIP_TRY(hipMemsetAsync(w.flags, 0, sizeof(int32_t) * 4, s));
     HIP_TRY(hipMemsetAsync(w.obs, 0, sizeof(double) * nout * 4 * Mp, s));
-    HIP_TRY(hipMemcpy2DAsync(w.obs, sizeof(double) * Mp, sh.obs0, sizeof(double) * plan.opc * M,
-                             sizeof(double) * Mp, 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(w.obs, sizeof(double) * Mp, sh.obs0, sizeof(double) * sh.lanes, sizeof(double) * Mp, 4,
+                             hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipEventRecord(a, s));
     return SPC_OK;
 }
@@ -1357,6 +1404,11 @@ int end_orientation_chunk(spc_ctx *ctx, OrientSlot &sl, const BatchShared &sh, i
     HIP_TRY(hipEventRecord(b, s));
     HIP_TRY(obs_reduce_groups(w.obs, nout, n, M, M, w.Mp, d_acc + (size_t)o0 * 4 * nout, s));
     HIP_TRY(hipMemcpyAsync(sh.flags + 4 * (size_t)c, w.flags, sizeof(int32_t) * 4, hipMemcpyDeviceToDevice, s));
+    if (sh.cond && w.cond) { // this chunk's conditioning words, then the slot's cleared for its next chunk
+        HIP_TRY(hipMemcpyAsync(sh.cond + kCondWords * (size_t)c, w.cond, sizeof(unsigned long long) * kCondWords,
+                               hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemsetAsync(w.cond, 0, sizeof(unsigned long long) * kCondWords, s));
+    }
     return SPC_OK;
 }
 
@@ -1539,6 +1591,7 @@ int batch_orientations(spc_ctx *ctx, const spc_sim_params *sim, const int64_t *s
     for (int x = 0; x < 4; ++x)
         for (int r = 0; r < Mw; ++r) obs0[(size_t)x * Mw + r] = pop[x];
     BatchShared sh;
+    sh.lanes = Mw;
     double *d_acc = nullptr;
     if ((rc = ensure(ctx, "bacc", (size_t)n_orient * 4 * nout, &d_acc))) return rc;
     if ((rc = ensure(ctx, "bfields", f.size(), &sh.fields))) return rc;
@@ -2003,6 +2056,334 @@ static int trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t s
         return fail(ctx, SPC_ERR_INVALID, "unknown vec_kind");
     }
     return propagate(ctx, sim, vi, M, seeds, trace_vectors, P_S, P_T0, P_Tp, P_Tm, n_out, Phi_S, Phi_T);
+}
+
+} // extern "C"
+
+namespace {
+
+// Swaps a resident system into the context for a scope.
+struct ActiveSys {
+    spc_ctx *ctx;
+    SysState *st;
+    ActiveSys(spc_ctx *c, SysState *s) : ctx(c), st(s) { swap_state(ctx, *st); }
+    ~ActiveSys() { swap_state(ctx, *st); }
+};
+
+// Pairs of uneven dimension, each at the same n_orient directions, in one
+// call.  Every pair's system is resident at once (its Hamiltonian with the
+// Newton shifts and its field basis); the chunks of all pairs, the largest
+// dimension first, share the lane streams: each stream takes the next chunk
+// of any pair when its current one ends (a chunk at tol = 0 is enqueued whole,
+// one at tol > 0 runs in rounds with a host check of its lane clocks between
+// rounds), its step or round replayed from the stream's graph for that pair
+// and width.  A pair whose chunks failed a Kronecker-table check, broke down
+// or built a poorly conditioned block basis runs again alone with the
+// fallback (batch_orientations).  phi[p * n_orient + o]; the context's own
+// system is set aside for the call and restored after it.
+int pairs_batch(spc_ctx *ctx, int n_pairs, const spc_spin_system *systems, const spc_sim_params *sim,
+                const int64_t *sample_seeds, int n_orient, const double *dirs, const int32_t *n_krylov,
+                const double *tols, const int32_t *orient_per_chunk, std::vector<double> &phi, double *total_ms,
+                int64_t *rounds_out)
+{
+    int rc;
+    const int M = sim->n_samples;
+    const int64_t K = steps_taken(sim->t_end, sim->dt), nout = spc_n_steps(sim->t_end, sim->dt) + 1;
+    if (K + 1 > nout) return fail(ctx, SPC_ERR_INVALID, "time loop exceeds the output length");
+    SysState saved;
+    swap_state(ctx, saved); // the context now holds no system
+    std::vector<SysState> res(n_pairs);
+    struct Restore {
+        spc_ctx *ctx;
+        SysState *saved;
+        std::vector<SysState> *res;
+        ~Restore()
+        {
+            (void)hipStreamSynchronize(ctx->stream);
+            for (hipStream_t ls : ctx->lane_stream)
+                if (ls) (void)hipStreamSynchronize(ls);
+            for (SysState &st : *res) {
+                st.ham.release();
+                st.adj.release();
+                st.fpat.release();
+                for (const char *nm : {"fv0", "fzx", "fzy", "fzz"}) release_buf(ctx, (nm + st.ftag).c_str());
+            }
+            swap_state(ctx, *saved);
+        }
+    } restore{ctx, &saved, &res};
+    // every pair's system, field basis and shifts; its plan
+    const int S = lane_streams();
+    std::vector<BatchPlan> plans(n_pairs);
+    std::vector<spc_sim_params> sims(n_pairs);
+    std::vector<double> bmag(n_pairs);
+    for (int p = 0; p < n_pairs; ++p) {
+        ActiveSys act(ctx, &res[p]);
+        ctx->ftag = "@" + std::to_string(p);
+        if ((rc = system_set(ctx, &systems[p], false))) return rc;
+        sims[p] = *sim;
+        if (n_krylov && n_krylov[p] > 0) sims[p].n_krylov = n_krylov[p];
+        if (tols) sims[p].tol = tols[p];
+        if ((rc = check_sim(ctx, &sims[p]))) return rc;
+        if ((rc = build_field_basis(ctx))) return rc;
+        (void)ham_shifts(ctx, ctx->ham, ctx->N);
+        const DevSystem &hs = ctx->hsys;
+        bmag[p] = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
+        BatchPlan &pl = plans[p];
+        pl.m = sims[p].n_krylov;
+        pl.tol = sims[p].tol;
+        pl.sys_id = p;
+        pl.graph = true;
+        const int cap = max_orient_per_chunk(ctx, pl.m, M, n_orient, nout, pl.tol > 0.0);
+        const int want = orient_per_chunk && orient_per_chunk[p] > 0 ? std::min<int>(orient_per_chunk[p], n_orient)
+                                                                      : cap;
+        pl.opc = std::max(1, std::min(want, std::max(1, cap / S))); // S chunks in flight
+    }
+    // chunks: pairs by dimension, largest first
+    struct Chunk {
+        int p, o0, n;
+    };
+    std::vector<int> order(n_pairs);
+    for (int p = 0; p < n_pairs; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return res[a].N > res[b].N; });
+    std::vector<Chunk> chunks;
+    int Wmax = 0;
+    for (int p : order) {
+        for (int o0 = 0; o0 < n_orient; o0 += plans[p].opc)
+            chunks.push_back({p, o0, std::min(plans[p].opc, n_orient - o0)});
+        Wmax = std::max(Wmax, plans[p].opc * M);
+    }
+    const int nch = (int)chunks.size();
+    // shared inputs: each pair's field vectors, the lanes' seeds and initial populations
+    std::vector<double> f((size_t)3 * n_orient * n_pairs);
+    for (int p = 0; p < n_pairs; ++p)
+        for (int o = 0; o < n_orient; ++o) {
+            const double *d = dirs + 3 * o;
+            const double nd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (!(nd > 0.0)) return fail(ctx, SPC_ERR_INVALID, "orientation direction must be nonzero");
+            for (int x = 0; x < 3; ++x) f[((size_t)p * n_orient + o) * 3 + x] = bmag[p] * d[x] / nd;
+        }
+    std::vector<int64_t> sd((size_t)2 * Wmax);
+    for (int r = 0; r < Wmax; ++r) {
+        sd[2 * r] = sample_seeds[2 * (r % M)];
+        sd[2 * r + 1] = sample_seeds[2 * (r % M) + 1];
+    }
+    double2 th[4];
+    double pop[4];
+    theta_of(sim->init_state, th, pop);
+    std::vector<double> obs0((size_t)4 * Wmax);
+    for (int x = 0; x < 4; ++x)
+        for (int r = 0; r < Wmax; ++r) obs0[(size_t)x * Wmax + r] = pop[x];
+    BatchShared sh;
+    sh.lanes = Wmax;
+    double *d_acc = nullptr;
+    if ((rc = ensure(ctx, "pacc", (size_t)n_pairs * n_orient * 4 * nout, &d_acc))) return rc;
+    if ((rc = ensure(ctx, "pfields", f.size(), &sh.fields))) return rc;
+    if ((rc = ensure(ctx, "pseeds", sd.size(), &sh.seeds))) return rc;
+    if ((rc = ensure(ctx, "pobs0", obs0.size(), &sh.obs0))) return rc;
+    if ((rc = ensure(ctx, "pflags", (size_t)4 * nch, &sh.flags))) return rc;
+    if ((rc = ensure(ctx, "pbad", (size_t)nch, &sh.bad))) return rc;
+    if ((rc = ensure(ctx, "pcond", (size_t)kCondWords * nch, &sh.cond))) return rc;
+    hipStream_t s0 = ctx->stream;
+    HIP_TRY(hipMemsetAsync(d_acc, 0, sizeof(double) * n_pairs * n_orient * 4 * nout, s0));
+    HIP_TRY(hipMemsetAsync(sh.flags, 0, sizeof(int32_t) * 4 * nch, s0));
+    HIP_TRY(hipMemsetAsync(sh.bad, 0, sizeof(int32_t) * nch, s0));
+    HIP_TRY(hipMemsetAsync(sh.cond, 0, sizeof(unsigned long long) * kCondWords * nch, s0));
+    HIP_TRY(hipMemcpyAsync(sh.fields, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipMemcpyAsync(sh.seeds, sd.data(), sizeof(int64_t) * sd.size(), hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipMemcpyAsync(sh.obs0, obs0.data(), sizeof(double) * obs0.size(), hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipStreamSynchronize(s0));
+    // slots, events, pinned lane clocks
+    const int nslots = std::max(1, std::min(S, nch));
+    OrientSlot slots[kMaxLaneSets];
+    for (int j = 0; j < nslots; ++j) {
+        slots[j].idx = j;
+        if (j == 0) {
+            slots[j].s = s0;
+        } else {
+            if (!ctx->lane_stream[j - 1]) HIP_TRY(hipStreamCreate(&ctx->lane_stream[j - 1]));
+            slots[j].s = ctx->lane_stream[j - 1];
+        }
+    }
+    std::vector<hipEvent_t> evs((size_t)2 * nch + 1, nullptr);
+    struct Host {
+        double *lt = nullptr;
+        ~Host()
+        {
+            if (lt) (void)hipHostFree(lt);
+        }
+    } host[kMaxLaneSets];
+    struct Cleanup {
+        OrientSlot *slots;
+        int n;
+        std::vector<hipEvent_t> *evs;
+        ~Cleanup()
+        {
+            for (int j = 0; j < n; ++j) {
+                (void)hipStreamSynchronize(slots[j].s);
+                slots[j].release();
+            }
+            for (auto e : *evs)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } cleanup{slots, nslots, &evs};
+    for (auto &e : evs) HIP_TRY(hipEventCreate(&e));
+    for (int j = 0; j < nslots; ++j)
+        HIP_TRY(hipHostMalloc((void **)&host[j].lt, sizeof(double) * (size_t)Wmax, hipHostMallocDefault));
+    HIP_TRY(hipEventRecord(evs[0], s0));
+    for (int j = 1; j < nslots; ++j) HIP_TRY(hipStreamWaitEvent(slots[j].s, evs[0], 0));
+    int cur[kMaxLaneSets];
+    int64_t cur_rounds[kMaxLaneSets] = {};
+    for (int j = 0; j < nslots; ++j) cur[j] = -1;
+    int next = 0;
+    int64_t rounds = 0;
+    auto shared_of = [&](int p) {
+        BatchShared x = sh;
+        x.fields = sh.fields + (size_t)3 * p * n_orient;
+        return x;
+    };
+    auto acc_of = [&](int p) { return d_acc + (size_t)p * n_orient * 4 * nout; };
+    // the start of the next chunk on slot j; a chunk at tol = 0 is enqueued whole
+    auto start = [&](int j) -> int {
+        const int c = next++;
+        const Chunk &ch = chunks[c];
+        ActiveSys act(ctx, &res[ch.p]);
+        const BatchPlan &pl = plans[ch.p];
+        const BatchShared shp = shared_of(ch.p);
+        int r2;
+        if ((r2 = begin_orientation_chunk(ctx, slots[j], &sims[ch.p], pl, shp, M, c, ch.o0, ch.n, nout,
+                                          evs[1 + 2 * c])))
+            return r2;
+        if (pl.tol > 0.0) {
+            cur[j] = c;
+            cur_rounds[j] = 0;
+            return SPC_OK;
+        }
+        if ((r2 = enqueue_steps(ctx, slots[j], &sims[ch.p], pl, K))) return r2;
+        rounds += K;
+        return end_orientation_chunk(ctx, slots[j], shp, M, c, ch.o0, ch.n, nout, acc_of(ch.p), evs[2 + 2 * c]);
+    };
+    for (;;) {
+        for (int j = 0; j < nslots; ++j)
+            if (cur[j] < 0 && next < nch && (rc = start(j))) return rc;
+        bool any = false;
+        for (int j = 0; j < nslots; ++j) { // every slot's round in flight before the host waits for any
+            if (cur[j] < 0) continue;
+            any = true;
+            const Chunk &ch = chunks[cur[j]];
+            ActiveSys act(ctx, &res[ch.p]);
+            if ((rc = enqueue_steps(ctx, slots[j], &sims[ch.p], plans[ch.p], 1))) return rc;
+            HIP_TRY(hipMemcpyAsync(host[j].lt, slots[j].w.lane_t, sizeof(double) * slots[j].w.Mp,
+                                   hipMemcpyDeviceToHost, slots[j].s));
+            ++rounds;
+        }
+        if (!any) {
+            if (next >= nch) break;
+            continue;
+        }
+        for (int j = 0; j < nslots; ++j) {
+            if (cur[j] < 0) continue;
+            OrientSlot &sl = slots[j];
+            HIP_TRY(hipStreamSynchronize(sl.s));
+            bool done = true;
+            for (int r = 0; r < sl.w.Mp; ++r) done = done && !(host[j].lt[r] < sim->t_end);
+            if (++cur_rounds[j] > 4 * K + 16) return fail(ctx, SPC_ERR_INVALID, "time loop does not advance");
+            if (!done) continue;
+            const int c = cur[j];
+            const Chunk &ch = chunks[c];
+            ActiveSys act(ctx, &res[ch.p]);
+            if ((rc = end_orientation_chunk(ctx, sl, shared_of(ch.p), M, c, ch.o0, ch.n, nout, acc_of(ch.p),
+                                            evs[2 + 2 * c])))
+                return rc;
+            cur[j] = -1;
+        }
+    }
+    hipError_t es = hipSuccess;
+    for (int j = 0; j < nslots; ++j) {
+        const hipError_t e = hipStreamSynchronize(slots[j].s);
+        if (es == hipSuccess) es = e;
+    }
+    if (es != hipSuccess) return hip_fail(ctx, es, "hipStreamSynchronize");
+    std::vector<std::pair<double, double>> iv;
+    for (int c = 0; c < nch; ++c) {
+        float ta = 0.f, tb = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ta, evs[0], evs[1 + 2 * c]));
+        HIP_TRY(hipEventElapsedTime(&tb, evs[0], evs[2 + 2 * c]));
+        iv.emplace_back(ta, tb);
+    }
+    std::vector<int32_t> flags((size_t)4 * nch), bad((size_t)nch);
+    std::vector<unsigned long long> cond((size_t)kCondWords * nch);
+    HIP_TRY(hipMemcpy(flags.data(), sh.flags, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bad.data(), sh.bad, sizeof(int32_t) * bad.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cond.data(), sh.cond, sizeof(unsigned long long) * cond.size(), hipMemcpyDeviceToHost));
+    std::vector<double> acc((size_t)n_pairs * n_orient * 4 * nout);
+    HIP_TRY(hipMemcpy(acc.data(), d_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
+    std::vector<char> redo_kron(n_pairs, 0), redo_cols(n_pairs, 0);
+    for (int c = 0; c < nch; ++c) {
+        const int p = chunks[c].p;
+        if (flags[4 * c + 3]) return fail(ctx, SPC_ERR_INVALID, "time loop exceeded the output length");
+        if (flags[4 * c + 1] && plans[p].tol == 0.0)
+            return fail(ctx, SPC_ERR_UNSUPPORTED,
+                        "a trace vector needed a second step from one Krylov basis (|c_m|^2 = 0 at tol = 0); "
+                        "the batched path takes one step per basis");
+        if (bad[c]) redo_kron[p] = 1;
+        if (flags[4 * c] || cond_poor(&cond[(size_t)kCondWords * c])) redo_cols[p] = 1;
+    }
+    phi.assign((size_t)n_pairs * n_orient, 0.0);
+    for (int p = 0; p < n_pairs; ++p) {
+        if (redo_kron[p] || redo_cols[p]) { // this pair alone again, with the fallback
+            ActiveSys act(ctx, &res[p]);
+            BatchPlan p2 = plans[p];
+            p2.kron = !redo_kron[p];
+            p2.columns = redo_cols[p] != 0;
+            p2.slots = nslots;
+            std::vector<double> ps;
+            if ((rc = batch_orientations(ctx, &sims[p], sample_seeds, n_orient, dirs, p2, ps, nullptr))) return rc;
+            for (int o = 0; o < n_orient; ++o)
+                phi[(size_t)p * n_orient + o] = final_yield(ps.data() + (size_t)o * nout, nout, sim->dt, ctx->kS);
+            continue;
+        }
+        for (int o = 0; o < n_orient; ++o) {
+            const double *a = acc.data() + ((size_t)p * n_orient + o) * 4 * nout;
+            std::vector<double> ps(nout);
+            for (int64_t k = 0; k < nout; ++k) {
+                ps[k] = a[k] / (double)M;
+                if (!std::isfinite(ps[k])) return fail(ctx, SPC_ERR_BREAKDOWN, "non-finite population");
+            }
+            phi[(size_t)p * n_orient + o] = final_yield(ps.data(), nout, sim->dt, res[p].kS);
+        }
+    }
+    if (total_ms) *total_ms = union_ms(iv);
+    if (rounds_out) *rounds_out = rounds;
+    return SPC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int spc_pairs_batch(spc_ctx *ctx, int32_t n_pairs, const spc_spin_system *systems, const spc_sim_params *sim,
+                    const int64_t *sample_seeds, int32_t n_orient, const double *dirs, const int32_t *n_krylov,
+                    const double *tol, const int32_t *orient_per_chunk, double *Phi_S)
+{
+    if (!ctx) return SPC_ERR_INVALID;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n_pairs < 1 || !systems || !sim || !sample_seeds || !dirs || n_orient < 1 || !Phi_S)
+        return fail(ctx, SPC_ERR_INVALID, "NULL argument, no pairs or no orientations");
+    if (sim->vec_kind != SPC_VEC_SUZ && sim->vec_kind != SPC_VEC_PHASE)
+        return fail(ctx, SPC_ERR_INVALID, "vec_kind must be SPC_VEC_SUZ or SPC_VEC_PHASE");
+    if (sim->n_samples < 64 || sim->n_samples % 64)
+        return fail(ctx, SPC_ERR_UNSUPPORTED, "batched orientations need N_samples a multiple of 64");
+    std::vector<double> phi;
+    double ms = 0.0;
+    int64_t rounds = 0;
+    const int rc = pairs_batch(ctx, n_pairs, systems, sim, sample_seeds, n_orient, dirs, n_krylov, tol,
+                               orient_per_chunk, phi, &ms, &rounds);
+    if (rc) return rc;
+    std::copy(phi.begin(), phi.end(), Phi_S);
+    ctx->last_ms = ms;
+    ctx->last_steps = steps_taken(sim->t_end, sim->dt);
+    ctx->last_rebuilds = rounds;
+    return SPC_OK;
 }
 
 int spc_trace_sample(spc_ctx *ctx, const spc_sim_params *sim, const int64_t seed[2], const double *trace_vectors,
