@@ -31,6 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = ("state·steps/s (dim×M×steps) at 1e-10 yield err, %HBM roofline; grad/fwd time ratio")
+_T0 = time.perf_counter()
+
+
+def _log(msg: str) -> None:
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench [{time.perf_counter() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -108,9 +114,11 @@ def side_workloads(ctx, a):
     gradients.yield_gradient(warm.sys, warm.sim, B=warm.B, ctx=ctx)
     dynamics.trace_sampling_para(w4.sys, w4.sim, B=w4.B, ctx=ctx)
     fms, _, steps = dynamics.last_run_stats(ctx)
+    _log(f"gradient workload: forward {fms:.0f} ms")
     t0 = time.perf_counter()
     g = gradients.yield_gradient(w4.sys, w4.sim, B=w4.B, ctx=ctx)
     g_wall = time.perf_counter() - t0
+    _log(f"gradient workload: gradient {g.device_ms:.0f} ms")
     out["grad_fwd"] = {"ratio": round(g.device_ms / fms, 3), "forward_ms": round(fms, 1),
                        "gradient_ms": round(g.device_ms, 1), "gradient_wall_ms": round(1000 * g_wall, 1),
                        "steps": steps,
@@ -165,6 +173,8 @@ def side_workloads(ctx, a):
         ms, rounds, st = dynamics.last_run_stats(ctx)
         per = ms / (96 * st)
         same = plain_per_step(p.sys, m, tol, steps_r // 2, 3 * steps_r // 2) if tol > 0 else plain
+        _log(f"batched orientations N={p.N}: m={m} tol={tol:.0e} {opc}/chunk, {per:.4f} ms per orientation-step "
+             f"(plain {plain:.4f}, at the same settings {same:.4f})")
         classes[str(p.N)] = {"plain_ms_per_orientation_step": round(plain, 4),
                              "batched_ms_per_orientation_step": round(per, 4), "speedup": round(plain / per, 2),
                              "plain_same_settings_ms_per_orientation_step": round(same, 4),
@@ -183,6 +193,40 @@ def side_workloads(ctx, a):
                   f"tolerance and width spc_batch_tune_adaptive picks for the 10 us run (1e-10 yield tolerance "
                   f"against m=24, tol=0), device time of the chunks' rounds; plain_same_settings = the plain loop "
                   f"at the batch's order and tolerance; weighted by the 48 pairs' classes"}
+    # the 48 pairs in one call (every pair resident, chunks of all pairs sharing the
+    # streams) against the same pairs one orientation batch each, at each class's
+    # tuned settings, on a bounded sample: n_dir orientations x steps_p steps
+    n_dir, steps_p = 8, 64
+    allp = batch.radical_pairs()
+    sub = dirs[np.linspace(0, 95, n_dir).astype(int)]
+    cl = [classes[str(p.N)] for p in allp]
+    sp = batch.orientation_sim(T_ns=float(steps_p) - 0.5)
+    per_pair_ms, per_pair_wall = 0.0, 0.0
+    for i, (p, c) in enumerate(zip(allp, cl)):
+        if i % 12 == 0:
+            _log(f"pairs one by one: {i}/{len(allp)}")
+        s1 = copy.deepcopy(sp)
+        s1.N_krylov, s1.tol = c["krylov"], c["tol"]
+        t0 = time.perf_counter()
+        batch.orientation_yields(p.sys, s1, sub, B=0.05, orient_per_chunk=min(c["orient_per_chunk"], n_dir), ctx=ctx)
+        per_pair_wall += time.perf_counter() - t0
+        per_pair_ms += dynamics.last_run_stats(ctx)[0]
+    _log("pairs in one call")
+    t0 = time.perf_counter()
+    batch.pairs_yields([p.sys for p in allp], sp, sub, B=[0.05] * len(allp), n_krylov=[c["krylov"] for c in cl],
+                       tol=[c["tol"] for c in cl], orient_per_chunk=[min(c["orient_per_chunk"], n_dir) for c in cl],
+                       ctx=ctx)
+    one_wall = time.perf_counter() - t0
+    one_ms = dynamics.last_run_stats(ctx)[0]
+    out["pairs_one_call"] = {
+        "device_ms": round(one_ms, 1), "pair_by_pair_device_ms": round(per_pair_ms, 1),
+        "speedup_device": round(per_pair_ms / one_ms, 2), "wall_s": round(one_wall, 2),
+        "pair_by_pair_wall_s": round(per_pair_wall, 2),
+        "sample": f"the 48 pairs x {n_dir} orientations x {steps_p} steps, each pair at its class's tuned order, "
+                  f"tolerance and width; one spc_pairs_batch call against one spc_orientation_batch per pair; device "
+                  f"time = the union of the chunks' intervals (set-up of the systems excluded), wall time includes "
+                  f"each pair's Hamiltonian, field basis and Newton shifts"}
+    _log(f"pairs: one call {one_ms:.0f} ms, one by one {per_pair_ms:.0f} ms (device)")
     # semiclassical
     w5 = workloads.semiclassical_48()
     sc_dynamics.run(w5.a1, w5.a2, w5.m1, w5.m2, ctx=ctx, **{**w5.kwargs(), "n_steps_": 10})  # warm
@@ -240,10 +284,12 @@ def main(argv=None):
         return taken, outs
 
     # warmup (allocations, code objects), then exactly K timed steps
+    _log(f"N={N} M={M} m={m}: warm-up")
     run(max(a.warmup, 1))
     t0 = time.perf_counter()
     taken, outs = run(a.steps)
     wall = time.perf_counter() - t0
+    _log(f"timed {taken} steps: {1000.0 * wall / taken:.3f} ms per step")
     dev_ms, rebuilds, _ = dynamics.last_run_stats(ctx)
     chunks = rebuilds // max(taken, 1)  # lane chunks run concurrently (one rebuild each per step)
     value = N * M * taken / wall
@@ -315,8 +361,11 @@ def main(argv=None):
                         "frac": round(value * 16 * (m * m + 8 * m + 1) / 1e9 / HBM_PEAK_GBS, 4),
                         "loop_frac": round(value * 16 * (m * m + 6 * m) / 1e9 / HBM_PEAK_GBS, 4)}}
 
+    _log("cpu baseline")
     cpu = None if a.no_cpu_baseline else cpu_baseline(w, a.cpu_seconds)
+    _log("side workloads")
     extras = {} if a.no_extras else side_workloads(ctx, a)
+    _log("done")
     line = {"metric": METRIC, "value": value, "unit": "state·steps/s", "n_gpus": 1, "steps": int(taken),
             "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "complex128", "data": "synthetic",

@@ -3390,7 +3390,7 @@ int spc_batch_tune_adaptive(spc_ctx *ctx, const spc_sim_params *sim, const int64
     spc_sim_params ts = *sim;
     ts.n_krylov = m_best;
     ts.tol = tol_best;
-    const int64_t tsteps = tol_best > 0.0 ? pilot : std::min<int64_t>(K, 6);
+    const int64_t tsteps = tol_best > 0.0 ? std::min<int64_t>(K, 64) : std::min<int64_t>(K, 6); // a few rounds
     ts.t_end = sim->dt * (double)tsteps - 0.5 * sim->dt;
     const int64_t tout = spc_n_steps(ts.t_end, ts.dt) + 1;
     const int cap = max_orient_per_chunk(ctx, m_best, sim->n_samples, n_orient, tout, tol_best > 0.0);
