@@ -278,6 +278,7 @@ __device__ __forceinline__ void axis_tan(const double *S, const double *dS, cons
 // partd[st][wave].  With BAL the electron-2 slots start at NL/2 (p.nl1 =
 // NL/2), known at compile time: the unrolled slot loops then pick each
 // slot's electron without selects.
+constexpr int kRing = 8; // steps per flush of a wave's partial sums (8 steps x 2 or 4 values = 16 or 32 sums)
 template <int NL, int L, bool TAN, bool BAL>
 __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__restrict__ seeds,
                                                    double *__restrict__ part, double *__restrict__ partd,
@@ -298,6 +299,16 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
     const double dwz[3] = {p.dw[0], p.dw[1], p.dw[2]};
     const double th = 0.5 * p.dt;
     const int nl1 = BAL ? NL / 2 : p.nl1;
+    // per wave: the lane values of the last kRing steps (NV values per step)
+    constexpr int NV = TAN ? 4 : 2, NP = 64 / (kRing * NV);
+    static_assert(NP >= 1 && kRing * NV * NP == 64, "one sum per lane group");
+    __shared__ double ring[kScThreads / kWave][kRing][NV][kWave];
+    const int wv = threadIdx.x >> 6;
+    auto ring_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 
     for (int64_t grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
         const int64_t sample = grp * (kScThreads / L) + threadIdx.x / L;
@@ -405,29 +416,31 @@ __global__ __launch_bounds__(kScThreads) void k_sc(ScParams p, const int64_t *__
                     electron_half(x, kA, kB, r1, r2);
                 }
             }
-            double v = counts ? p0 * ps_of(x) : 0.0, u = counts ? p0 * x[15] : 0.0;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                v += __shfl_xor(v, o);
-                u += __shfl_xor(u, o);
-            }
-            if (lane == 0) {
-                double2 *pp = reinterpret_cast<double2 *>(part) + st * nwaves + gwave;
-                const double2 old = *pp;
-                *pp = make_double2(old.x + v, old.y + u);
-            }
+            // this step's lane values into the wave's ring; every kRing steps (and at
+            // the end) the wave's sums go to part/partd, one read-modify-write per
+            // sum, so a load's latency is paid once per kRing steps
+            const int slot = (int)(st % kRing);
+            ring[wv][slot][0][lane] = counts ? p0 * ps_of(x) : 0.0;
+            ring[wv][slot][1][lane] = counts ? p0 * x[15] : 0.0;
             if constexpr (TAN) {
-                double vd = counts ? p0 * ps_of(dx) : 0.0, ud = counts ? p0 * dx[15] : 0.0;
+                ring[wv][slot][2][lane] = counts ? p0 * ps_of(dx) : 0.0;
+                ring[wv][slot][3][lane] = counts ? p0 * dx[15] : 0.0;
+            }
+            if (slot == kRing - 1 || st == n_steps) {
+                ring_sync();
+                // sum q = lane / NP (step q / NV, value q % NV) over the NP parts of 64 / NP lanes
+                const int q = lane / NP, part_i = lane % NP, j = q / NV, c = q % NV;
+                double acc = 0.0;
 #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    vd += __shfl_xor(vd, o);
-                    ud += __shfl_xor(ud, o);
+                for (int i = 0; i < 64 / NP; ++i) acc += ring[wv][j][c][part_i * (64 / NP) + i];
+#pragma unroll
+                for (int o = 1; o < NP; o <<= 1) acc += __shfl_xor(acc, o);
+                const int64_t st0 = st - slot;
+                if (part_i == 0 && j <= slot) {
+                    double *base = (c < 2 ? part : partd) + 2 * ((st0 + j) * nwaves + gwave) + (c & 1);
+                    *base += acc;
                 }
-                if (lane == 0) {
-                    double2 *pp = reinterpret_cast<double2 *>(partd) + st * nwaves + gwave;
-                    const double2 old = *pp;
-                    *pp = make_double2(old.x + vd, old.y + ud);
-                }
+                ring_sync(); // the ring is rewritten from the next step on
             }
         }
     }
