@@ -13,8 +13,10 @@
 #include <complex>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -2111,11 +2113,18 @@ int pairs_batch(spc_ctx *ctx, int n_pairs, const spc_spin_system *systems, const
             swap_state(ctx, *saved);
         }
     } restore{ctx, &saved, &res};
-    // every pair's system, field basis and shifts; its plan
+    // every pair's system, field basis and shifts; its plan.  The shifts' host
+    // Lanczos runs (the longest part of a pair's set-up) go to host threads
+    // once every pair's CSR has been copied back.
     const int S = lane_streams();
     std::vector<BatchPlan> plans(n_pairs);
     std::vector<spc_sim_params> sims(n_pairs);
     std::vector<double> bmag(n_pairs);
+    struct HostCsr {
+        std::vector<int32_t> rp, cl;
+        std::vector<double2> vl;
+    };
+    std::vector<HostCsr> csr(n_pairs);
     for (int p = 0; p < n_pairs; ++p) {
         ActiveSys act(ctx, &res[p]);
         ctx->ftag = "@" + std::to_string(p);
@@ -2125,7 +2134,20 @@ int pairs_batch(spc_ctx *ctx, int n_pairs, const spc_spin_system *systems, const
         if (tols) sims[p].tol = tols[p];
         if ((rc = check_sim(ctx, &sims[p]))) return rc;
         if ((rc = build_field_basis(ctx))) return rc;
-        (void)ham_shifts(ctx, ctx->ham, ctx->N);
+        const HamDev &hd = ctx->ham;
+        if (hd.shifts.empty() && hd.rowptr && hd.nnz > 0) {
+            HostCsr &hc = csr[p];
+            hc.rp.resize((size_t)ctx->N + 1);
+            hc.cl.resize((size_t)hd.nnz);
+            hc.vl.resize((size_t)hd.nnz);
+            HIP_TRY(hipMemcpyAsync(hc.rp.data(), hd.rowptr, sizeof(int32_t) * hc.rp.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipMemcpyAsync(hc.cl.data(), hd.col, sizeof(int32_t) * hc.cl.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipMemcpyAsync(hc.vl.data(), hd.val, sizeof(double2) * hc.vl.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
         const DevSystem &hs = ctx->hsys;
         bmag[p] = std::sqrt(hs.w[0] * hs.w[0] + hs.w[1] * hs.w[1] + hs.w[2] * hs.w[2]);
         BatchPlan &pl = plans[p];
@@ -2137,6 +2159,29 @@ int pairs_batch(spc_ctx *ctx, int n_pairs, const spc_spin_system *systems, const
         const int want = orient_per_chunk && orient_per_chunk[p] > 0 ? std::min<int>(orient_per_chunk[p], n_orient)
                                                                       : cap;
         pl.opc = std::max(1, std::min(want, std::max(1, cap / S))); // S chunks in flight
+    }
+    {
+        std::vector<std::vector<double>> sh(n_pairs);
+        std::atomic<int> nextp{0};
+        auto work = [&]() {
+            for (int p; (p = nextp++) < n_pairs;) {
+                const HostCsr &hc = csr[p];
+                if (hc.rp.empty()) continue;
+                sh[p].assign(SPC_MAX_KRYLOV, 0.0);
+                if (newton_shifts_csr((int64_t)hc.rp.size() - 1, hc.rp.data(), hc.cl.data(), hc.vl.data(),
+                                      SPC_MAX_KRYLOV, sh[p].data()))
+                    sh[p].clear();
+            }
+        };
+        const int nt = std::max(1, std::min<int>({n_pairs, (int)std::thread::hardware_concurrency(), 16}));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+        for (int p = 0; p < n_pairs; ++p) {
+            if (!sh[p].empty()) res[p].ham.shifts = std::move(sh[p]);
+            csr[p] = HostCsr{};
+        }
     }
     // chunks: pairs by dimension, largest first
     struct Chunk {
