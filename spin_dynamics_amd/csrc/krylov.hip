@@ -2586,12 +2586,19 @@ size_t sstep_partials(int64_t N, int m, int S, int Mp)
     return std::max(mono, newton) * (size_t)Mp;
 }
 
+// Elements between consecutive basis vectors: N Mp plus a pad of 256 bytes
+// (SPC_BASIS_PAD_BYTES sets it; read once).  Without it the vectors of the
+// 64-lane workload lie 9 x 4 MiB apart, so the same row of every vector falls
+// on the same HBM channels, and the passes that read 25 vectors in 64-byte
+// runs (the Newton Gram pass) queue on them: 295 -> 240 us per Gram pass and
+// 4.33-4.39 -> 4.21-4.27 ms per step at N = 36 864, M = 256 (MI355X; pads of
+// 128-768 bytes alike, 1-3 KiB 255 us).
 int64_t basis_stride(int64_t N, int Mp)
 {
     static int64_t pad = -1;
     if (pad < 0) {
         const char *e = std::getenv("SPC_BASIS_PAD_BYTES");
-        pad = e ? std::max<long long>(0, std::atoll(e)) : 0;
+        pad = e ? std::max<long long>(0, std::atoll(e)) : 256;
     }
     return N * (int64_t)Mp + (pad + 15) / 16;
 }
