@@ -502,16 +502,20 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
 // instead of L2.  Needs the last Q + BD nuclei spin-1/2.  Same summation order
 // as k_spmm_kron (bit-identical products).  rpg = tiles per workgroup.
 // DOTS = false: v only (the block Arnoldi's chain products need no dots).
-template <int G, int Q, int NW, bool DOTS = true>
+// MPC > 0: the lane stride Mp as a compile-time constant (it must equal the
+// argument), so that the row offsets of a gathered index become immediate
+// offsets of its loads and the neighbour shifts single scalar shifts.
+template <int G, int Q, int NW, bool DOTS = true, int MPC = 0>
 __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__restrict__ kd,
                                                             const uint64_t *__restrict__ kmask,
                                                             const double2 *__restrict__ diags,
                                                             const double2 *__restrict__ gens, int64_t dstride,
                                                             int64_t gstride, int chunks_per_set,
                                                             const double2 *__restrict__ u, double2 *__restrict__ v,
-                                                            double2 *__restrict__ part, int64_t Z, int Mp, int rpg,
+                                                            double2 *__restrict__ part, int64_t Z, int Mp_arg, int rpg,
                                                             int nrb, int lslow, double sig)
 {
+    const int Mp = MPC > 0 ? MPC : Mp_arg;
     constexpr int W = kWave >> Q;
     constexpr int NG = 1 << Q;
     constexpr int BD = NW == 16 ? 4 : NW == 8 ? 3 : NW == 4 ? 2 : NW == 2 ? 1 : 0;
@@ -1575,7 +1579,9 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
                   {k_spmm_kron_tiles<2, 3, 8, false>, k_spmm_kron_tiles<3, 3, 8, false>},
                   {k_spmm_kron_tiles<2, 3, 16, false>, k_spmm_kron_tiles<3, 3, 16, false>}}}};
             const int ti = nw == 4 ? 0 : nw == 8 ? 1 : 2;
-            hipLaunchKernelGGL(tfn[w.no_dots ? 1 : 0][q - 2][ti][std::min(group, 3) - 2], dim3(nrb * nsl), dim3(64 * nw),
+            TileFn fn = tfn[w.no_dots ? 1 : 0][q - 2][ti][std::min(group, 3) - 2];
+            if (w.no_dots && w.Mp == 64 && q == 3 && nw == 4 && group == 2) fn = k_spmm_kron_tiles<2, 3, 4, false, 64>;
+            hipLaunchKernelGGL(fn, dim3(nrb * nsl), dim3(64 * nw),
                                0, s, w.kron,
                                w.kmask, w.kdiag, w.kgen,
                                w.kdiag_stride, w.kgen_stride, std::max(w.chunks_per_bval, 1), w.u, w.v, w.part, w.Z,
