@@ -66,6 +66,7 @@ struct HamDev {
     KronDev *kron = nullptr; // Kronecker form (when H has it exactly; else null)
     double2 *kdiag = nullptr, *kgen = nullptr;
     int kq = 0; // in-wave nuclei of the Kronecker product (kron_inwave)
+    int kne2p = 0; // 1 + electron-2 nuclei (KrylovWork::kne2p)
     int slow_mult = 1;        // multiplicity of the slowest nucleus (digit of largest stride)
     int64_t slow_stride = 0;  // its stride in nuclear indices
     uint64_t *kmask = nullptr; // [Z][3] neighbour masks of the Kronecker product
@@ -379,6 +380,7 @@ void set_ham(KrylovWork &w, const HamDev &h)
     w.kgen = kr ? h.kgen : nullptr;
     w.kdiag_stride = w.kgen_stride = 0;
     w.kq = kr ? h.kq : 0;
+    w.kne2p = kr ? h.kne2p : 0;
     w.slow_mult = h.slow_mult;
     w.slow_stride = h.slow_stride;
     w.kmask = kr ? h.kmask : nullptr;
@@ -446,6 +448,7 @@ int build_ham(spc_ctx *ctx, const DevSystem &S, HamDev &h)
         return SPC_OK;
     }
     h.kq = kron_inwave(hk);
+    h.kne2p = 1 + __builtin_popcountll(hk.e2mask);
     h.slow_mult = hk.nn > 0 ? hk.mult[0] : 1;
     h.slow_stride = hk.nn > 0 ? hk.stride[0] : 0;
     HIP_TRY(hipMalloc(&h.kron, sizeof(KronDev)));
@@ -1288,6 +1291,7 @@ int slot_setup(spc_ctx *ctx, OrientSlot &sl, const BatchPlan &plan, int M, int n
         w.kdiag = kd;
         w.kgen = kg;
         w.kq = kron_inwave(kron_of(hs));
+        w.kne2p = 1 + __builtin_popcountll(kron_of(hs).e2mask);
         w.kmask = ctx->fpat.kmask;
         w.kdiag_stride = Z * 16;
         w.kgen_stride = (int64_t)std::max(hs.nn, 1) * 24;

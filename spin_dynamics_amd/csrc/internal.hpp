@@ -189,6 +189,7 @@ struct KrylovWork {
     const double2 *kgen;     // [nn][2][3][4]
     int64_t kdiag_stride, kgen_stride;
     int kq;                  // trailing spin-1/2 nuclei of strides 1, 2, 4, .. (kron_inwave, 0..8)
+    int kne2p;               // 1 + the number of electron-2 nuclei (they are the last); 0: not known
     int slow_mult;           // multiplicity and stride (nuclear indices) of the slowest nucleus
     int64_t slow_stride;
     const uint64_t *kmask;   // [Z][3] neighbour masks (kron_masks), or null

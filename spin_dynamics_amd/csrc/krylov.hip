@@ -505,7 +505,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
 // MPC > 0: the lane stride Mp as a compile-time constant (it must equal the
 // argument), so that the row offsets of a gathered index become immediate
 // offsets of its loads and the neighbour shifts single scalar shifts.
-template <int G, int Q, int NW, bool DOTS = true, int MPC = 0>
+// NE2 >= 0: the last NE2 of the in-tile nuclei (counted from the last) are
+// electron 2's, the others electron 1's (electron 1's nuclei come first), so
+// every in-tile flip has a compile-time 2x2 layout; and the in-wave flips run
+// without divergence: a lane group whose flip falls on the other side of the
+// diagonal block reads a row of zeros (adding M 0 changes at most the sign of
+// a zero), where a branch on the lane group had made the compiler copy y at
+// every merge.
+template <int G, int Q, int NW, bool DOTS = true, int MPC = 0, int NE2 = -1>
 __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__restrict__ kd,
                                                             const uint64_t *__restrict__ kmask,
                                                             const double2 *__restrict__ diags,
@@ -523,6 +530,10 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
     __shared__ double2 red[NW - 1][2][W];
     __shared__ double2 dl[NW][NG * 16];
     __shared__ double2 xl[NW][NG][4][W]; // the workgroup's own rows, for the in-tile flips
+    __shared__ double2 xz[4][W];         // zeros (NE2 >= 0)
+    if (NE2 >= 0 && threadIdx.x < 4 * W) (&xz[0][0])[threadIdx.x] = cz(0, 0); // read after the tile's barrier
+    // the electron of the in-tile nucleus k at position pos from the last
+    auto elec_of = [&](int pos, int k) { return NE2 >= 0 ? (pos < NE2 ? 1 : 0) : kd->elec[k]; };
     const int nsl = Mp / W;
     int rb, sl;
     if ((nsl & 7) == 0) {
@@ -574,14 +585,20 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
                 double2 xp[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) xp[f] = xl[wave ^ (1 << b)][g][f][lane];
-                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, kd->elec[k], xp, y);
+                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, elec_of(Q + b, k), xp, y);
             }
         }
         // the in-wave nuclei (bit j of g: partner lane group g ^ 2^j), k ascending
 #pragma unroll
         for (int j = Q - 1; j >= 0; --j) {
             const int k = nn - 1 - j;
-            if ((g >> j) & 1) {
+            if constexpr (NE2 >= 0) {
+                const double2 *src = ((g >> j) & 1) ? &xl[wave][g ^ (1 << j)][0][lane] : &xz[0][lane];
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = src[f * W];
+                kron_apply(gen + ((int64_t)k * 6 + 4) * 4, elec_of(j, k), xp, y);
+            } else if ((g >> j) & 1) {
                 double2 xp[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
@@ -603,7 +620,13 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
 #pragma unroll
         for (int j = 0; j < Q; ++j) { // in-wave, k descending
             const int k = nn - 1 - j;
-            if (!((g >> j) & 1)) {
+            if constexpr (NE2 >= 0) {
+                const double2 *src = ((g >> j) & 1) ? &xz[0][lane] : &xl[wave][g ^ (1 << j)][0][lane];
+                double2 xp[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) xp[f] = src[f * W];
+                kron_apply(gen + ((int64_t)k * 6) * 4, elec_of(j, k), xp, y);
+            } else if (!((g >> j) & 1)) {
                 double2 xp[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) xp[f] = xl[wave][g ^ (1 << j)][f][lane];
@@ -617,7 +640,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
                 double2 xp[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) xp[f] = xl[wave ^ (1 << b)][g][f][lane];
-                kron_apply(gen + ((int64_t)k * 6) * 4, kd->elec[k], xp, y);
+                kron_apply(gen + ((int64_t)k * 6) * 4, elec_of(Q + b, k), xp, y);
             }
         }
         // neighbours n + stride_k of the uniform nuclei, k descending
@@ -1580,7 +1603,17 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
                   {k_spmm_kron_tiles<2, 3, 16, false>, k_spmm_kron_tiles<3, 3, 16, false>}}}};
             const int ti = nw == 4 ? 0 : nw == 8 ? 1 : 2;
             TileFn fn = tfn[w.no_dots ? 1 : 0][q - 2][ti][std::min(group, 3) - 2];
-            if (w.no_dots && w.Mp == 64 && q == 3 && nw == 4 && group == 2) fn = k_spmm_kron_tiles<2, 3, 4, false, 64>;
+            if (w.no_dots && w.Mp == 64 && q == 3 && nw == 4 && group == 2) {
+                // the chain product at 64 lanes; with the electrons of the 5 in-tile nuclei known (kne2p)
+                static const TileFn c64[7] = {k_spmm_kron_tiles<2, 3, 4, false, 64>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 0>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 1>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 2>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 3>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 4>,
+                                              k_spmm_kron_tiles<2, 3, 4, false, 64, 5>};
+                fn = c64[w.kne2p > 0 ? 1 + std::min(w.kne2p - 1, 5) : 0];
+            }
             hipLaunchKernelGGL(fn, dim3(nrb * nsl), dim3(64 * nw),
                                0, s, w.kron,
                                w.kmask, w.kdiag, w.kgen,
