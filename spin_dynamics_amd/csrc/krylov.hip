@@ -511,7 +511,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
 // without divergence: a lane group whose flip falls on the other side of the
 // diagonal block reads a row of zeros (adding M 0 changes at most the sign of
 // a zero), where a branch on the lane group had made the compiler copy y at
-// every merge.
+// every merge; and the thread's own rows go to LDS as soon as they arrive and
+// are read back there, so no registers hold them over the neighbour walks
+// (76 VGPRs instead of 88: 6 waves per SIMD instead of 5).
 template <int G, int Q, int NW, bool DOTS = true, int MPC = 0, int NE2 = -1>
 __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__restrict__ kd,
                                                             const uint64_t *__restrict__ kmask,
@@ -570,12 +572,16 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         double2 x0[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) x0[f] = u[rowb + toff + f * Mp];
+        if constexpr (NE2 >= 0) // this thread's rows go to LDS at once and are read back there: no registers
+#pragma unroll           // hold them over the neighbour walks
+            for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
         double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
         // neighbours n - stride_k of the uniform nuclei, k ascending
         const NbrMasks nm = masks_at(kmask, kd, nb, nuni);
         kron_neighbours<G, true>(kd, gen, u, rowb + toff, M4, Mp, nm.minus, nm.one, y);
+        if constexpr (NE2 < 0)
 #pragma unroll
-        for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
+            for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
         __syncthreads();
         // the tile nuclei (bit b of the wave: partner wave ^ 2^b), k ascending
 #pragma unroll
@@ -607,13 +613,16 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         }
         {
             const double2 *D = &dl[wave][g * 16];
+            double2 xd[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) xd[f] = NE2 >= 0 ? xl[wave][g][f][lane] : x0[f];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 double2 de[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) de[f] = D[4 * e + f];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) y[e] = cfma(de[f], x0[f], y[e]);
+                for (int f = 0; f < 4; ++f) y[e] = cfma(de[f], xd[f], y[e]);
                 __asm__ volatile("" ::: "memory");
             }
         }
@@ -647,7 +656,7 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         kron_neighbours<G, false>(kd, gen, u, rowb + toff, M4, Mp, nm.plus, nm.one, y);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            y[e] = shift_sub(y[e], sig, x0[e]);
+            y[e] = shift_sub(y[e], sig, NE2 >= 0 ? xl[wave][g][e][lane] : x0[e]);
             // the column form's v is streamed past L2, which keeps u's rows; a
             // chain product's output is the next product's input (cached:
             // 36.5 -> 32.7 us per 64-lane product at N = 36 864)
