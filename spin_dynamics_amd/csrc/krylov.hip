@@ -237,6 +237,13 @@ __device__ __forceinline__ NbrMasks masks_at(const uint64_t *__restrict__ kmask,
     return NbrMasks{kmask[3 * (int64_t)n] & keep, kmask[3 * (int64_t)n + 1] & keep, kmask[3 * (int64_t)n + 2] & keep};
 }
 
+// The masks of n from the table (which the caller has).
+__device__ __forceinline__ NbrMasks masks_table(const uint64_t *__restrict__ kmask, int n, int kmax)
+{
+    const uint64_t keep = kmax >= 64 ? ~0ull : (1ull << kmax) - 1;
+    return NbrMasks{kmask[3 * (int64_t)n] & keep, kmask[3 * (int64_t)n + 1] & keep, kmask[3 * (int64_t)n + 2] & keep};
+}
+
 // y += the flips of the existing neighbours in `mask`, G at a time: minus
 // neighbours n - stride_k in ascending k, plus neighbours n + stride_k in
 // descending k (ascending columns either way).  The G rows of a batch are
@@ -505,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_kron_slices(const KronDev *__re
 // MPC > 0: the lane stride Mp as a compile-time constant (it must equal the
 // argument), so that the row offsets of a gathered index become immediate
 // offsets of its loads and the neighbour shifts single scalar shifts.
-// NE2 >= 0: the last NE2 of the in-tile nuclei (counted from the last) are
+// NE2 >= 0 (with the neighbour-mask table): the last NE2 of the in-tile nuclei (counted from the last) are
 // electron 2's, the others electron 1's (electron 1's nuclei come first), so
 // every in-tile flip has a compile-time 2x2 layout; and the in-wave flips run
 // without divergence: a lane group whose flip falls on the other side of the
@@ -566,6 +573,8 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
         const int64_t ti = (tj % lslow) * zs + tj / lslow;
         const int nb = (int)((((ti << BD) + wave)) << Q); // this wave's quad
         const int64_t rowb = (int64_t)nb * M4;
+        // the neighbour masks (scalar loads) first: their latency then overlaps the rows' loads
+        const NbrMasks nm = NE2 >= 0 ? masks_table(kmask, nb, nuni) : masks_at(kmask, kd, nb, nuni);
 #pragma unroll
         for (int i = 0; i < (NG * 16 + kWave - 1) / kWave; ++i)
             if (i * kWave + t < NG * 16) dl[wave][i * kWave + t] = diag[(int64_t)nb * 16 + i * kWave + t];
@@ -577,7 +586,6 @@ __global__ __launch_bounds__(64 * NW) void k_spmm_kron_tiles(const KronDev *__re
             for (int f = 0; f < 4; ++f) xl[wave][g][f][lane] = x0[f];
         double2 y[4] = {cz(0, 0), cz(0, 0), cz(0, 0), cz(0, 0)};
         // neighbours n - stride_k of the uniform nuclei, k ascending
-        const NbrMasks nm = masks_at(kmask, kd, nb, nuni);
         kron_neighbours<G, true>(kd, gen, u, rowb + toff, M4, Mp, nm.minus, nm.one, y);
         if constexpr (NE2 < 0)
 #pragma unroll
@@ -1621,7 +1629,7 @@ hipError_t spmm_launch(KrylovWork &w, hipStream_t s)
                                               k_spmm_kron_tiles<2, 3, 4, false, 64, 3>,
                                               k_spmm_kron_tiles<2, 3, 4, false, 64, 4>,
                                               k_spmm_kron_tiles<2, 3, 4, false, 64, 5>};
-                fn = c64[w.kne2p > 0 ? 1 + std::min(w.kne2p - 1, 5) : 0];
+                fn = c64[w.kne2p > 0 && w.kmask ? 1 + std::min(w.kne2p - 1, 5) : 0];
             }
             hipLaunchKernelGGL(fn, dim3(nrb * nsl), dim3(64 * nw),
                                0, s, w.kron,
