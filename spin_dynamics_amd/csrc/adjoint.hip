@@ -67,13 +67,18 @@ __global__ __launch_bounds__(kBlock) void k_combine(const double2 *__restrict__ 
                 if (o < nout) acc[o] = cfma(cs[o][k][lr], q, acc[o]);
         }
         if (src && (e == 1 || e == 2)) {
-            const double2 d = csub(src[(4 * n + 1) * (int64_t)Mp + r], src[(4 * n + 2) * (int64_t)Mp + r]);
+            const double2 d = out.src_diff ? src[n * (int64_t)Mp + r]
+                                           : csub(src[(4 * n + 1) * (int64_t)Mp + r], src[(4 * n + 2) * (int64_t)Mp + r]);
             const double f = (e == 1 ? 0.5 : -0.5) * omega;
             acc[0] = cadd(acc[0], cscale(f, d));
         }
+        if (out.diff0) { // row 4n+2's value (lane group e = 2) to the row 4n+1 thread (e = 1) of the same lane
+            const double2 o2 = cz(__shfl_down(acc[0].x, kGroup), __shfl_down(acc[0].y, kGroup));
+            if (e == 1 && out.y[0]) out.y[0][n * (int64_t)Mp + r] = live ? csub(acc[0], o2) : cz(0, 0);
+        }
 #pragma unroll
         for (int o = 0; o < kCombineOut; ++o)
-            if (o < nout && out.y[o]) out.y[o][off] = live ? acc[o] : cz(0, 0);
+            if (o < nout && out.y[o] && !(o == 0 && out.diff0)) out.y[o][off] = live ? acc[o] : cz(0, 0);
     }
 }
 

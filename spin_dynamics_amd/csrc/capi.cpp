@@ -2912,14 +2912,17 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     }
     const size_t budget = free_b / 10 * 8;
     const size_t V = (size_t)N * Mp * sizeof(double2);
+    // a stored step: psi_n's singlet difference psi[4n+1] - psi[4n+2] (all the adjoint reads of psi_n: V / 4)
+    // and the kGL states a_n(s_j)
+    const size_t VS = V / 4 + V * kGL;
     const size_t base = V * ((size_t)m + 3 + kGL + (overlap ? (size_t)m + 1 : 0));
     int64_t L = checkpoint_every > 0 ? checkpoint_every : 64;
     auto need = [&](int64_t l, int64_t kp) {
-        return base + V * ((size_t)((ntop - kp + l - 1) / l) + (size_t)(nbuf * l + kp) * (kGL + 1));
+        return base + V * (size_t)((ntop - kp + l - 1) / l) + VS * (size_t)(nbuf * l + kp);
     };
     while (L > 1 && need(L, 0) > budget) L /= 2;
     if (need(L, 0) > budget) return fail(ctx, SPC_ERR_OOM, "not enough device memory for the adjoint");
-    int64_t keep = std::min<int64_t>(ntop, (int64_t)((budget - need(L, 0)) / (V * (kGL + 1))));
+    int64_t keep = std::min<int64_t>(ntop, (int64_t)((budget - need(L, 0)) / VS));
     if (const char *env = std::getenv("SPC_ADJOINT_KEEP")) // cap on kept steps (tuning and tests)
         keep = std::min<int64_t>(keep, std::max<int64_t>(0, std::atoll(env)));
     const int64_t kstart = ntop - keep;         // steps [kstart, ntop) keep their vectors
@@ -2990,7 +2993,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
     if ((rc = ensure(ctx, "adj_lam", (size_t)N * Mp, &lam))) return rc;
     if ((rc = ensure(ctx, "adj_b", (size_t)kGL * N * Mp, &bvec))) return rc;
     if ((rc = ensure(ctx, "adj_ckpt", (size_t)std::max<int64_t>(nseg, 1) * N * Mp, &ckpt))) return rc;
-    if ((rc = ensure(ctx, "adj_store", (size_t)(nbuf * L + keep) * (kGL + 1) * N * Mp, &store))) return rc;
+    if ((rc = ensure(ctx, "adj_store", (size_t)(nbuf * L + keep) * (VS / sizeof(double2)), &store))) return rc;
     if ((rc = ensure(ctx, "adj_C", (size_t)kMaxTaus * m * Mp, &Cb))) return rc;
     if ((rc = ensure(ctx, "adj_CF", (size_t)kMaxTaus * m * Mp, &Cf))) return rc;
     if ((rc = ensure(ctx, "adj_S", (size_t)Z * nslot * 16, &Sg))) return rc;
@@ -3016,9 +3019,9 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
         if (wb.sstep >= 2 && (rc = ensure(ctx, "adjb_sbuf", (size_t)kSStepBuf * Mp, &wb.sbuf))) return rc;
     }
     const int64_t NMp = (int64_t)N * Mp;
-    auto stored = [&](int64_t n, int o) {
+    auto stored = [&](int64_t n, int o) { // o = 0: psi_n's singlet difference ([Z][Mp]); o > 0: a_n(s_o)
         const int64_t slot = n >= kstart ? nbuf * L + (n - kstart) : ((n / L) % nbuf) * L + n % L;
-        return store + (slot * (kGL + 1) + o) * NMp;
+        return store + slot * (NMp / 4 + kGL * NMp) + (o == 0 ? 0 : NMp / 4 + (int64_t)(o - 1) * NMp);
     };
     TauSet tf{}, tb{};
     tf.n = kGL + 2; // psi_n, a(s_j), psi_{n+1}
@@ -3100,6 +3103,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
             CombineOut out{};
             out.n = kGL + 1;
             for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
+            out.diff0 = true;
             const double2 *Cu = nullptr, *q0 = nullptr;
             HIP_TRY(fold_sets(w, Cb, out.n, &Cu, &q0, s));
             HIP_TRY(combine(w.Q, qstride(w), m, Cu, out, nullptr, 0.0, Z, (int)M, Mp, s, q0));
@@ -3139,6 +3143,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
         out.n = kGL + 1;
         out.y[0] = lam;
         for (int j = 0; j < kGL; ++j) out.y[1 + j] = bvec + j * NMp;
+        out.src_diff = true; // stored(n, 0) is psi_n's singlet difference
         const double2 *Cu = nullptr, *q0 = nullptr;
         HIP_TRY(fold_sets(wb, Cb, out.n, &Cu, &q0, s));
         HIP_TRY(combine(wb.Q, qstride(wb), m, Cu, out, stored(n, 0), cw[n] * wscale, Z, (int)M, Mp, s, q0));
@@ -3159,6 +3164,7 @@ static int trace_sample_grad(spc_ctx *ctx, const spc_sim_params *sim, const int6
             CombineOut out{};
             out.n = kGL + 2;
             for (int o = 0; o <= kGL; ++o) out.y[o] = stored(n, o);
+            out.diff0 = true;
             out.y[kGL + 1] = w.psi;
             const double2 *Cu = nullptr, *q0 = nullptr;
             HIP_TRY(fold_sets(w, Cf, out.n, &Cu, &q0, sr));

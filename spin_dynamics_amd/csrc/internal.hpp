@@ -299,11 +299,15 @@ hipError_t fold_sets(KrylovWork &w, const double2 *C, int nout, const double2 **
 // Up to kCombineOut outputs y_o = sum_k C[o][k] q_k (+ omega P_S src for o = 0)
 // from one read of the basis Q ([m][N][Mp]; q_0 from q0 when given, which may
 // be one of the outputs); outputs with y[o] == nullptr skipped; lanes r >= M
-// written as zeros.
+// written as zeros.  diff0: output 0 is stored as its singlet difference
+// y[4n+1] - y[4n+2] only ([Z][Mp]).  src_diff: src holds such a difference
+// (the singlet source term then reads it instead of src's two rows).
 constexpr int kCombineOut = 6;
 struct CombineOut {
     double2 *y[kCombineOut];
     int n;
+    bool diff0 = false;
+    bool src_diff = false;
 };
 hipError_t combine(const double2 *Q, int64_t qs, int m, const double2 *C, const CombineOut &out, const double2 *src,
                    double omega, int64_t Z, int M, int Mp, hipStream_t s, const double2 *q0 = nullptr);
