@@ -11,7 +11,9 @@ passes, the largest order); the monomial blocks (SPC_NEWTON=0) against the
 same; every lane chunk split; a random-phase and an injected start at
 N = 36 864 against the oracle over two steps (test_gpu_sstep); the equal-rate
 invariant (exact decay of every norm, 1e-12); the batched orientations and
-the gradient's forward steps against their column runs."""
+the gradient's forward steps against their column runs; the 64-lane chain
+product's compile-time electron layouts of the in-tile nuclei (every count
+of electron-2 nuclei among them) against the oracle."""
 import copy
 
 import numpy as np
@@ -83,3 +85,18 @@ def test_newton_block_batched_orientations_match_the_column_path(ctx, monkeypatc
     monkeypatch.setenv("SPC_SSTEP", "0")
     phi_c = batch.orientation_yields(p.sys, sim, dirs, B=0.05, ctx=ctx)
     assert max_rel(phi, phi_c) < 1e-11
+
+
+@pytest.mark.parametrize("n2", [0, 1, 2, 3, 4, 5, 6])
+def test_chain_product_with_every_electron_split_of_the_tile_nuclei(ctx, n2):
+    # seven spin-1/2 nuclei (N = 512), n2 of them (the last ones) on electron 2:
+    # a 64-lane chunk takes tiles of 32 nuclear indices, whose 5 in-tile nuclei
+    # then hold min(n2, 5) electron-2 nuclei; the chain product's compile-time
+    # electron layout for each count against the oracle over five steps
+    w = workloads.small_anisotropic(m1=(2,) * (7 - n2), m2=(2,) * n2, M=64, m=16, T_ns=4.5)
+    res = dynamics.trace_sampling_para(w.sys, w.sim, B=w.B, ctx=ctx)
+    s = oracle_system(w.sys, w.B)
+    ref = pyoracle.trace_sampling(pyoracle.build_general_csr(s), s.Z, M=64, m=16, t_end=w.sim.t_end, dt=w.sim.dt,
+                                  vec_kind=1, seed=w.sim.seed)
+    for k in KEYS:
+        assert max_rel(getattr(res, k), ref[k]) < 1e-10, k
