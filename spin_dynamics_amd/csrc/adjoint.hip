@@ -110,7 +110,10 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
                                                           int J, int64_t NMp, const DevSystem *__restrict__ sys, int M,
                                                           int Mp, double2 *__restrict__ S)
 {
-    __shared__ double red[kBlock / kWave][16][kWave];
+    // the lane sums' transpose: lane l's value d at red[d][17 (l / 16) + l % 16] (rows of 68, runs of 16 padded
+    // to 17), so that neither the writes nor the reads of 16 consecutive values by 4 lanes per row share a bank
+    // (unpadded, the 16 rows d lay 512 bytes apart, on the same banks)
+    __shared__ double red[kBlock / kWave][16][4 * 17];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
     const int64_t Z = sys->Z;
     const int64_t n = (int64_t)blockIdx.x * (kBlock / kWave) + wave;
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
 #pragma unroll
             for (int d = 0; d < 16; ++d) {
                 const int q = (round * 16 + d) >> 1;
-                red[wave][d][lane] = ((round * 16 + d) & 1) ? P[q].y : P[q].x;
+                red[wave][d][17 * (lane >> 4) + (lane & 15)] = ((round * 16 + d) & 1) ? P[q].y : P[q].x;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_density(const double2 *__restri
             const int d = lane >> 2, h = lane & 3;
             double sum = 0.0;
 #pragma unroll
-            for (int t = 0; t < 16; ++t) sum += red[wave][d][16 * h + t];
+            for (int t = 0; t < 16; ++t) sum += red[wave][d][17 * h + t];
             const double o1 = __shfl_xor(sum, 1);
             const double s01 = (h & 1) ? o1 + sum : sum + o1;
             const double o2 = __shfl_xor(s01, 2);
